@@ -1,0 +1,346 @@
+"""bootcmatchg_amd - MI355X (gfx950) AMG-preconditioned flexible CG.
+
+Python mirror of BootCMatchG's solve path over the C-ABI in include/bcmg.h
+(libbcmg.so, hand-written HIP kernels). The names follow the reference's
+facade (src/BootCMatchG.h): a settings record read by `params_from_file`
+(AMG::Params::initFromFile, src/AMG/AMG.cu:480-526) and `solve(A, p, rhs)`
+(src/SOLVER/solver.cu:20-129). Errors raise BcmgError carrying the C status
+code instead of terminating the process as the reference does.
+
+There is no CPU fallback: if libbcmg.so is missing or no gfx950 device is
+present, the calls that need the GPU raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbcmg.so")
+
+OK = 0
+EINVAL = -1
+ESETTINGS = -2
+EZERORHS = -3
+EUNSUP = -4
+EBREAKDOWN = -5
+EHIP = -6
+ENOMEM = -7
+ENODEV = -8
+
+_i32p = ctypes.POINTER(ctypes.c_int)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+class BcmgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("bcmg error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """bcmg_params: field order of the reference's struct params
+    (src/AMG/AMG.cu:20-37)."""
+    _fields_ = [
+        ("solver_type", ctypes.c_int),
+        ("max_hrc", ctypes.c_int),
+        ("conv_ratio", ctypes.c_double),
+        ("matchtype", ctypes.c_int),
+        ("aggrsweeps", ctypes.c_int),
+        ("aggrtype", ctypes.c_int),
+        ("max_levels", ctypes.c_int),
+        ("cycle_type", ctypes.c_int),
+        ("coarse_solver", ctypes.c_int),
+        ("relax_type", ctypes.c_int),
+        ("prerelax_sweeps", ctypes.c_int),
+        ("postrelax_sweeps", ctypes.c_int),
+        ("itnlim", ctypes.c_int),
+        ("rtol", ctypes.c_double),
+        ("relaxnumber_coarse", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def make_params(solver_type=0, max_hrc=1, conv_ratio=0.8, aggrsweeps=2, aggrtype=0,
+                max_levels=39, cycle_type=0, coarse_solver=4, relax_type=4,
+                relaxnumber_coarse=16, prerelax_sweeps=4, postrelax_sweeps=4,
+                itnlim=2000, rtol=1e-6):
+    """Settings in the order of the 14-line AMGsettings file."""
+    return Params(solver_type, max_hrc, conv_ratio, 3, aggrsweeps, aggrtype, max_levels,
+                  cycle_type, coarse_solver, relax_type, prerelax_sweeps, postrelax_sweeps,
+                  itnlim, rtol, relaxnumber_coarse)
+
+
+_lib = None
+
+
+def lib():
+    """Load libbcmg.so (built by __graft_entry__.build or `make -C csrc`)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BcmgError(ENODEV, "libbcmg.so is not built (%s)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.bcmg_last_error.restype = ctypes.c_char_p
+        L.bcmg_hier_levels.restype = ctypes.c_int
+        for name in ("bcmg_ctx_create", "bcmg_setup", "bcmg_csr_upload", "bcmg_galerkin",
+                     "bcmg_dmalloc"):
+            getattr(L, name).restype = ctypes.c_int
+        L.bcmg_dmalloc.argtypes = [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]
+        L.bcmg_dfree.argtypes = [_vp, _vp]
+        L.bcmg_memcpy_h2d.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
+        L.bcmg_memcpy_d2h.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
+        L.bcmg_memcpy_d2d.argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != OK:
+        raise BcmgError(rc, lib().bcmg_last_error().decode(errors="replace"))
+    return rc
+
+
+def params_from_file(path):
+    """AMG::Params::initFromFile (src/AMG/AMG.cu:480-526)."""
+    p = Params()
+    _check(lib().bcmg_params_from_file(os.fsencode(path), ctypes.byref(p)))
+    return p
+
+
+def _as(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Context:
+    """One GPU and one HIP stream (replaces Handles, src/utility/handles.cu)."""
+
+    def __init__(self, device=0):
+        self.h = _vp()
+        _check(lib().bcmg_ctx_create(device, ctypes.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bcmg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def sync(self):
+        _check(lib().bcmg_sync(self.h))
+
+
+class DeviceArray:
+    """A device buffer of a numpy dtype owned by a Context."""
+
+    def __init__(self, ctx, n, dtype=np.float64):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.ptr = _vp()
+        _check(lib().bcmg_dmalloc(ctx.h, max(self.n, 1) * self.dtype.itemsize,
+                                  ctypes.byref(self.ptr)))
+
+    @classmethod
+    def from_numpy(cls, ctx, a):
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = _as(a, self.dtype)
+        assert a.size == self.n
+        _check(lib().bcmg_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        _check(lib().bcmg_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            lib().bcmg_dfree(self.ctx.h, self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Csr:
+    """Device CSR (CSRm::copyToDevice, src/matrix/CSR.cu:194-217)."""
+
+    def __init__(self, ctx, A=None, handle=None):
+        self.ctx = ctx
+        self.h = _vp()
+        if handle is not None:
+            self.h = handle
+        else:
+            row, col, val = A
+            row, col, val = _as(row, np.int32), _as(col, np.int32), _as(val, np.float64)
+            n = len(row) - 1
+            _check(lib().bcmg_csr_upload(ctx.h, n, len(col), row.ctypes.data_as(_i32p),
+                                         col.ctypes.data_as(_i32p), val.ctypes.data_as(_f64p),
+                                         ctypes.byref(self.h)))
+        n = ctypes.c_int()
+        nnz = ctypes.c_int()
+        _check(lib().bcmg_csr_info(self.h, ctypes.byref(n), ctypes.byref(nnz)))
+        self.n, self.nnz = n.value, nnz.value
+
+    def download(self):
+        row = np.empty(self.n + 1, np.int32)
+        col = np.empty(self.nnz, np.int32)
+        val = np.empty(self.nnz)
+        _check(lib().bcmg_csr_download(self.h, row.ctypes.data_as(_i32p),
+                                       col.ctypes.data_as(_i32p), val.ctypes.data_as(_f64p)))
+        return row, col, val
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().bcmg_csr_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.free()
+
+
+class Boot:
+    """A built composite of AMG hierarchies (Bootstrap::bootstrap,
+    src/BOOT/bootstrap.cu:46-90)."""
+
+    def __init__(self, ctx, A, prm):
+        self.ctx = ctx
+        self.A = A
+        self.h = _vp()
+        _check(lib().bcmg_setup(ctx.h, A.h, ctypes.byref(prm), ctypes.byref(self.h)))
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().bcmg_boot_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.free()
+
+    def info(self):
+        nh = ctypes.c_int()
+        er = ctypes.c_double()
+        _check(lib().bcmg_boot_info(self.h, ctypes.byref(nh), ctypes.byref(er)))
+        return nh.value, er.value
+
+    def levels(self, k=0):
+        r = lib().bcmg_hier_levels(self.h, k)
+        if r < 0:
+            _check(r)
+        return r
+
+    def level_size(self, k, l):
+        n = ctypes.c_int()
+        nnz = ctypes.c_int()
+        _check(lib().bcmg_level_size(self.h, k, l, ctypes.byref(n), ctypes.byref(nnz)))
+        return n.value, nnz.value
+
+    def level_A(self, k, l):
+        n, nnz = self.level_size(k, l)
+        row = np.empty(n + 1, np.int32)
+        col = np.empty(nnz, np.int32)
+        val = np.empty(nnz)
+        _check(lib().bcmg_level_A(self.h, k, l, row.ctypes.data_as(_i32p),
+                                  col.ctypes.data_as(_i32p), val.ctypes.data_as(_f64p)))
+        return row, col, val
+
+    def level_P(self, k, l):
+        n, _ = self.level_size(k, l)
+        agg = np.empty(n, np.int32)
+        p = np.empty(n)
+        _check(lib().bcmg_level_P(self.h, k, l, agg.ctypes.data_as(_i32p),
+                                  p.ctypes.data_as(_f64p)))
+        return agg, p
+
+    def level_DM(self, k, l):
+        n, _ = self.level_size(k, l)
+        D = np.empty(n)
+        M = np.empty(n)
+        hd = ctypes.c_int()
+        hm = ctypes.c_int()
+        _check(lib().bcmg_level_DM(self.h, k, l, D.ctypes.data_as(_f64p),
+                                   M.ctypes.data_as(_f64p), ctypes.byref(hd), ctypes.byref(hm)))
+        return (D if hd.value else None), (M if hm.value else None)
+
+    def fcg(self, b_dev, x_dev):
+        """FCG v3 from x0 = 0 on device arrays; returns (iterations, residual)."""
+        it = ctypes.c_int()
+        res = ctypes.c_double()
+        _check(lib().bcmg_fcg(self.ctx.h, self.h, b_dev.ptr, x_dev.ptr, ctypes.byref(it),
+                              ctypes.byref(res)))
+        return it.value, res.value
+
+    def precond(self, r_dev, x_dev):
+        _check(lib().bcmg_precond_apply(self.ctx.h, self.h, r_dev.ptr, x_dev.ptr))
+
+
+def solve(A, p, rhs):
+    """solve() of the reference (src/SOLVER/solver.cu:20-129) on host arrays:
+    setup + FCG from x0 = 0. Returns (x, info) with info = dict(iterations,
+    residual, setup_ms, solve_ms)."""
+    row, col, val = A
+    row, col, val = _as(row, np.int32), _as(col, np.int32), _as(val, np.float64)
+    n = len(row) - 1
+    rhs = _as(rhs, np.float64)
+    x = np.empty(n)
+    it = ctypes.c_int()
+    res = ctypes.c_double()
+    tsu = ctypes.c_double()
+    tso = ctypes.c_double()
+    _check(lib().bcmg_solve(n, len(col), row.ctypes.data_as(_i32p), col.ctypes.data_as(_i32p),
+                            val.ctypes.data_as(_f64p), ctypes.byref(p),
+                            rhs.ctypes.data_as(_f64p), x.ctypes.data_as(_f64p),
+                            ctypes.byref(it), ctypes.byref(res), ctypes.byref(tsu),
+                            ctypes.byref(tso)))
+    return x, dict(iterations=it.value, residual=res.value, setup_ms=tsu.value,
+                   solve_ms=tso.value)
+
+
+# --------------------------------------------------------- building blocks
+
+def spmv(ctx, A, x_dev, y_dev):
+    _check(lib().bcmg_spmv(ctx.h, A.h, x_dev.ptr, y_dev.ptr))
+
+
+def residual(ctx, A, x_dev, f_dev, y_dev):
+    _check(lib().bcmg_residual(ctx.h, A.h, x_dev.ptr, f_dev.ptr, y_dev.ptr))
+
+
+def relax(ctx, A, d_dev, f_dev, u_dev, u2_dev, sweeps):
+    _check(lib().bcmg_relax(ctx.h, A.h, d_dev.ptr, f_dev.ptr, u_dev.ptr, u2_dev.ptr, sweeps))
+
+
+def weights(ctx, A, w_dev, W_dev):
+    _check(lib().bcmg_weights(ctx.h, A.h, w_dev.ptr, W_dev.ptr))
+
+
+def matching(ctx, A, W_dev, Mraw_dev, Msym_dev):
+    nu = ctypes.c_int()
+    _check(lib().bcmg_matching(ctx.h, A.h, W_dev.ptr, Mraw_dev.ptr, Msym_dev.ptr,
+                               ctypes.byref(nu)))
+    return nu.value
+
+
+def aggregate(ctx, n, Msym_dev, w_dev, agg_dev, p_dev):
+    nc = ctypes.c_int()
+    _check(lib().bcmg_aggregate(ctx.h, n, Msym_dev.ptr, w_dev.ptr, agg_dev.ptr, p_dev.ptr,
+                                ctypes.byref(nc)))
+    return nc.value
+
+
+def galerkin(ctx, A, agg_dev, p_dev, nc):
+    h = _vp()
+    _check(lib().bcmg_galerkin(ctx.h, A.h, agg_dev.ptr, p_dev.ptr, nc, ctypes.byref(h)))
+    return Csr(ctx, handle=h)

@@ -1,0 +1,205 @@
+/*
+ * bcmg_internal.h - internal types of the gfx950 AMG-FCG library.
+ *
+ * Host code (setup orchestration, cycles, FCG) and the kernel launch wrappers
+ * share these declarations. Kernels live in kernels_*.hip; each launch wrapper
+ * takes the stream explicitly. Everything here stays behind the C-ABI of
+ * include/bcmg.h: no exceptions cross it, no torch types appear in it.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bcmg.h"
+
+namespace bcmg {
+
+/* ------------------------------------------------------------ errors */
+
+struct Error : public std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+
+#define BCMG_HIP(call)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            throw ::bcmg::Error(BCMG_EHIP, std::string(#call) + ": " +              \
+                                               hipGetErrorString(e_) + " @" +       \
+                                               __FILE__ + ":" + std::to_string(__LINE__)); \
+    } while (0)
+
+#define BCMG_CHECK_LAUNCH() BCMG_HIP(hipGetLastError())
+
+/* ------------------------------------------------------- device buffer */
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        BCMG_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+        n = count;
+    }
+    void ensure(size_t count) { if (count > n || !p) alloc(count); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T *get() const { return p; }
+};
+
+/* -------------------------------------------------------------- matrix */
+
+/* CSR, 0-based, int32 indices, FP64 values; columns ascending with a diagonal
+   entry in every row (the layout of src/matrix/CSR.h:13-28). */
+struct DevCsr {
+    int n = 0, m = 0, nnz = 0;
+    DBuf<int> row, col;
+    DBuf<double> val;
+    int group = 8;          /* lanes per row of the row-group kernels */
+};
+
+/* Lanes per row for an average row length (a power of two in [1, 64]). */
+int lanes_for(int n, long long nnz);
+
+/* ------------------------------------------------------------- context */
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    /* pinned host scalars used for the few device->host reads */
+    double *h_scalars = nullptr;
+    int *h_ints = nullptr;
+    /* reduction scratch */
+    DBuf<double> red;           /* partial sums (per block) + results */
+    DBuf<int> iscratch;
+    /* timing of the dominant kernel (optional) */
+    bool time_smoother = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    hipEvent_t next_event();
+    ~Ctx();
+};
+
+/* ---------------------------------------------------------- hierarchy */
+
+struct Level {
+    DevCsr *Aext = nullptr;     /* level 0: the caller's matrix */
+    DevCsr Aown;                /* coarse levels: owned */
+    const DevCsr &mat() const { return Aext ? *Aext : Aown; }
+    DBuf<double> D, M;          /* diag(A), sum_j |a_ij| (may be empty) */
+    bool hasD = false, hasM = false;
+    /* prolongator to the next level: one entry per fine row */
+    DBuf<int> agg;
+    DBuf<double> p;
+    /* R = P^T as CSR (n_next rows) */
+    DevCsr R;
+};
+
+struct Hierarchy {
+    std::vector<Level> L;
+    double op_cmplx = 0, op_wcmplx = 0, avg_cratio = 0;
+    int num_levels() const { return (int)L.size(); }
+};
+
+struct Boot {
+    bcmg_params prm{};
+    int coarse_solver = 4;
+    std::vector<int> num_grid_sweeps;
+    std::vector<Hierarchy> H;
+    double estimated_ratio = 1.0;
+    DevCsr *A0 = nullptr;       /* fine operator (owned by the caller's handle) */
+    const double *f0 = nullptr; /* level-0 right-hand side of the running cycle */
+    /* cycle work vectors, per level, sized to the max over hierarchies */
+    std::vector<DBuf<double>> xa, xb, f;
+    DBuf<double> res;           /* residual buffer, size n0 */
+    DBuf<double> xsave;         /* additive composite: input x */
+    DBuf<double> xadd;
+    /* FCG vectors */
+    DBuf<double> v, w, d, q, r;
+    DBuf<double> scal;          /* FCG scalars on device */
+    /* setup timing (ms) */
+    double t_setup_ms = 0;
+    void ensure_work();
+};
+
+/* ----------------------------------------------------- launch wrappers */
+/* solve-phase kernels (kernels_solve.hip) */
+enum RowOp { OP_SPMV = 0, OP_RESID = 1, OP_JACOBI = 2 };
+void launch_rows(hipStream_t s, const DevCsr &A, RowOp op, const double *x, const double *f,
+                 const double *d, double *y);
+void launch_jacobi_zero(hipStream_t s, int n, const double *f, const double *d, double *y);
+void launch_prolong(hipStream_t s, int n, const int *agg, const double *p, const double *xc,
+                    double *x);
+void launch_fill(hipStream_t s, int n, double v, double *x);
+void launch_copy(hipStream_t s, int n, const double *a, double *b);
+/* y = alpha x + y */
+void launch_axpy(hipStream_t s, int n, double alpha, const double *x, double *y);
+void launch_scale(hipStream_t s, int n, double alpha, double *x);
+/* deterministic dot products: up to 3 dots of (a_k, b) ; results to out[k] (device) */
+void launch_dots(Ctx &c, int n, int k, const double *a0, const double *a1, const double *a2,
+                 const double *b, double *out);
+/* FCG v3 scalar recurrences on device; s = [alpha, beta, gamma, delta, theta, theta2, l2, ...] */
+void launch_fcg_scalars(hipStream_t s, double *scal);
+/* z = -theta zo + z; x += theta2 z; t = -theta to + t; r += -theta2 t; scal[6] = ||r|| */
+void launch_fcg_update(Ctx &c, int n, double *scal, const double *zo, double *z, double *x,
+                       const double *to, double *t, double *r);
+/* first FCG step: x += theta v; r -= theta w; ||r|| */
+void launch_fcg_first(Ctx &c, int n, double *scal, const double *v, double *x,
+                      const double *w, double *r);
+
+/* setup-phase kernels (kernels_setup.hip) */
+void launch_diag(hipStream_t s, const DevCsr &A, double *D);
+void launch_abs_rowsum(hipStream_t s, const DevCsr &A, double *M);
+/* W aligned with A's nnz (diag slots 0) */
+void match_weights(Ctx &c, const DevCsr &A, const double *w, double *W);
+/* suitor + symmetrisation: Msym (and M raw); returns number of non-mutual v */
+int match_suitor(Ctx &c, const DevCsr &A, const double *W, int *M, int *Msym);
+/* canonical pairwise aggregation; returns n_c. members[2*I], [2*I+1] (or -1) */
+int aggregate_pairs(Ctx &c, int n, const int *Msym, const double *w, int *agg, double *p,
+                    int *members);
+/* A_c = R (A P) in the fixed order, pairwise P given by (agg, p, members) */
+void galerkin_pairwise(Ctx &c, const DevCsr &A, const int *agg, const double *p,
+                       const int *members, int nc, DevCsr &Ac);
+/* w_c(I) = 0 + sum_{members} p_i w_i */
+void restrict_w(hipStream_t s, int nc, const int *members, const double *p, const double *w,
+                double *wc);
+/* composed P: aggP[i] = agg[aggP[i]], pP[i] = 0 + pP[i] * p[aggP_old[i]] */
+void compose_P(hipStream_t s, int n0, int *aggP, double *pP, const int *agg, const double *p);
+/* R = P^T (rows ascending in fine index) */
+void transpose_P(Ctx &c, int n, int nc, const int *agg, const double *p, DevCsr &R);
+/* members of a given pairwise aggregation (throws if not pairwise) */
+void members_from_agg(Ctx &c, int n, const int *agg, int nc, int *members);
+/* exclusive scan of int32 counts into out[0..n], out[n] = total; returns total */
+long long exclusive_scan(Ctx &c, const int *in, int *out, int n);
+
+/* ------------------------------------------------------------ host algo */
+void build_boot(Ctx &c, DevCsr &A, const bcmg_params &p, Boot &B);
+void precond_apply(Ctx &c, Boot &B, const double *r, double *x, bool x_is_zero);
+int fcg_solve(Ctx &c, Boot &B, const double *b, double *x, int *iters, double *resid,
+              double *delta0_out);
+double a_norm(Ctx &c, const DevCsr &A, const double *x, double *tmp);
+
+}  // namespace bcmg

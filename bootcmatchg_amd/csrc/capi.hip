@@ -1,0 +1,609 @@
+/*
+ * capi.hip - the extern "C" entry points of include/bcmg.h.
+ *
+ * Every entry point catches all C++ exceptions and returns a status code;
+ * the message is kept per thread for bcmg_last_error().
+ */
+#include "bcmg_internal.h"
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+struct bcmg_ctx_s {
+    bcmg::Ctx c;
+};
+
+struct bcmg_csr_s {
+    bcmg_ctx ctx = nullptr;
+    bcmg::DevCsr A;
+};
+
+struct bcmg_boot_s {
+    bcmg_ctx ctx = nullptr;
+    bcmg_csr A = nullptr;
+    bcmg::Boot B;
+};
+
+namespace bcmg {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+} // namespace bcmg
+
+using namespace bcmg;
+
+#define API_BEGIN try {
+#define API_END                                                  \
+    }                                                            \
+    catch (const bcmg::Error &e) {                               \
+        set_last_error(e.what());                                \
+        return e.code;                                           \
+    }                                                            \
+    catch (const std::bad_alloc &) {                             \
+        set_last_error("host allocation failed");                \
+        return BCMG_ENOMEM;                                      \
+    }                                                            \
+    catch (const std::exception &e) {                            \
+        set_last_error(e.what());                                \
+        return BCMG_EINVAL;                                      \
+    }                                                            \
+    return BCMG_OK;
+
+#define REQUIRE(cond, msg)                                       \
+    do {                                                         \
+        if (!(cond)) throw bcmg::Error(BCMG_EINVAL, msg);        \
+    } while (0)
+
+namespace {
+
+void use_device(bcmg_ctx ctx) { BCMG_HIP(hipSetDevice(ctx->c.device)); }
+
+void check_csr_host(int n, int nnz, const int *row, const int *col)
+{
+    REQUIRE(n > 0 && nnz >= n && row && col, "CSR: bad sizes or null pointers");
+    REQUIRE(row[0] == 0 && row[n] == nnz, "CSR: row pointer must start at 0 and end at nnz");
+    for (int i = 0; i < n; i++) {
+        REQUIRE(row[i + 1] >= row[i], "CSR: row pointer not monotone");
+        bool diag = false;
+        for (int j = row[i]; j < row[i + 1]; j++) {
+            REQUIRE(col[j] >= 0 && col[j] < n, "CSR: column index out of range");
+            REQUIRE(j == row[i] || col[j] > col[j - 1], "CSR: columns must be strictly ascending");
+            diag = diag || col[j] == i;
+        }
+        REQUIRE(diag, "CSR: every row needs a diagonal entry");
+    }
+}
+
+int params_check(const bcmg_params &p)
+{
+    if (p.solver_type < 0 || p.solver_type > 2) return BCMG_ESETTINGS;
+    if (p.aggrtype != 0 && p.aggrtype != 1) return BCMG_ESETTINGS;
+    if (p.cycle_type < 0 || p.cycle_type > 2) return BCMG_ESETTINGS;
+    if (p.coarse_solver != 0 && p.coarse_solver != 4) return BCMG_ESETTINGS;
+    if (p.relax_type != 0 && p.relax_type != 4) return BCMG_ESETTINGS;
+    if (p.aggrtype == 1) return BCMG_EUNSUP;
+    if (p.max_hrc < 1 || p.aggrsweeps < 1 || p.max_levels < 1 || p.itnlim < 1)
+        return BCMG_EINVAL;
+    if (p.prerelax_sweeps < 0 || p.postrelax_sweeps < 0 || p.relaxnumber_coarse < 0)
+        return BCMG_EINVAL;
+    return BCMG_OK;
+}
+
+const Level &get_level(bcmg_boot B, int k, int l)
+{
+    REQUIRE(B, "null composite");
+    REQUIRE(k >= 0 && k < (int)B->B.H.size(), "hierarchy index out of range");
+    REQUIRE(l >= 0 && l < B->B.H[k].num_levels(), "level index out of range");
+    return B->B.H[k].L[l];
+}
+
+void d2h(bcmg_ctx ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes || !dst) return;
+    BCMG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+    BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
+}
+
+} // namespace
+
+extern "C" {
+
+const char *bcmg_last_error(void) { return g_last_error.c_str(); }
+
+/* src/AMG/AMG.cu:480-526 with the line parser of src/AMG/utility.cu:7-25:
+   one value per line, text after '%' ignored. */
+int bcmg_params_from_file(const char *path, bcmg_params *out)
+{
+    if (!path || !out) return BCMG_EINVAL;
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) {
+        set_last_error("Setting file not found!");
+        return BCMG_ESETTINGS;
+    }
+    char buf[1025];
+    bool ok = true;
+    auto token = [&]() -> const char * {
+        if (!std::fgets(buf, 1024, fp)) {
+            ok = false;
+            return nullptr;
+        }
+        char *save = nullptr;
+        const char *t = strtok_r(buf, "%", &save);
+        if (!t) ok = false;
+        return t;
+    };
+    auto gi = [&]() -> int {
+        int v = 0;
+        const char *t = token();
+        if (!t || std::sscanf(t, "%d", &v) != 1) ok = false;
+        return v;
+    };
+    auto gd = [&]() -> double {
+        double v = 0;
+        const char *t = token();
+        if (!t || std::sscanf(t, "%lf", &v) != 1) ok = false;
+        return v;
+    };
+    bcmg_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.solver_type = gi();
+    p.max_hrc = gi();
+    p.conv_ratio = gd();
+    p.matchtype = 3;
+    p.aggrsweeps = gi();
+    p.aggrtype = gi();
+    p.max_levels = gi();
+    p.cycle_type = gi();
+    p.coarse_solver = gi();
+    p.relax_type = gi();
+    p.relaxnumber_coarse = gi();
+    p.prerelax_sweeps = gi();
+    p.postrelax_sweeps = gi();
+    p.itnlim = gi();
+    p.rtol = gd();
+    std::fclose(fp);
+    if (!ok) {
+        set_last_error("settings file: expected 14 lines, one value each");
+        return BCMG_ESETTINGS;
+    }
+    const int rc = params_check(p);
+    if (rc == BCMG_ESETTINGS) {
+        set_last_error("settings file: value outside the allowed set");
+        return rc;
+    }
+    *out = p;
+    return BCMG_OK;
+}
+
+int bcmg_ctx_create(int device, bcmg_ctx *out)
+{
+    if (!out) return BCMG_EINVAL;
+    *out = nullptr;
+    API_BEGIN
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+        throw Error(BCMG_ENODEV, "no HIP device with that index");
+    hipDeviceProp_t prop;
+    BCMG_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        throw Error(BCMG_ENODEV, std::string("library is built for gfx950, device is ") +
+                                     prop.gcnArchName);
+    auto *ctx = new bcmg_ctx_s();
+    try {
+        ctx->c.device = device;
+        BCMG_HIP(hipSetDevice(device));
+        BCMG_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+        BCMG_HIP(hipHostMalloc((void **)&ctx->c.h_scalars, 64 * sizeof(double), hipHostMallocDefault));
+        BCMG_HIP(hipHostMalloc((void **)&ctx->c.h_ints, 64 * sizeof(int), hipHostMallocDefault));
+        ctx->c.red.alloc(1024 * 3 + 16);
+    } catch (...) {
+        delete ctx;
+        throw;
+    }
+    *out = ctx;
+    API_END
+}
+
+int bcmg_ctx_destroy(bcmg_ctx ctx)
+{
+    if (!ctx) return BCMG_OK;
+    API_BEGIN
+    use_device(ctx);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    delete ctx;
+    API_END
+}
+
+int bcmg_sync(bcmg_ctx ctx)
+{
+    if (!ctx) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
+    API_END
+}
+
+int bcmg_dmalloc(bcmg_ctx ctx, size_t bytes, void **ptr)
+{
+    if (!ctx || !ptr) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    *ptr = nullptr;
+    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(BCMG_ENOMEM, "device allocation failed");
+    }
+    API_END
+}
+
+int bcmg_dfree(bcmg_ctx ctx, void *ptr)
+{
+    if (!ctx) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
+    if (ptr) BCMG_HIP(hipFree(ptr));
+    API_END
+}
+
+int bcmg_memcpy_h2d(bcmg_ctx ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !src))) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    if (bytes) {
+        BCMG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+        BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
+    }
+    API_END
+}
+
+int bcmg_memcpy_d2h(bcmg_ctx ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !src))) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    d2h(ctx, dst, src, bytes);
+    API_END
+}
+
+int bcmg_memcpy_d2d(bcmg_ctx ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !src))) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    if (bytes) {
+        BCMG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->c.stream));
+        BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
+    }
+    API_END
+}
+
+int bcmg_csr_upload(bcmg_ctx ctx, int n, int nnz, const int *row, const int *col,
+                    const double *val, bcmg_csr *out)
+{
+    if (!ctx || !out || !val) return BCMG_EINVAL;
+    *out = nullptr;
+    API_BEGIN
+    check_csr_host(n, nnz, row, col);
+    use_device(ctx);
+    auto *h = new bcmg_csr_s();
+    try {
+        h->ctx = ctx;
+        DevCsr &A = h->A;
+        A.n = A.m = n;
+        A.nnz = nnz;
+        A.row.alloc(n + 1);
+        A.col.alloc(nnz);
+        A.val.alloc(nnz);
+        hipStream_t s = ctx->c.stream;
+        BCMG_HIP(hipMemcpyAsync(A.row.get(), row, sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+        BCMG_HIP(hipMemcpyAsync(A.col.get(), col, sizeof(int) * nnz, hipMemcpyHostToDevice, s));
+        BCMG_HIP(hipMemcpyAsync(A.val.get(), val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+        BCMG_HIP(hipStreamSynchronize(s));
+        A.group = lanes_for(n, nnz);
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    *out = h;
+    API_END
+}
+
+int bcmg_csr_download(bcmg_csr A, int *row, int *col, double *val)
+{
+    if (!A) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(A->ctx);
+    d2h(A->ctx, row, A->A.row.get(), sizeof(int) * (A->A.n + 1));
+    d2h(A->ctx, col, A->A.col.get(), sizeof(int) * A->A.nnz);
+    d2h(A->ctx, val, A->A.val.get(), sizeof(double) * A->A.nnz);
+    API_END
+}
+
+int bcmg_csr_info(bcmg_csr A, int *n, int *nnz)
+{
+    if (!A) return BCMG_EINVAL;
+    if (n) *n = A->A.n;
+    if (nnz) *nnz = A->A.nnz;
+    return BCMG_OK;
+}
+
+int bcmg_csr_free(bcmg_csr A)
+{
+    if (!A) return BCMG_OK;
+    API_BEGIN
+    use_device(A->ctx);
+    BCMG_HIP(hipStreamSynchronize(A->ctx->c.stream));
+    delete A;
+    API_END
+}
+
+int bcmg_setup(bcmg_ctx ctx, bcmg_csr A, const bcmg_params *p, bcmg_boot *out)
+{
+    if (!ctx || !A || !p || !out) return BCMG_EINVAL;
+    *out = nullptr;
+    const int rc = params_check(*p);
+    if (rc) {
+        set_last_error(rc == BCMG_EUNSUP ? "aggr_type 1 (smoothed aggregation) is not supported"
+                                         : "invalid settings");
+        return rc;
+    }
+    API_BEGIN
+    use_device(ctx);
+    auto *h = new bcmg_boot_s();
+    try {
+        h->ctx = ctx;
+        h->A = A;
+        build_boot(ctx->c, A->A, *p, h->B);
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    *out = h;
+    API_END
+}
+
+int bcmg_boot_free(bcmg_boot B)
+{
+    if (!B) return BCMG_OK;
+    API_BEGIN
+    use_device(B->ctx);
+    BCMG_HIP(hipStreamSynchronize(B->ctx->c.stream));
+    delete B;
+    API_END
+}
+
+int bcmg_fcg(bcmg_ctx ctx, bcmg_boot B, const double *b, double *x, int *iters, double *resid)
+{
+    if (!ctx || !B || !b || !x) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    fcg_solve(ctx->c, B->B, b, x, iters, resid, nullptr);
+    API_END
+}
+
+int bcmg_solve(int n, int nnz, const int *row, const int *col, const double *val,
+               const bcmg_params *p, const double *rhs, double *x, int *iters, double *resid,
+               double *setup_ms, double *solve_ms)
+{
+    if (!p || !rhs || !x) return BCMG_EINVAL;
+    bcmg_ctx ctx = nullptr;
+    bcmg_csr A = nullptr;
+    bcmg_boot B = nullptr;
+    double *db = nullptr, *dx = nullptr;
+    int rc = bcmg_ctx_create(0, &ctx);
+    if (rc) return rc;
+    auto cleanup = [&]() {
+        if (db) bcmg_dfree(ctx, db);
+        if (dx) bcmg_dfree(ctx, dx);
+        bcmg_boot_free(B);
+        bcmg_csr_free(A);
+        bcmg_ctx_destroy(ctx);
+    };
+    rc = bcmg_csr_upload(ctx, n, nnz, row, col, val, &A);
+    if (!rc) rc = bcmg_dmalloc(ctx, sizeof(double) * n, (void **)&db);
+    if (!rc) rc = bcmg_dmalloc(ctx, sizeof(double) * n, (void **)&dx);
+    if (!rc) rc = bcmg_memcpy_h2d(ctx, db, rhs, sizeof(double) * n);
+    auto t0 = std::chrono::steady_clock::now();
+    if (!rc) rc = bcmg_setup(ctx, A, p, &B);
+    auto t1 = std::chrono::steady_clock::now();
+    if (!rc) rc = bcmg_fcg(ctx, B, db, dx, iters, resid);
+    auto t2 = std::chrono::steady_clock::now();
+    if (!rc) rc = bcmg_memcpy_d2h(ctx, x, dx, sizeof(double) * n);
+    if (setup_ms) *setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    if (solve_ms) *solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    const std::string msg = g_last_error;
+    cleanup();
+    if (rc) g_last_error = msg;
+    return rc;
+}
+
+int bcmg_boot_info(bcmg_boot B, int *n_hrc, double *estimated_ratio)
+{
+    if (!B) return BCMG_EINVAL;
+    if (n_hrc) *n_hrc = (int)B->B.H.size();
+    if (estimated_ratio) *estimated_ratio = B->B.estimated_ratio;
+    return BCMG_OK;
+}
+
+int bcmg_hier_levels(bcmg_boot B, int k)
+{
+    if (!B || k < 0 || k >= (int)B->B.H.size()) return BCMG_EINVAL;
+    return B->B.H[k].num_levels();
+}
+
+int bcmg_level_size(bcmg_boot B, int k, int l, int *n, int *nnz)
+{
+    API_BEGIN
+    const Level &lv = get_level(B, k, l);
+    if (n) *n = lv.mat().n;
+    if (nnz) *nnz = lv.mat().nnz;
+    API_END
+}
+
+int bcmg_level_A(bcmg_boot B, int k, int l, int *row, int *col, double *val)
+{
+    API_BEGIN
+    const Level &lv = get_level(B, k, l);
+    use_device(B->ctx);
+    const DevCsr &A = lv.mat();
+    d2h(B->ctx, row, A.row.get(), sizeof(int) * (A.n + 1));
+    d2h(B->ctx, col, A.col.get(), sizeof(int) * A.nnz);
+    d2h(B->ctx, val, A.val.get(), sizeof(double) * A.nnz);
+    API_END
+}
+
+int bcmg_level_P(bcmg_boot B, int k, int l, int *agg, double *p)
+{
+    API_BEGIN
+    const Level &lv = get_level(B, k, l);
+    REQUIRE(l + 1 < B->B.H[k].num_levels(), "the coarsest level has no prolongator");
+    use_device(B->ctx);
+    d2h(B->ctx, agg, lv.agg.get(), sizeof(int) * lv.mat().n);
+    d2h(B->ctx, p, lv.p.get(), sizeof(double) * lv.mat().n);
+    API_END
+}
+
+int bcmg_level_DM(bcmg_boot B, int k, int l, double *D, double *M, int *hasD, int *hasM)
+{
+    API_BEGIN
+    const Level &lv = get_level(B, k, l);
+    use_device(B->ctx);
+    if (hasD) *hasD = lv.hasD;
+    if (hasM) *hasM = lv.hasM;
+    if (lv.hasD) d2h(B->ctx, D, lv.D.get(), sizeof(double) * lv.mat().n);
+    if (lv.hasM) d2h(B->ctx, M, lv.M.get(), sizeof(double) * lv.mat().n);
+    API_END
+}
+
+int bcmg_spmv(bcmg_ctx ctx, bcmg_csr A, const double *x, double *y)
+{
+    if (!ctx || !A || !x || !y) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    launch_rows(ctx->c.stream, A->A, OP_SPMV, x, nullptr, nullptr, y);
+    BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
+    API_END
+}
+
+int bcmg_residual(bcmg_ctx ctx, bcmg_csr A, const double *x, const double *f, double *y)
+{
+    if (!ctx || !A || !x || !f || !y) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    launch_rows(ctx->c.stream, A->A, OP_RESID, x, f, nullptr, y);
+    BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
+    API_END
+}
+
+int bcmg_relax(bcmg_ctx ctx, bcmg_csr A, const double *d, const double *f, double *u,
+               double *u2, int sweeps)
+{
+    if (!ctx || !A || !d || !f || !u || !u2 || sweeps < 0) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    double *x = u, *y = u2;
+    for (int k = 0; k < sweeps; k++) {
+        launch_rows(ctx->c.stream, A->A, OP_JACOBI, x, f, d, y);
+        std::swap(x, y);
+    }
+    if (x != u) launch_copy(ctx->c.stream, A->A.n, x, u);
+    BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
+    API_END
+}
+
+int bcmg_weights(bcmg_ctx ctx, bcmg_csr A, const double *w, double *W)
+{
+    if (!ctx || !A || !w || !W) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    match_weights(ctx->c, A->A, w, W);
+    API_END
+}
+
+int bcmg_matching(bcmg_ctx ctx, bcmg_csr A, const double *W, int *M_raw, int *M_sym,
+                  int *n_unsym)
+{
+    if (!ctx || !A || !W || !M_raw || !M_sym) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    const int nu = match_suitor(ctx->c, A->A, W, M_raw, M_sym);
+    BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
+    if (n_unsym) *n_unsym = nu;
+    API_END
+}
+
+int bcmg_aggregate(bcmg_ctx ctx, int n, const int *M_sym, const double *w, int *agg, double *p,
+                   int *n_coarse)
+{
+    if (!ctx || n <= 0 || !M_sym || !w || !agg || !p || !n_coarse) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    DBuf<int> members(2 * (size_t)n);
+    *n_coarse = aggregate_pairs(ctx->c, n, M_sym, w, agg, p, members.get());
+    API_END
+}
+
+int bcmg_galerkin(bcmg_ctx ctx, bcmg_csr A, const int *agg, const double *p, int n_coarse,
+                  bcmg_csr *Ac)
+{
+    if (!ctx || !A || !agg || !p || n_coarse <= 0 || !Ac) return BCMG_EINVAL;
+    *Ac = nullptr;
+    API_BEGIN
+    use_device(ctx);
+    DBuf<int> members(2 * (size_t)n_coarse);
+    members_from_agg(ctx->c, A->A.n, agg, n_coarse, members.get());
+    auto *h = new bcmg_csr_s();
+    try {
+        h->ctx = ctx;
+        galerkin_pairwise(ctx->c, A->A, agg, p, members.get(), n_coarse, h->A);
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    *Ac = h;
+    API_END
+}
+
+int bcmg_precond_apply(bcmg_ctx ctx, bcmg_boot B, const double *r, double *x)
+{
+    if (!ctx || !B || !r || !x) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    precond_apply(ctx->c, B->B, r, x, false);
+    BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
+    API_END
+}
+
+int bcmg_time_smoother(bcmg_ctx ctx, int enable)
+{
+    if (!ctx) return BCMG_EINVAL;
+    ctx->c.time_smoother = enable != 0;
+    ctx->c.ev_used = 0;
+    return BCMG_OK;
+}
+
+int bcmg_smoother_timing(bcmg_ctx ctx, double *total_ms, int *launches)
+{
+    if (!ctx) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
+    double tot = 0.0;
+    for (size_t i = 0; i + 1 < ctx->c.ev_used; i += 2) {
+        float ms = 0.f;
+        BCMG_HIP(hipEventElapsedTime(&ms, ctx->c.ev_pool[i], ctx->c.ev_pool[i + 1]));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = (int)(ctx->c.ev_used / 2);
+    ctx->c.ev_used = 0;
+    API_END
+}
+
+} // extern "C"

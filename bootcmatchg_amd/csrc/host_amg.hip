@@ -1,0 +1,461 @@
+/*
+ * host_amg.hip - host orchestration of the AMG setup (bootstrap, coarsening)
+ * and of the solve (V/H/W cycles, composite preconditioner, FCG v3).
+ *
+ * All device work is ordered on the context stream. The setup synchronises
+ * only where a size is needed to allocate (coarse row count, coarse nnz);
+ * the solve reads one scalar per FCG iteration (the residual norm) for the
+ * stopping test, as the reference does (src/SOLVER/FCG.cu:558-565).
+ */
+#include "bcmg_internal.h"
+
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+
+namespace bcmg {
+
+namespace {
+
+/* src/AMG/AMG.cu:371-394 */
+std::vector<int> grid_sweeps(int cycle_type, int max_levels)
+{
+    const int ml = max_levels - 1;
+    std::vector<int> g(ml > 0 ? ml : 1, 1);
+    if (cycle_type == 1) {
+        for (int i = 0; i < ml; i++)
+            if (i % 2 == 0) g[i] = 2;
+    } else if (cycle_type == 2) {
+        for (int i = 0; i < ml - 1; i++) g[i] = 2;
+    }
+    return g;
+}
+
+struct BuildState {
+    int maxcoarsesize = 0;
+    int ftcoarse = 1;
+};
+
+/* src/AMG/matchingAggregation.cu:18-60 */
+void relax_prepare(Ctx &c, Level &lv, const DevCsr &A, int type)
+{
+    if (type == 0 || type == 4) {
+        lv.D.ensure(A.n);
+        launch_diag(c.stream, A, lv.D.get());
+        lv.hasD = true;
+    }
+    if (type == 4) {
+        lv.M.ensure(A.n);
+        launch_abs_rowsum(c.stream, A, lv.M.get());
+        lv.hasM = true;
+    }
+}
+
+/* aggrsweeps pairwise sweeps (src/AMG/matchingAggregation.cu:211-295):
+   A_{i+1} = R_i (A_i P_i), w_{i+1} = R_i w_i, P = P P_i. */
+void matching_aggregation(Ctx &c, const bcmg_params &p, BuildState &bs, const DevCsr &A0,
+                          DBuf<double> &w, DBuf<int> &aggP, DBuf<double> &pP, DevCsr &Aout)
+{
+    const int n0 = A0.n;
+    const DevCsr *Ai = &A0;
+    DevCsr cur;
+    DBuf<double> wi;
+    const double *wptr = w.get();
+    for (int s = 0; s < p.aggrsweeps; s++) {
+        const int n = Ai->n;
+        DBuf<double> W(Ai->nnz > 0 ? Ai->nnz : 1);
+        match_weights(c, *Ai, wptr, W.get());
+        DBuf<int> M(n), Ms(n);
+        match_suitor(c, *Ai, W.get(), M.get(), Ms.get());
+        W.release();
+        DBuf<int> agg(n), members(2 * (size_t)n);
+        DBuf<double> pv(n);
+        const int nc = aggregate_pairs(c, n, Ms.get(), wptr, agg.get(), pv.get(), members.get());
+        DevCsr Ac;
+        galerkin_pairwise(c, *Ai, agg.get(), pv.get(), members.get(), nc, Ac);
+        DBuf<double> wc(nc);
+        restrict_w(c.stream, nc, members.get(), pv.get(), wptr, wc.get());
+        if (s == 0) {
+            aggP = std::move(agg);
+            pP = std::move(pv);
+        } else {
+            compose_P(c.stream, n0, aggP.get(), pP.get(), agg.get(), pv.get());
+        }
+        BCMG_HIP(hipStreamSynchronize(c.stream));
+        const double coarse_ratio = (double)n / (double)nc;
+        cur = std::move(Ac);
+        Ai = &cur;
+        wi = std::move(wc);
+        wptr = wi.get();
+        if (coarse_ratio <= 1.2) bs.ftcoarse = 100;
+        if ((double)nc <= (double)bs.ftcoarse * bs.maxcoarsesize) break;
+    }
+    w = std::move(wi);
+    Aout = std::move(cur);
+}
+
+/* src/AMG/matchingAggregation.cu:298-423 */
+void adaptive_coarsening(Ctx &c, const bcmg_params &p, BuildState &bs, DevCsr &A,
+                         const double *w_in, Hierarchy &H)
+{
+    const int n = A.n;
+    DBuf<double> w(n);
+    launch_copy(c.stream, n, w_in, w.get());
+    H.L.clear();
+    H.L.emplace_back();
+    H.L[0].Aext = &A;
+    DBuf<double> tmp(n);
+    const double normw = a_norm(c, A, w.get(), tmp.get());
+    tmp.release();
+    relax_prepare(c, H.L[0], A, p.relax_type);
+    bs.ftcoarse = 1;
+    double avcoarseratio = 0.0;
+    int level = 0;
+    if (!(normw > DBL_EPSILON))
+        throw Error(BCMG_EINVAL, "||w||_A <= DBL_EPSILON: no hierarchy can be built "
+                                 "(the reference asserts in Hierarchy::finalize_level)");
+    for (level = 1; level < p.max_levels;) {
+        H.L.emplace_back();
+        Level &prev = H.L[level - 1];
+        Level &next = H.L[level];
+        const DevCsr &Aprev = prev.mat();
+        matching_aggregation(c, p, bs, Aprev, w, prev.agg, prev.p, next.Aown);
+        transpose_P(c, Aprev.n, next.Aown.n, prev.agg.get(), prev.p.get(), prev.R);
+        relax_prepare(c, next, next.Aown, p.relax_type);
+        const double size_coarse = next.Aown.n;
+        avcoarseratio += Aprev.n / size_coarse;
+        level++;
+        if (size_coarse <= (double)bs.ftcoarse * bs.maxcoarsesize) break;
+    }
+    if (p.coarse_solver != p.relax_type)
+        relax_prepare(c, H.L[level - 1], H.L[level - 1].mat(), p.coarse_solver);
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+    double cx = 0, cw = 0;
+    for (int i = 0; i < level; i++) {
+        cx += H.L[i].mat().nnz;
+        cw += std::pow(2.0, i) * H.L[i].mat().nnz;
+    }
+    H.op_cmplx = cx / H.L[0].mat().nnz;
+    H.op_wcmplx = cw / H.L[0].mat().nnz;
+    H.avg_cratio = avcoarseratio / (level - 1);
+}
+
+/* ------------------------------------------------------------ cycles */
+
+struct LevelState {
+    double *x = nullptr, *x2 = nullptr;
+    bool zero = true;
+};
+
+struct CycleRun {
+    Ctx &c;
+    Boot &B;
+    const Hierarchy &H;
+    std::vector<LevelState> &st;
+    double *final_out;          /* where the very last level-0 sweep should write */
+    bool last_hierarchy;
+};
+
+const double *divisor(const Level &lv, int type) { return type == 4 ? lv.M.get() : lv.D.get(); }
+
+/* `sweeps` Jacobi sweeps on level lv, ping-pong between st.x and st.x2;
+   a sweep from a known-zero x is f / d (the SpMV of a zero vector is skipped) */
+void relax_level(Ctx &c, const Level &lv, int type, int sweeps, const double *f, LevelState &s,
+                 double *final_out, bool level0)
+{
+    const DevCsr &A = lv.mat();
+    const double *d = divisor(lv, type);
+    for (int k = 0; k < sweeps; k++) {
+        double *out = s.x2;
+        if (final_out && k == sweeps - 1 && s.x != final_out) out = final_out;
+        const bool timed = level0 && c.time_smoother;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timed) {
+            e0 = c.next_event();
+            e1 = c.next_event();
+            BCMG_HIP(hipEventRecord(e0, c.stream));
+        }
+        if (s.zero)
+            launch_jacobi_zero(c.stream, A.n, f, d, out);
+        else
+            launch_rows(c.stream, A, OP_JACOBI, s.x, f, d, out);
+        if (timed) BCMG_HIP(hipEventRecord(e1, c.stream));
+        s.zero = false;
+        if (out == s.x2) {
+            s.x2 = s.x;
+            s.x = out;
+        } else {
+            /* wrote straight into final_out: the old x becomes scratch */
+            s.x2 = s.x;
+            s.x = out;
+        }
+    }
+}
+
+/* recursive cycle (src/SOLVER/GAMG_cycle.cu:25-140); l is 1-based */
+void cycle(CycleRun &R, int l)
+{
+    Ctx &c = R.c;
+    Boot &B = R.B;
+    const Hierarchy &H = R.H;
+    const int L = H.num_levels();
+    const Level &lv = H.L[l - 1];
+    LevelState &s = R.st[l - 1];
+    const double *f = (l == 1) ? R.B.f0 : B.f[l - 1].get();
+    const bool top = (l == 1);
+    if (l == L) {
+        relax_level(c, lv, B.coarse_solver, B.prm.relaxnumber_coarse, f, s,
+                    top ? R.final_out : nullptr, top);
+        return;
+    }
+    relax_level(c, lv, B.prm.relax_type, B.prm.prerelax_sweeps, f, s, nullptr, top);
+    const DevCsr &A = lv.mat();
+    const double *rsrc = f;
+    if (!s.zero) {
+        launch_rows(c.stream, A, OP_RESID, s.x, f, nullptr, B.res.get());
+        rsrc = B.res.get();
+    }
+    launch_rows(c.stream, lv.R, OP_SPMV, rsrc, nullptr, nullptr, B.f[l].get());
+    LevelState &sc = R.st[l];
+    sc.zero = true;
+    for (int i = 1; i <= B.num_grid_sweeps[l - 1]; i++) {
+        cycle(R, l + 1);
+        if (l == L - 1) break;
+    }
+    if (!sc.zero) {
+        if (s.zero) {
+            launch_fill(c.stream, A.n, 0.0, s.x);
+            s.zero = false;
+        }
+        launch_prolong(c.stream, A.n, lv.agg.get(), lv.p.get(), sc.x, s.x);
+    }
+    relax_level(c, lv, B.prm.relax_type, B.prm.postrelax_sweeps, f, s, top ? R.final_out : nullptr,
+                top);
+}
+
+void run_hierarchy(Ctx &c, Boot &B, int k, std::vector<LevelState> &st, double *final_out)
+{
+    const Hierarchy &H = B.H[k];
+    for (int l = 1; l < H.num_levels(); l++) {
+        st[l].x = B.xa[l].get();
+        st[l].x2 = B.xb[l].get();
+        st[l].zero = true;
+    }
+    CycleRun R{c, B, H, st, final_out, true};
+    cycle(R, 1);
+}
+
+} // namespace
+
+void Boot::ensure_work()
+{
+    size_t maxL = 0;
+    for (auto &h : H) maxL = std::max(maxL, h.L.size());
+    if (xa.size() < maxL) {
+        xa.resize(maxL);
+        xb.resize(maxL);
+        f.resize(maxL);
+    }
+    for (size_t l = 0; l < maxL; l++) {
+        size_t nl = 0;
+        for (auto &h : H)
+            if (l < h.L.size()) nl = std::max(nl, (size_t)h.L[l].mat().n);
+        xa[l].ensure(nl);
+        xb[l].ensure(nl);
+        f[l].ensure(nl);
+    }
+    const size_t n0 = H[0].L[0].mat().n;
+    res.ensure(n0);
+}
+
+double a_norm(Ctx &c, const DevCsr &A, const double *x, double *tmp)
+{
+    launch_rows(c.stream, A, OP_SPMV, x, nullptr, nullptr, tmp);
+    c.red.ensure(1024 * 3 + 8);
+    double *out = c.red.get() + 1024 * 3;
+    launch_dots(c, A.n, 1, tmp, nullptr, nullptr, x, out);
+    BCMG_HIP(hipMemcpyAsync(c.h_scalars, out, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+    return std::sqrt(c.h_scalars[0]);
+}
+
+/* composite preconditioner x <- B(r; x) (src/SOLVER/FCG.cu:112-187). When
+   x_is_zero the content of x is ignored and treated as zero. */
+void precond_apply(Ctx &c, Boot &B, const double *r, double *x, bool x_is_zero)
+{
+    const int n = B.H[0].L[0].mat().n;
+    const int nh = (int)B.H.size();
+    size_t maxL = B.xa.size();
+    std::vector<LevelState> st(maxL);
+    B.f0 = r;
+    if (B.prm.solver_type == 0 || B.prm.solver_type == 1) {
+        st[0].x = x;
+        st[0].x2 = B.xa[0].get();
+        st[0].zero = x_is_zero;
+        std::vector<int> order;
+        for (int k = 0; k < nh; k++) order.push_back(k);
+        if (B.prm.solver_type == 1)
+            for (int k = nh - 1; k >= 0; k--) order.push_back(k);
+        for (size_t t = 0; t < order.size(); t++) {
+            const bool last = (t + 1 == order.size());
+            run_hierarchy(c, B, order[t], st, last ? x : nullptr);
+        }
+        if (st[0].zero) {
+            launch_fill(c.stream, n, 0.0, x);
+        } else if (st[0].x != x) {
+            launch_copy(c.stream, n, st[0].x, x);
+        }
+    } else {
+        /* additive: every hierarchy starts from the same x; mean of results */
+        B.xadd.ensure(n);
+        B.xsave.ensure(n);
+        launch_fill(c.stream, n, 0.0, B.xadd.get());
+        if (!x_is_zero) launch_copy(c.stream, n, x, B.xsave.get());
+        for (int k = 0; k < nh; k++) {
+            st[0].x = B.xb[0].get();
+            st[0].x2 = B.xa[0].get();
+            st[0].zero = x_is_zero;
+            if (!x_is_zero) launch_copy(c.stream, n, B.xsave.get(), st[0].x);
+            run_hierarchy(c, B, k, st, nullptr);
+            if (st[0].zero) launch_fill(c.stream, n, 0.0, st[0].x);
+            launch_axpy(c.stream, n, 1.0, st[0].x, B.xadd.get());
+        }
+        launch_scale(c.stream, n, 1.0 / (double)nh, B.xadd.get());
+        launch_copy(c.stream, n, B.xadd.get(), x);
+    }
+}
+
+namespace {
+
+/* src/BOOT/bootstrap.cu:5-44 */
+void inner_iterations(Ctx &c, Boot &B, DBuf<double> &w)
+{
+    const DevCsr &A = *B.A0;
+    const int n = A.n;
+    DBuf<double> x(n), rhs(n), tmp(n);
+    launch_fill(c.stream, n, 1.0, x.get());
+    launch_fill(c.stream, n, 0.0, rhs.get());
+    double normold = a_norm(c, A, x.get(), tmp.get());
+    double normnew = normold, ratio = 1.0;
+    for (int i = 1; i <= 15; i++) {
+        precond_apply(c, B, rhs.get(), x.get(), false);
+        normnew = a_norm(c, A, x.get(), tmp.get());
+        ratio = normnew / normold;
+        normold = normnew;
+    }
+    launch_scale(c.stream, n, 1. / normnew, x.get());
+    launch_copy(c.stream, n, x.get(), w.get());
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+    B.estimated_ratio = ratio;
+}
+
+} // namespace
+
+/* src/BOOT/bootstrap.cu:46-90 with src/AMG/AMG.cu:426-448 */
+void build_boot(Ctx &c, DevCsr &A, const bcmg_params &p, Boot &B)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    B.prm = p;
+    B.A0 = &A;
+    B.coarse_solver = p.coarse_solver;
+    B.num_grid_sweeps = grid_sweeps(p.cycle_type, p.max_levels);
+    BuildState bs;
+    bs.maxcoarsesize = (int)(40 * std::pow((double)A.n, (double)1. / 3.));
+    DBuf<double> w(A.n);
+    launch_fill(c.stream, A.n, 1.0, w.get());
+    B.estimated_ratio = 1.0;
+    B.H.clear();
+    while (B.estimated_ratio > p.conv_ratio && (int)B.H.size() < p.max_hrc) {
+        B.H.emplace_back();
+        adaptive_coarsening(c, p, bs, A, w.get(), B.H.back());
+        B.ensure_work();
+        if (B.H.size() > 1) inner_iterations(c, B, w);
+    }
+    B.ensure_work();
+    const int n = A.n;
+    B.v.ensure(n);
+    B.w.ensure(n);
+    B.d.ensure(n);
+    B.q.ensure(n);
+    B.r.ensure(n);
+    B.scal.ensure(16);
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+    B.t_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+/* FCG v3 from x0 = 0 (src/SOLVER/FCG.cu:433-585) */
+int fcg_solve(Ctx &c, Boot &B, const double *b, double *x, int *iters, double *resid,
+              double *delta0_out)
+{
+    const DevCsr &A = *B.A0;
+    const int n = A.n;
+    double *scal = B.scal.get();
+    double *v = B.v.get(), *w = B.w.get(), *d = B.d.get(), *q = B.q.get(), *r = B.r.get();
+    /* x0 = 0: r = b - A 0 = b */
+    launch_copy(c.stream, n, b, r);
+    launch_fill(c.stream, n, 0.0, x);
+    launch_dots(c, n, 1, r, nullptr, nullptr, r, scal + 7);
+    BCMG_HIP(hipMemcpyAsync(c.h_scalars, scal + 7, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+    const double delta0 = std::sqrt(c.h_scalars[0]);
+    const double rhs_norm = delta0;
+    if (delta0_out) *delta0_out = delta0;
+    if (delta0 <= DBL_EPSILON * rhs_norm) {
+        if (iters) *iters = 0;
+        throw Error(BCMG_EZERORHS, "initial residual is zero (the reference exits)");
+    }
+    precond_apply(c, B, r, v, true);
+    launch_rows(c.stream, A, OP_SPMV, v, nullptr, nullptr, w);
+    launch_dots(c, n, 2, r, w, nullptr, v, scal);
+    launch_fcg_first(c, n, scal, v, x, w, r);
+    /* d, q take the first direction; the next one goes to the other pair */
+    std::swap(v, d);
+    std::swap(w, q);
+    const double rtol = B.prm.rtol;
+    const int max_iter = B.prm.itnlim;
+    int iter = 0;
+    double l2 = 0.0;
+    do {
+        const bool even = (iter % 2) == 0;
+        double *z = even ? v : d;
+        double *t = even ? w : q;
+        const double *zo = even ? d : v;
+        const double *to = even ? q : w;
+        precond_apply(c, B, r, z, true);
+        launch_rows(c.stream, A, OP_SPMV, z, nullptr, nullptr, t);
+        launch_dots(c, n, 3, r, t, to, z, scal);
+        launch_fcg_scalars(c.stream, scal);
+        launch_fcg_update(c, n, scal, zo, z, x, to, t, r);
+        BCMG_HIP(hipMemcpyAsync(c.h_scalars, scal + 6, sizeof(double), hipMemcpyDeviceToHost,
+                                c.stream));
+        BCMG_HIP(hipStreamSynchronize(c.stream));
+        l2 = c.h_scalars[0];
+        iter++;
+    } while (l2 > rtol * delta0 && iter < max_iter);
+    if (iters) *iters = iter + 1;
+    if (resid) *resid = l2;
+    if (!std::isfinite(l2)) throw Error(BCMG_EBREAKDOWN, "non-finite residual");
+    return BCMG_OK;
+}
+
+hipEvent_t Ctx::next_event()
+{
+    if (ev_used == ev_pool.size()) {
+        hipEvent_t e;
+        BCMG_HIP(hipEventCreate(&e));
+        ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+}
+
+Ctx::~Ctx()
+{
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+    if (h_scalars) (void)hipHostFree(h_scalars);
+    if (h_ints) (void)hipHostFree(h_ints);
+    red.release();
+    iscratch.release();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+} // namespace bcmg

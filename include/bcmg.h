@@ -1,0 +1,168 @@
+/*
+ * bcmg.h - C-ABI of the gfx950 (MI355X) AMG-preconditioned flexible CG.
+ *
+ * This is the drop-in boundary for BootCMatchG's setup + solve path. Each
+ * entry point names the reference interface it replaces (file:line in the
+ * BootCMatchG source tree). Conventions:
+ *   - every function returns BCMG_OK (0) or a negative BCMG_E* code; the
+ *     reference calls exit()/assert instead (see the notes per function);
+ *     bcmg_last_error() gives the message of the calling thread's last error;
+ *   - matrices are CSR, 0-based, int32 indices, FP64 values, columns ascending
+ *     and an explicit diagonal entry in every row (src/matrix/CSR.h:13-28,
+ *     src/utility/setting.h:20-25);
+ *   - "device pointer" means memory of the context's GPU (bcmg_dmalloc or any
+ *     HIP allocation on that device); all work is ordered on the context's
+ *     stream and entry points return after it is complete unless noted;
+ *   - a context is used by one host thread at a time.
+ */
+#ifndef BCMG_H
+#define BCMG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BCMG_OK 0
+#define BCMG_EINVAL -1       /* bad argument or malformed matrix */
+#define BCMG_ESETTINGS -2    /* settings file missing, short or out of range */
+#define BCMG_EZERORHS -3     /* ||b - A x0|| <= DBL_EPSILON ||b||: the reference exits (FCG.cu:459-462) */
+#define BCMG_EUNSUP -4       /* a setting the reference aborts on (aggr_type 1: CSR.cu:402-404) */
+#define BCMG_EBREAKDOWN -5   /* non-finite residual (the reference asserts: FCG.cu:567) */
+#define BCMG_EHIP -6         /* HIP runtime error */
+#define BCMG_ENOMEM -7
+#define BCMG_ENODEV -8       /* no usable gfx950 device */
+
+/* The 14 settings of the AMGsettings file plus matchtype, in the field order
+   of the reference's struct params (src/AMG/AMG.cu:20-37). */
+typedef struct {
+    int solver_type;        /* 0 multiplicative, 1 symmetrized, 2 additive */
+    int max_hrc;
+    double conv_ratio;      /* rho */
+    int matchtype;          /* always 3 (AMG.cu:497) */
+    int aggrsweeps;
+    int aggrtype;           /* 0 unsmoothed (1 is rejected: BCMG_EUNSUP) */
+    int max_levels;
+    int cycle_type;         /* 0 V, 1 H, 2 W */
+    int coarse_solver;      /* 0 Jacobi, 4 L1 */
+    int relax_type;         /* 0 Jacobi, 4 L1 */
+    int prerelax_sweeps;
+    int postrelax_sweeps;
+    int itnlim;
+    double rtol;
+    int relaxnumber_coarse;
+} bcmg_params;
+
+typedef struct bcmg_ctx_s *bcmg_ctx;     /* one GPU, one stream */
+typedef struct bcmg_csr_s *bcmg_csr;     /* a device-resident CSR matrix */
+typedef struct bcmg_boot_s *bcmg_boot;   /* a built composite (1..max_hrc hierarchies) */
+
+const char *bcmg_last_error(void);
+
+/* Parse a 14-line AMGsettings file. Replaces AMG::Params::initFromFile
+   (src/AMG/AMG.cu:480-526, parser src/AMG/utility.cu:7-25); the reference
+   exits on a missing file or a value outside the allowed sets (AMG.cu:8-18). */
+int bcmg_params_from_file(const char *path, bcmg_params *out);
+
+/* Context on GPU `device`. Replaces Handles::init / Handles::free
+   (src/utility/handles.cu:12-48): one HIP stream, no library handles. */
+int bcmg_ctx_create(int device, bcmg_ctx *out);
+int bcmg_ctx_destroy(bcmg_ctx ctx);
+int bcmg_sync(bcmg_ctx ctx);
+
+/* Device memory helpers (Vector::init, copyToDevice, copyToHost, free: src/matrix/vector.cu). */
+int bcmg_dmalloc(bcmg_ctx ctx, size_t bytes, void **ptr);
+int bcmg_dfree(bcmg_ctx ctx, void *ptr);
+int bcmg_memcpy_h2d(bcmg_ctx ctx, void *dst_dev, const void *src_host, size_t bytes);
+int bcmg_memcpy_d2h(bcmg_ctx ctx, void *dst_host, const void *src_dev, size_t bytes);
+int bcmg_memcpy_d2d(bcmg_ctx ctx, void *dst_dev, const void *src_dev, size_t bytes);
+
+/* Copy a host CSR to the device. Replaces CSRm::copyToDevice
+   (src/matrix/CSR.cu:194-217). The input is validated (sorted columns,
+   diagonal present); the reference leaves such inputs undefined. */
+int bcmg_csr_upload(bcmg_ctx ctx, int n, int nnz, const int *row, const int *col,
+                    const double *val, bcmg_csr *out);
+int bcmg_csr_download(bcmg_csr A, int *row, int *col, double *val);
+int bcmg_csr_info(bcmg_csr A, int *n, int *nnz);
+/* Replaces CSRm::free (src/matrix/CSR.cu:160-181). */
+int bcmg_csr_free(bcmg_csr A);
+
+/* AMG setup: the bootstrap composite of up to max_hrc hierarchies built by
+   compatible weighted matching. Replaces Bootstrap::bootstrap
+   (src/BOOT/bootstrap.cu:46-90) with AMG::BootBuildData::initByParams,
+   AMG::ApplyData::initByParams/setGridSweeps (src/AMG/AMG.cu:359-448).
+   A must stay alive while the result is used. */
+int bcmg_setup(bcmg_ctx ctx, bcmg_csr A, const bcmg_params *p, bcmg_boot *out);
+/* Replaces AMG::Boot::free (src/AMG/AMG.cu:465-469). */
+int bcmg_boot_free(bcmg_boot B);
+
+/* FCG v3 from x0 = 0 with the composite preconditioner; b and x are device
+   pointers of length n. Replaces flexibileConjugateGradients_v3
+   (src/SOLVER/FCG.cu:433-585). iters follows the reference's count
+   (loops + 1, FCG.cu:569); resid is the last recursive ||r||. */
+int bcmg_fcg(bcmg_ctx ctx, bcmg_boot B, const double *b_dev, double *x_dev, int *iters,
+             double *resid);
+
+/* setup + FCG + free on host arrays: the whole of solve()
+   (src/SOLVER/solver.cu:20-129), with x returned in host memory.
+   setup_ms / solve_ms (may be NULL) are the two phases' wall times. */
+int bcmg_solve(int n, int nnz, const int *row, const int *col, const double *val,
+               const bcmg_params *p, const double *rhs, double *x, int *iters, double *resid,
+               double *setup_ms, double *solve_ms);
+
+/* ---- inspection of a built composite (host outputs) ---------------------- */
+int bcmg_boot_info(bcmg_boot B, int *n_hrc, double *estimated_ratio);
+/* number of levels of hierarchy k (AMG hierarchy->num_levels, AMG.cu:51) */
+int bcmg_hier_levels(bcmg_boot B, int k);
+int bcmg_level_size(bcmg_boot B, int k, int l, int *n, int *nnz);
+int bcmg_level_A(bcmg_boot B, int k, int l, int *row, int *col, double *val);
+/* prolongator from level l to l+1: coarse index and value per fine row */
+int bcmg_level_P(bcmg_boot B, int k, int l, int *agg, double *p);
+/* smoother divisors: D = diag(A), M = L1 row sums; has* tell which exist */
+int bcmg_level_DM(bcmg_boot B, int k, int l, double *D, double *M, int *hasD, int *hasM);
+
+/* ---- building blocks (device pointers), for parity tests ---------------- */
+/* y = A x. Replaces CSRm::CSRVector_product_adaptive_miniwarp with alpha 1,
+   beta 0 (src/matrix/CSR.cu:926-960). */
+int bcmg_spmv(bcmg_ctx ctx, bcmg_csr A, const double *x, double *y);
+/* y = f - A x (the OP 2 form of the same kernel, GAMG_cycle.cu:59-67). */
+int bcmg_residual(bcmg_ctx ctx, bcmg_csr A, const double *x, const double *f, double *y);
+/* `sweeps` Jacobi sweeps u <- u + (f - A u) / d, result in u; u2 is scratch.
+   Replaces jacobi_adaptive_miniwarp (src/AMG/relaxation.cu:211-285). */
+int bcmg_relax(bcmg_ctx ctx, bcmg_csr A, const double *d, const double *f, double *u,
+               double *u2, int sweeps);
+/* matching weights W aligned with A's nnz (diagonal slots 0). Replaces
+   Matching::makeAH + toMaximumProductMatrix (src/AMG/matching.cu:293-472). */
+int bcmg_weights(bcmg_ctx ctx, bcmg_csr A, const double *w, double *W);
+/* suitor matching and symmetrisation given W. Replaces the body of
+   Matching::suitor (src/AMG/matching.cu:489-496, suitor.cu:427-454,
+   unsymMatching.cu:58-71). n_unsym = number of non-mutual suitor pairs. */
+int bcmg_matching(bcmg_ctx ctx, bcmg_csr A, const double *W, int *M_raw, int *M_sym,
+                  int *n_unsym);
+/* pairwise aggregation of a symmetric matching. Replaces
+   matchingPairAggregation's aggregation (src/AMG/matchingAggregation.cu:181-208);
+   aggregates are numbered by their smallest fine index. */
+int bcmg_aggregate(bcmg_ctx ctx, int n, const int *M_sym, const double *w, int *agg, double *p,
+                   int *n_coarse);
+/* A_c = R (A P) for a pairwise P (every aggregate has 1 or 2 fine rows).
+   Replaces the two CSRm::CSRCSR_product calls of matchingAggregation
+   (src/AMG/matchingAggregation.cu:241-244). */
+int bcmg_galerkin(bcmg_ctx ctx, bcmg_csr A, const int *agg, const double *p, int n_coarse,
+                  bcmg_csr *Ac);
+/* x <- B(r; x): one application of the composite preconditioner with x as
+   the initial guess. Replaces preconditionApply (src/SOLVER/FCG.cu:112-187). */
+int bcmg_precond_apply(bcmg_ctx ctx, bcmg_boot B, const double *r, double *x);
+
+/* ---- measurement -------------------------------------------------------- */
+/* When enabled, every fine-level (level 0) smoother sweep launched by
+   bcmg_fcg is bracketed by HIP events on the context stream; the totals are
+   read (and reset) with bcmg_smoother_timing. */
+int bcmg_time_smoother(bcmg_ctx ctx, int enable);
+int bcmg_smoother_timing(bcmg_ctx ctx, double *total_ms, int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BCMG_H */

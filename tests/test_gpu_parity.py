@@ -1,0 +1,231 @@
+"""Parity of the gfx950 library (through the C-ABI) against the serial oracle.
+
+Integer and setup results (matching weights, matchings, aggregates, coarse
+patterns and values, prolongators, smoother divisors) must be bit-exact.
+Solve-phase floating point is compared with stated tolerances: the kernels
+sum rows in a lane-strided order with fma, the oracle sequentially.
+FCG iteration counts must agree within +-2.
+"""
+import numpy as np
+import pytest
+
+import bootcmatchg_amd as bc
+from bootcmatchg_amd import operators as ops
+from oracle import oracle as orc
+
+from common import OPERATORS, SETTINGS, bits
+
+pytestmark = pytest.mark.gpu
+
+# |y_gpu - y_oracle| <= RTOL_ROW * (|A| |x|)_i : two orders of summation
+RTOL_ROW = 1e-13
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = bc.Context(0)
+    yield c
+    c.close()
+
+
+def dev(ctx, a):
+    return bc.DeviceArray.from_numpy(ctx, a)
+
+
+def absmv(A, x):
+    return ops.to_scipy((A[0], A[1], np.abs(A[2]))) @ np.abs(x)
+
+
+@pytest.mark.parametrize("name", ["poisson3d_17", "aniso2d_48", "lognormal3d_14"])
+def test_spmv_residual_relax(ctx, name):
+    A = OPERATORS[name]()
+    n = len(A[0]) - 1
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, n)
+    f = rng.uniform(-1, 1, n)
+    dA = bc.Csr(ctx, A)
+    dx, df, dy = dev(ctx, x), dev(ctx, f), bc.DeviceArray(ctx, n)
+    bound = RTOL_ROW * (absmv(A, x) + np.abs(f)) + 1e-300
+    bc.spmv(ctx, dA, dx, dy)
+    assert np.all(np.abs(dy.numpy() - orc.spmv(A, x)) <= bound)
+    bc.residual(ctx, dA, dx, df, dy)
+    assert np.all(np.abs(dy.numpy() - (f - orc.spmv(A, x))) <= bound)
+    S = ops.to_scipy(A)
+    d = np.abs(S).sum(axis=1).A1
+    u = dev(ctx, x)
+    u2 = bc.DeviceArray(ctx, n)
+    bc.relax(ctx, dA, dev(ctx, d), df, u, u2, 4)
+    uo = orc.relax(A, d, f, x, 4)
+    assert np.max(np.abs(u.numpy() - uo)) <= 1e-12 * max(1.0, np.max(np.abs(uo)))
+
+
+def test_spmv_long_rows(ctx):
+    """Rows longer than a wave (64 lanes) and empty-ish rows: a dense-ish
+    random symmetric matrix with a diagonal in every row."""
+    rng = np.random.default_rng(5)
+    n = 300
+    import scipy.sparse as sp
+    M = sp.random(n, n, density=0.4, random_state=3, format="csr")
+    M = M + M.T + sp.eye(n) * n
+    M = M.tocsr()
+    M.sort_indices()
+    A = (M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.astype(np.float64))
+    x = rng.uniform(-1, 1, n)
+    dA = bc.Csr(ctx, A)
+    dy = bc.DeviceArray(ctx, n)
+    bc.spmv(ctx, dA, dev(ctx, x), dy)
+    bound = RTOL_ROW * absmv(A, x) + 1e-300
+    assert np.all(np.abs(dy.numpy() - orc.spmv(A, x)) <= bound)
+
+
+@pytest.mark.parametrize("name", ["poisson3d_12", "aniso2d_48", "lognormal3d_14"])
+def test_weights_bitexact(ctx, name):
+    A = OPERATORS[name]()
+    n = len(A[0]) - 1
+    rng = np.random.default_rng(2)
+    for w in (np.ones(n), rng.uniform(0.1, 2.0, n), rng.uniform(-1, 1, n)):
+        W = bc.DeviceArray(ctx, len(A[1]))
+        bc.weights(ctx, bc.Csr(ctx, A), dev(ctx, w), W)
+        assert np.array_equal(bits(W.numpy()), bits(orc.weights(A, w)))
+
+
+@pytest.mark.parametrize("name", ["poisson3d_12", "poisson2d_33", "lognormal3d_14", "aniso2d_48"])
+def test_matching_aggregation_galerkin_bitexact(ctx, name):
+    A = OPERATORS[name]()
+    n = len(A[0]) - 1
+    rng = np.random.default_rng(4)
+    for w in (np.ones(n), rng.uniform(0.2, 1.5, n)):
+        W = orc.weights(A, w)
+        Mr_o, Ms_o, nu_o = orc.matching(A, W)
+        dA = bc.Csr(ctx, A)
+        Mr, Ms = bc.DeviceArray(ctx, n, np.int32), bc.DeviceArray(ctx, n, np.int32)
+        nu = bc.matching(ctx, dA, dev(ctx, W), Mr, Ms)
+        assert np.array_equal(Mr.numpy(), Mr_o)
+        assert np.array_equal(Ms.numpy(), Ms_o)
+        assert nu == nu_o
+        agg_o, p_o, nc_o = orc.aggregate(Ms_o, w)
+        agg, p = bc.DeviceArray(ctx, n, np.int32), bc.DeviceArray(ctx, n)
+        nc = bc.aggregate(ctx, n, dev(ctx, Ms_o), dev(ctx, w), agg, p)
+        assert nc == nc_o
+        assert np.array_equal(agg.numpy(), agg_o)
+        assert np.array_equal(bits(p.numpy()), bits(p_o))
+        Ac_o = orc.galerkin(A, agg_o, p_o, nc_o)
+        Ac = bc.galerkin(ctx, dA, dev(ctx, agg_o), dev(ctx, p_o), nc_o).download()
+        assert np.array_equal(Ac[0], Ac_o[0])
+        assert np.array_equal(Ac[1], Ac_o[1])
+        assert np.array_equal(bits(Ac[2]), bits(Ac_o[2]))
+
+
+def _compare_hierarchy(B, O, k):
+    L = B.levels(k)
+    assert L == O.levels(k)
+    for l in range(L):
+        Ag = B.level_A(k, l)
+        Ao = O.level_A(k, l)
+        for a, b in zip(Ag[:2], Ao[:2]):
+            assert np.array_equal(a, b), (k, l)
+        assert np.array_equal(bits(Ag[2]), bits(Ao[2])), (k, l)
+        Dg, Mg = B.level_DM(k, l)
+        Do, Mo = O.level_DM(k, l)
+        assert (Dg is None) == (Do is None) and (Mg is None) == (Mo is None)
+        if Do is not None:
+            assert np.array_equal(bits(Dg), bits(Do)), (k, l)
+        if Mo is not None:
+            assert np.array_equal(bits(Mg), bits(Mo)), (k, l)
+        if l + 1 < L:
+            ag, pg = B.level_P(k, l)
+            ao, po = O.level_P(k, l)
+            assert np.array_equal(ag, ao), (k, l)
+            assert np.array_equal(bits(pg), bits(po)), (k, l)
+
+
+@pytest.mark.parametrize("name,setting", [
+    ("poisson3d_17", "V_L1_2sweeps"),
+    ("aniso2d_48", "W_L1_3sweeps"),
+    ("lognormal3d_14", "V_L1_2sweeps"),
+    ("poisson2d_33", "H_jacobi_1sweep"),
+])
+def test_hierarchy_and_fcg(ctx, name, setting):
+    A = OPERATORS[name]()
+    n = len(A[0]) - 1
+    prm_o = orc.params(**SETTINGS[setting])
+    prm = bc.make_params(**SETTINGS[setting])
+    O = orc.Boot(A, prm_o)
+    assert O.rc == 0
+    dA = bc.Csr(ctx, A)
+    B = bc.Boot(ctx, dA, prm)
+    assert B.info()[0] == O.info()[0]
+    _compare_hierarchy(B, O, 0)
+    b = ops.uniform_pm1(0, n)
+    rc, xo, res_o, _ = O.fcg(b)
+    assert rc == 0
+    db, dx = dev(ctx, b), bc.DeviceArray(ctx, n)
+    it, resid = B.fcg(db, dx)
+    x = dx.numpy()
+    assert abs(it - res_o.iterations) <= 2, (it, res_o.iterations)
+    S = ops.to_scipy(A)
+    rtol = prm.rtol
+    assert np.linalg.norm(b - S @ x) <= 10 * rtol * np.linalg.norm(b)
+    assert np.linalg.norm(x - xo) <= 1e-5 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("name,setting", [
+    ("poisson3d_12", "boot3_mult"),
+    ("lognormal3d_10", "boot3_sym"),
+    ("poisson3d_12", "boot2_add_mixed"),
+])
+def test_bootstrap_composites(ctx, name, setting):
+    """Hierarchies 0 and 1 are built from w = 1 and must be bit-exact; the
+    bootstrapped smooth vector (15 composite cycles) carries solve-phase
+    rounding, so later hierarchies are compared through iterations."""
+    A = OPERATORS[name]()
+    n = len(A[0]) - 1
+    prm_o = orc.params(**SETTINGS[setting])
+    prm = bc.make_params(**SETTINGS[setting])
+    O = orc.Boot(A, prm_o)
+    B = bc.Boot(ctx, bc.Csr(ctx, A), prm)
+    nh_o, er_o = O.info()
+    nh, er = B.info()
+    assert nh == nh_o
+    assert abs(er - er_o) <= 1e-6 * abs(er_o)
+    for k in range(min(nh, 2)):
+        _compare_hierarchy(B, O, k)
+    b = ops.uniform_pm1(1, n)
+    rc, xo, res_o, _ = O.fcg(b)
+    dx = bc.DeviceArray(ctx, n)
+    it, _ = B.fcg(dev(ctx, b), dx)
+    assert abs(it - res_o.iterations) <= 2, (it, res_o.iterations)
+    assert np.linalg.norm(dx.numpy() - xo) <= 1e-5 * np.linalg.norm(xo)
+
+
+def test_precond_apply(ctx):
+    A = OPERATORS["lognormal3d_10"]()
+    n = len(A[0]) - 1
+    prm = SETTINGS["boot3_mult"]
+    O = orc.Boot(A, orc.params(**prm))
+    B = bc.Boot(ctx, bc.Csr(ctx, A), bc.make_params(**dict(prm, max_hrc=1)))
+    O1 = orc.Boot(A, orc.params(**dict(prm, max_hrc=1)))
+    r = ops.uniform_pm1(2, n)
+    x0 = ops.uniform_pm1(3, n)
+    dx = dev(ctx, x0)
+    B.precond(dev(ctx, r), dx)
+    xo = O1.precond(r, x0)
+    assert np.linalg.norm(dx.numpy() - xo) <= 1e-12 * np.linalg.norm(xo)
+    del O
+
+
+def test_solve_host_api_and_errors(ctx):
+    A = ops.poisson3d(10)
+    n = len(A[0]) - 1
+    prm = bc.make_params(rtol=1e-6)
+    x, info = bc.solve(A, prm, np.ones(n))
+    rc, xo, res = orc.solve(A, np.ones(n), orc.params(rtol=1e-6))
+    assert rc == 0
+    assert abs(info["iterations"] - res.iterations) <= 2
+    assert np.linalg.norm(x - xo) <= 1e-5 * np.linalg.norm(xo)
+    with pytest.raises(bc.BcmgError) as e:
+        bc.solve(A, prm, np.zeros(n))
+    assert e.value.code == bc.EZERORHS
+    with pytest.raises(bc.BcmgError) as e:
+        bc.solve(A, bc.make_params(aggrtype=1), np.ones(n))
+    assert e.value.code == bc.EUNSUP
