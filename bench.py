@@ -1,0 +1,177 @@
+"""Benchmark of the AMG-preconditioned FCG drop-in path on one MI355X.
+
+Workload (one step = one complete solve(): bootstrap setup of the AMG
+hierarchy + FCG v3 to rtol 1e-8, as the reference's solve() does,
+src/SOLVER/solver.cu:20-129): 3D 7-point Poisson on a 160^3 grid
+(4,096,000 rows, 28,518,400 nnz, int32 indices, FP64 values), b uniform[-1,1]
+from SplitMix64 stream 0, settings 0,1,0.8,2,0,39,0,4,4,16,4,4,2000,1e-8.
+The operator and b are resident in HBM before the timed region.
+
+Printed JSON line: value = solves/s of the whole step; extras give the
+setup/solve split, the rate with a cached hierarchy, the dominant kernel's
+roofline (level-0 Jacobi/L1 sweep, timed with HIP events inside the timed
+region) and the serial CPU oracle on the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("solves/s & p50/p99 latency, mixed vs FP64 AMG-FCG at rtol 1e-8; SpMV GB/s")
+HBM_PEAK_GBS = 8000.0
+GRID = 160
+SETTINGS = dict(solver_type=0, max_hrc=1, conv_ratio=0.8, aggrsweeps=2, aggrtype=0,
+                max_levels=39, cycle_type=0, coarse_solver=4, relax_type=4,
+                relaxnumber_coarse=16, prerelax_sweeps=4, postrelax_sweeps=4, itnlim=2000,
+                rtol=1e-8)
+
+
+def sweep_bytes(n, nnz):
+    """Algorithmic HBM bytes of one Jacobi/L1 sweep u' = u + (f - A u) / d:
+    val + col (12 B/nnz), row pointer, u (gathered, counted once), f, d, u'."""
+    return 12 * nnz + 4 * (n + 1) + 4 * 8 * n
+
+
+def spmv_bytes(n, nnz):
+    """y = A x: val + col, row pointer, x (counted once), y."""
+    return 12 * nnz + 4 * (n + 1) + 2 * 8 * n
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=GRID)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the path has no multi-GPU form here "
+              "(each GPU would serve its own solves; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import bootcmatchg_amd as bc
+    from bootcmatchg_amd import operators as ops
+
+    N = args.grid
+    A = ops.poisson3d(N)
+    n = len(A[0]) - 1
+    nnz = len(A[1])
+    b = ops.uniform_pm1(0, n)
+    prm = bc.make_params(**SETTINGS)
+
+    ctx = bc.Context(0)
+    dA = bc.Csr(ctx, A)
+    db = bc.DeviceArray.from_numpy(ctx, b)
+    dx = bc.DeviceArray(ctx, n)
+
+    def step(record=None):
+        t0 = time.perf_counter()
+        B = bc.Boot(ctx, dA, prm)
+        t1 = time.perf_counter()
+        it, res = B.fcg(db, dx)
+        t2 = time.perf_counter()
+        levels = [B.level_size(0, l) for l in range(B.levels(0))]
+        B.free()
+        if record is not None:
+            record.append(dict(setup_ms=(t1 - t0) * 1e3, solve_ms=(t2 - t1) * 1e3,
+                               total_ms=(t2 - t0) * 1e3, iterations=it, residual=res,
+                               levels=levels))
+        return it
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    rec = []
+    bc.time_smoother(ctx, True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(rec)
+    ctx.sync()
+    t1 = time.perf_counter()
+    sm_ms, sm_launches = bc.smoother_timing(ctx)
+    bc.time_smoother(ctx, False)
+    elapsed = t1 - t0
+    lat = np.array([r["total_ms"] for r in rec])
+
+    # cached hierarchy: FCG only (the serving form's unit of work)
+    B = bc.Boot(ctx, dA, prm)
+    B.fcg(db, dx)
+    ctx.sync()
+    tc0 = time.perf_counter()
+    for _ in range(args.steps):
+        it_c, _ = B.fcg(db, dx)
+    ctx.sync()
+    tc1 = time.perf_counter()
+    x = dx.numpy()
+    B.free()
+    S = ops.to_scipy(A)
+    true_rel = float(np.linalg.norm(b - S @ x) / np.linalg.norm(b))
+
+    # fine-level SpMV bandwidth (algorithmic bytes / event-timed launch)
+    dy = bc.DeviceArray(ctx, n)
+    spmv_ms = bc.time_spmv(ctx, dA, db, dy, 50)
+    spmv_gbs = spmv_bytes(n, nnz) / (spmv_ms * 1e-3) / 1e9
+
+    sweep_avg_ms = sm_ms / max(sm_launches, 1)
+    sweep_b = sweep_bytes(n, nnz)
+    achieved = sweep_b / (sweep_avg_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        tcpu0 = time.perf_counter()
+        rc, xo, ores = orc.solve(A, b, orc.params(**SETTINGS))
+        tcpu1 = time.perf_counter()
+        cpu = {"value": 1.0 / (tcpu1 - tcpu0), "unit": "solves/s", "cores": 1,
+               "kind": "port",
+               "sample": "one full solve (setup %.2f s + FCG %.2f s, %d iterations) of the "
+                         "same 160^3 workload by the serial C++ oracle, 1 thread"
+                         % (ores.setup_seconds, ores.solve_seconds, ores.iterations),
+               "iterations": ores.iterations,
+               "solution_rel_diff_vs_gpu": float(np.linalg.norm(x - xo) / np.linalg.norm(xo))}
+
+    out = {
+        "metric": METRIC,
+        "value": args.steps / elapsed,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "poisson3d_7pt_%d^3 fp64 drop-in solve (setup+FCG) rtol 1e-8" % N,
+                   "rows": n, "nnz": nnz, "rhs": "uniform[-1,1] splitmix64 seed 0",
+                   "settings": "0,1,0.8,2,0,39,0,4,4,16,4,4,2000,1e-8"},
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "setup_ms": float(np.mean([r["setup_ms"] for r in rec])),
+        "solve_ms": float(np.mean([r["solve_ms"] for r in rec])),
+        "iterations": rec[-1]["iterations"],
+        "levels": rec[-1]["levels"],
+        "true_rel_residual": true_rel,
+        "cached_hierarchy_solves_per_s": args.steps / (tc1 - tc0),
+        "spmv_gbs": spmv_gbs,
+        "spmv_frac_of_hbm": spmv_gbs / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "level-0 Jacobi/L1 sweep (k_rows<8, OP_JACOBI>)",
+                     "bytes_per_launch": sweep_b, "avg_launch_ms": sweep_avg_ms,
+                     "launches": sm_launches},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -340,6 +340,25 @@ def aggregate(ctx, n, Msym_dev, w_dev, agg_dev, p_dev):
     return nc.value
 
 
+def time_smoother(ctx, enable):
+    """Bracket every level-0 Jacobi sweep (from a non-zero iterate) with
+    HIP events until disabled; read with smoother_timing()."""
+    _check(lib().bcmg_time_smoother(ctx.h, 1 if enable else 0))
+
+
+def smoother_timing(ctx):
+    tot = ctypes.c_double()
+    cnt = ctypes.c_int()
+    _check(lib().bcmg_smoother_timing(ctx.h, ctypes.byref(tot), ctypes.byref(cnt)))
+    return tot.value, cnt.value
+
+
+def time_spmv(ctx, A, x_dev, y_dev, reps):
+    ms = ctypes.c_double()
+    _check(lib().bcmg_time_spmv(ctx.h, A.h, x_dev.ptr, y_dev.ptr, reps, ctypes.byref(ms)))
+    return ms.value
+
+
 def galerkin(ctx, A, agg_dev, p_dev, nc):
     h = _vp()
     _check(lib().bcmg_galerkin(ctx.h, A.h, agg_dev.ptr, p_dev.ptr, nc, ctypes.byref(h)))

@@ -79,10 +79,14 @@ struct DevCsr {
     DBuf<int> row, col;
     DBuf<double> val;
     int group = 8;          /* lanes per row of the row-group kernels */
+    int tile = 0;           /* max nnz of a 256-row block; 0 = not analysed */
 };
 
 /* Lanes per row for an average row length (a power of two in [1, 64]). */
 int lanes_for(int n, long long nnz);
+struct Ctx;
+/* choose the row kernel of A (LDS row blocks when a block's nnz fits) */
+void analyze_rows(Ctx &c, DevCsr &A);
 
 /* ------------------------------------------------------------- context */
 

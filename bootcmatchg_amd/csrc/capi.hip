@@ -305,7 +305,7 @@ int bcmg_csr_upload(bcmg_ctx ctx, int n, int nnz, const int *row, const int *col
         BCMG_HIP(hipMemcpyAsync(A.col.get(), col, sizeof(int) * nnz, hipMemcpyHostToDevice, s));
         BCMG_HIP(hipMemcpyAsync(A.val.get(), val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
         BCMG_HIP(hipStreamSynchronize(s));
-        A.group = lanes_for(n, nnz);
+        analyze_rows(ctx->c, A);
     } catch (...) {
         delete h;
         throw;
@@ -603,6 +603,28 @@ int bcmg_smoother_timing(bcmg_ctx ctx, double *total_ms, int *launches)
     if (total_ms) *total_ms = tot;
     if (launches) *launches = (int)(ctx->c.ev_used / 2);
     ctx->c.ev_used = 0;
+    API_END
+}
+
+int bcmg_time_spmv(bcmg_ctx ctx, bcmg_csr A, const double *x, double *y, int reps,
+                   double *avg_ms)
+{
+    if (!ctx || !A || !x || !y || reps < 1 || !avg_ms) return BCMG_EINVAL;
+    API_BEGIN
+    use_device(ctx);
+    hipEvent_t e0, e1;
+    BCMG_HIP(hipEventCreate(&e0));
+    BCMG_HIP(hipEventCreate(&e1));
+    launch_rows(ctx->c.stream, A->A, OP_SPMV, x, nullptr, nullptr, y);
+    BCMG_HIP(hipEventRecord(e0, ctx->c.stream));
+    for (int k = 0; k < reps; k++) launch_rows(ctx->c.stream, A->A, OP_SPMV, x, nullptr, nullptr, y);
+    BCMG_HIP(hipEventRecord(e1, ctx->c.stream));
+    BCMG_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    BCMG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
     API_END
 }
 

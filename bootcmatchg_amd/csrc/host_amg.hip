@@ -168,7 +168,7 @@ void relax_level(Ctx &c, const Level &lv, int type, int sweeps, const double *f,
     for (int k = 0; k < sweeps; k++) {
         double *out = s.x2;
         if (final_out && k == sweeps - 1 && s.x != final_out) out = final_out;
-        const bool timed = level0 && c.time_smoother;
+        const bool timed = level0 && c.time_smoother && !s.zero;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timed) {
             e0 = c.next_event();

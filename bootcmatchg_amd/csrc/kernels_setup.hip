@@ -756,8 +756,7 @@ void galerkin_pairwise(Ctx &c, const DevCsr &A, const int *agg, const double *p,
                       stoff.get(), Jst.get(), tst.get(), first.get(), Ac.row.get(), Ac.col.get(),
                       Ac.val.get());
     BCMG_CHECK_LAUNCH();
-    Ac.group = lanes_for(nc, nnzc);
-    BCMG_HIP(hipStreamSynchronize(c.stream));
+    analyze_rows(c, Ac);
 }
 
 void restrict_w(hipStream_t s, int nc, const int *members, const double *p, const double *w,
@@ -791,8 +790,7 @@ void transpose_P(Ctx &c, int n, int nc, const int *agg, const double *p, DevCsr 
     k_tfill<<<blocks_for(n), BS, 0, c.stream>>>(n, agg, p, cursor.get(), R.col.get(), R.val.get());
     k_sort_rows<<<blocks_for(nc), BS, 0, c.stream>>>(nc, R.row.get(), R.col.get(), R.val.get());
     BCMG_CHECK_LAUNCH();
-    R.group = lanes_for(nc, n);
-    BCMG_HIP(hipStreamSynchronize(c.stream));
+    analyze_rows(c, R);
 }
 
 } // namespace bcmg

@@ -21,6 +21,7 @@ namespace {
 
 constexpr int BS = 256;          /* threads per block */
 constexpr int RED_MAX_BLOCKS = 1024;
+constexpr int STREAM_TILE_MAX = 4096;  /* doubles of LDS per block (32 KB) */
 
 /* Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
    Return the logical block index so that XCD x owns a contiguous range. */
@@ -29,6 +30,70 @@ __device__ __forceinline__ int xcd_block(int bid, int nb)
     const int q = nb >> 3, rm = nb & 7;
     const int x = bid & 7, l = bid >> 3;
     return x * q + (x < rm ? x : rm) + l;
+}
+
+/* Row-block kernel: block b owns rows [256 b, 256 b + 256). Phase 1 streams
+   the block's slice of val/col with consecutive lanes on consecutive entries
+   (fully coalesced) and stores the products a_ij x_j in LDS; phase 2 gives one
+   row to each thread, which sums its products in column order and applies the
+   epilogue with coalesced loads of f, d, x and a coalesced store. The sum
+   order (products rounded, then added left to right) is the serial one. */
+template <int OP>
+__global__ __launch_bounds__(BS) void k_rows_stream(int n, const int *__restrict__ rp,
+                                                    const int *__restrict__ ci,
+                                                    const double *__restrict__ av,
+                                                    const double *__restrict__ x,
+                                                    const double *__restrict__ f,
+                                                    const double *__restrict__ d,
+                                                    double *__restrict__ y)
+{
+    extern __shared__ double prod[];
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = b * BS;
+    const int r1 = min(r0 + BS, n);
+    const int e0 = rp[r0];
+    const int cnt = rp[r1] - e0;
+    constexpr int U = 4;
+    for (int k0 = threadIdx.x; k0 < cnt; k0 += BS * U) {
+        int c[U];
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k0 + u * BS;
+            if (k < cnt) {
+                c[u] = ci[e0 + k];
+                v[u] = av[e0 + k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k0 + u * BS;
+            if (k < cnt) prod[k] = v[u] * x[c[u]];
+        }
+    }
+    __syncthreads();
+    const int r = r0 + threadIdx.x;
+    if (r >= r1) return;
+    const int a = rp[r] - e0, e = rp[r + 1] - e0;
+    double t = 0.0;
+    for (int k = a; k < e; k++) t = t + prod[k];
+    if (OP == OP_SPMV)
+        y[r] = t;
+    else if (OP == OP_RESID)
+        y[r] = f[r] - t;
+    else
+        y[r] = (f[r] - t) / d[r] + x[r];
+}
+
+/* largest nnz of any 256-row block (the LDS tile of k_rows_stream) */
+__global__ __launch_bounds__(BS) void k_block_nnz(int n, const int *__restrict__ rp,
+                                                  int *__restrict__ maxv)
+{
+    const int b = blockIdx.x * BS + threadIdx.x;
+    const long long r0 = (long long)b * BS;
+    if (r0 >= n) return;
+    const int r1 = (int)min(r0 + BS, (long long)n);
+    atomicMax(maxv, rp[r1] - rp[r0]);
 }
 
 template <int G, int OP>
@@ -250,6 +315,12 @@ void rows_dispatch(hipStream_t s, const DevCsr &A, const double *x, const double
                    const double *d, double *y)
 {
     const int n = A.n;
+    if (A.tile > 0 && A.tile <= STREAM_TILE_MAX) {
+        const int nb = (n + BS - 1) / BS;
+        k_rows_stream<OP><<<nb, BS, (size_t)A.tile * sizeof(double), s>>>(
+            n, A.row.get(), A.col.get(), A.val.get(), x, f, d, y);
+        return;
+    }
     const int G = A.group;
     const int nb = blocks_for((long long)n * G);
     const int *rp = A.row.get(), *ci = A.col.get();
@@ -273,6 +344,23 @@ int lanes_for(int n, long long nnz)
     int G = 1;
     while (G < 64 && G < avg) G <<= 1;
     return G;
+}
+
+void analyze_rows(Ctx &c, DevCsr &A)
+{
+    A.group = lanes_for(A.n, A.nnz);
+    if (A.n <= 0) {
+        A.tile = 0;
+        return;
+    }
+    DBuf<int> mx(1);
+    BCMG_HIP(hipMemsetAsync(mx.get(), 0, sizeof(int), c.stream));
+    const int nblk = (A.n + BS - 1) / BS;
+    k_block_nnz<<<blocks_for(nblk), BS, 0, c.stream>>>(A.n, A.row.get(), mx.get());
+    BCMG_CHECK_LAUNCH();
+    BCMG_HIP(hipMemcpyAsync(c.h_ints, mx.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+    A.tile = c.h_ints[0] > 0 ? c.h_ints[0] : 1;
 }
 
 void launch_rows(hipStream_t s, const DevCsr &A, RowOp op, const double *x, const double *f,

@@ -160,6 +160,10 @@ int bcmg_precond_apply(bcmg_ctx ctx, bcmg_boot B, const double *r, double *x);
    read (and reset) with bcmg_smoother_timing. */
 int bcmg_time_smoother(bcmg_ctx ctx, int enable);
 int bcmg_smoother_timing(bcmg_ctx ctx, double *total_ms, int *launches);
+/* `reps` back-to-back launches of y = A x on the context stream between two
+   HIP events; avg_ms is the mean time per launch. */
+int bcmg_time_spmv(bcmg_ctx ctx, bcmg_csr A, const double *x, double *y, int reps,
+                   double *avg_ms);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,122 @@
+"""Host-side checks that need no GPU: the settings parser (product and
+oracle agree, error cases), the exported C-ABI, the facade header."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import bootcmatchg_amd as bc
+from oracle import oracle as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+FIELDS = [f for f, _ in bc.Params._fields_]
+
+
+def write_settings(path, values, comments=True):
+    with open(path, "w") as f:
+        for v in values:
+            f.write(("%s    %% comment after the value\n" % v) if comments else ("%s\n" % v))
+
+
+def params_equal(p, q):
+    return all(getattr(p, f) == getattr(q, f) for f in FIELDS)
+
+
+@pytest.mark.parametrize("fname", ["settings_c0.txt", "settings_c2.txt", "settings_c3.txt"])
+def test_settings_files_parse_identically(fname):
+    path = os.path.join(GOLDEN, fname)
+    p = bc.params_from_file(path)
+    rc, q = orc.params_from_file(path)
+    assert rc == 0 and params_equal(p, q)
+    assert p.matchtype == 3
+
+
+def test_settings_values_and_layout(tmp_path):
+    vals = ["2", "3", "0.5", "2", "0", "39", "1", "0", "4", "16", "4", "3", "2000", "1.e-8"]
+    path = str(tmp_path / "s")
+    write_settings(path, vals)
+    p = bc.params_from_file(path)
+    assert (p.solver_type, p.max_hrc, p.conv_ratio, p.aggrsweeps, p.aggrtype, p.max_levels,
+            p.cycle_type, p.coarse_solver, p.relax_type, p.relaxnumber_coarse,
+            p.prerelax_sweeps, p.postrelax_sweeps, p.itnlim, p.rtol) == \
+        (2, 3, 0.5, 2, 0, 39, 1, 0, 4, 16, 4, 3, 2000, 1e-8)
+
+
+@pytest.mark.parametrize("line,value", [(0, "3"), (4, "2"), (6, "3"), (7, "1"), (8, "5")])
+def test_settings_out_of_range(tmp_path, line, value):
+    vals = ["0", "1", "0.8", "2", "0", "39", "0", "4", "4", "16", "4", "4", "2000", "1e-6"]
+    vals[line] = value
+    path = str(tmp_path / "s")
+    write_settings(path, vals, comments=False)
+    with pytest.raises(bc.BcmgError) as e:
+        bc.params_from_file(path)
+    assert e.value.code == bc.ESETTINGS
+    assert orc.params_from_file(path)[0] == -2
+
+
+def test_settings_missing_and_short(tmp_path):
+    with pytest.raises(bc.BcmgError) as e:
+        bc.params_from_file(str(tmp_path / "nope"))
+    assert e.value.code == bc.ESETTINGS
+    path = str(tmp_path / "short")
+    write_settings(path, ["0", "1", "0.8"])
+    with pytest.raises(bc.BcmgError):
+        bc.params_from_file(path)
+    assert orc.params_from_file(path)[0] == -2
+
+
+def test_smoothed_aggregation_is_parsed_then_rejected_at_setup(tmp_path):
+    vals = ["0", "1", "0.8", "2", "1", "39", "0", "4", "4", "16", "4", "4", "2000", "1e-6"]
+    path = str(tmp_path / "s")
+    write_settings(path, vals)
+    p = bc.params_from_file(path)
+    assert p.aggrtype == 1
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "bcmg.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\**\s*(bcmg_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 30
+    L = bc.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", bc.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(l.split()[-1] for l in out.splitlines() if l.strip())
+    assert set(names) <= exported
+
+
+def test_no_device_is_reported_not_faked():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    with pytest.raises(bc.BcmgError) as e:
+        bc.Context(0)
+    assert e.value.code == bc.ENODEV
+
+
+def test_facade_header_compiles_against_the_reference_usage(tmp_path):
+    """The reference's solve_example body compiles against BootCMatchG.h."""
+    src = os.path.join(ROOT, "bootcmatchg_amd", "csrc", "solve_example.cpp")
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-I",
+                    os.path.join(ROOT, "include"), src], check=True)
+    subprocess.run(["gcc", "-std=c99", "-fsyntax-only", "-x", "c",
+                    os.path.join(ROOT, "include", "bcmg.h")], check=True)
+
+
+def test_operator_generators_are_deterministic():
+    from bootcmatchg_amd import operators as ops
+    a = ops.lognormal3d(5, 4.0, seed=1)
+    b = ops.lognormal3d(5, 4.0, seed=1)
+    c = ops.lognormal3d(5, 4.0, seed=2)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    assert not np.array_equal(a[2], c[2])
