@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <unordered_map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -39,31 +41,67 @@ void set_last_error(const std::string &msg);
 
 #define BCMG_CHECK_LAUNCH() BCMG_HIP(hipGetLastError())
 
-/* ------------------------------------------------------- device buffer */
+/* ------------------------------------------------------- device memory */
+
+/* Caching allocator of one context. Blocks are kept per size class (four
+   classes per power of two) and handed out again in stream order: every
+   kernel of a context runs on its one stream, so a block released while a
+   kernel still reads it is only reused by work queued after that kernel.
+   hipMalloc/hipFree of large buffers cost milliseconds and hipFree
+   synchronises the device; the setup allocates and drops hundreds of
+   temporaries, so it goes through the pool. */
+struct Pool {
+    std::multimap<size_t, void *> free_;
+    std::unordered_map<void *, size_t> cls_;
+    size_t cached = 0, live = 0;
+    static size_t size_class(size_t bytes);
+    void *alloc(size_t bytes);
+    void release(void *p);
+    void trim();
+    ~Pool();
+};
+
+/* Allocations made on this thread while a PoolScope is active come from its
+   pool (the C-ABI opens one per call with the context's pool). */
+Pool *current_pool();
+struct PoolScope {
+    Pool *prev;
+    explicit PoolScope(Pool *p);
+    ~PoolScope();
+};
+
+void *dev_alloc(Pool *pool, size_t bytes);
+void dev_free(Pool *pool, void *p);
 
 template <typename T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    Pool *pool = nullptr;
     DBuf() = default;
     explicit DBuf(size_t count) { alloc(count); }
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), pool(o.pool) { o.p = nullptr; o.n = 0; }
     DBuf &operator=(DBuf &&o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n; pool = o.pool;
+            o.p = nullptr; o.n = 0;
+        }
         return *this;
     }
     ~DBuf() { release(); }
     void alloc(size_t count) {
         release();
         if (count == 0) count = 1;
-        BCMG_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+        pool = current_pool();
+        p = static_cast<T *>(dev_alloc(pool, count * sizeof(T)));
         n = count;
     }
     void ensure(size_t count) { if (count > n || !p) alloc(count); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) dev_free(pool, p);
         p = nullptr;
         n = 0;
     }
@@ -79,7 +117,8 @@ struct DevCsr {
     DBuf<int> row, col;
     DBuf<double> val;
     int group = 8;          /* lanes per row of the row-group kernels */
-    int tile = 0;           /* max nnz of a 256-row block; 0 = not analysed */
+    int tile = 0;           /* max nnz of a 256-row block; 0 = not analysed, -1 pending */
+    DBuf<int> tile_dev;     /* device copy while pending */
 };
 
 /* Lanes per row for an average row length (a power of two in [1, 64]). */
@@ -87,11 +126,15 @@ int lanes_for(int n, long long nnz);
 struct Ctx;
 /* choose the row kernel of A (LDS row blocks when a block's nnz fits) */
 void analyze_rows(Ctx &c, DevCsr &A);
+/* queue the tile computation of A (A.tile = -1 until resolve_tiles) */
+void launch_block_nnz(Ctx &c, DevCsr &A);
+void resolve_tiles(Ctx &c, const std::vector<DevCsr *> &mats);
 
 /* ------------------------------------------------------------- context */
 
 struct Ctx {
     int device = 0;
+    Pool pool;
     hipStream_t stream = nullptr;
     /* pinned host scalars used for the few device->host reads */
     double *h_scalars = nullptr;
@@ -180,7 +223,7 @@ void launch_abs_rowsum(hipStream_t s, const DevCsr &A, double *M);
 /* W aligned with A's nnz (diag slots 0) */
 void match_weights(Ctx &c, const DevCsr &A, const double *w, double *W);
 /* suitor + symmetrisation: Msym (and M raw); returns number of non-mutual v */
-int match_suitor(Ctx &c, const DevCsr &A, const double *W, int *M, int *Msym);
+int match_suitor(Ctx &c, const DevCsr &A, const double *W, int *M, int *Msym, bool want_count);
 /* canonical pairwise aggregation; returns n_c. members[2*I], [2*I+1] (or -1) */
 int aggregate_pairs(Ctx &c, int n, const int *Msym, const double *w, int *agg, double *p,
                     int *members);
@@ -197,7 +240,7 @@ void transpose_P(Ctx &c, int n, int nc, const int *agg, const double *p, DevCsr 
 /* members of a given pairwise aggregation (throws if not pairwise) */
 void members_from_agg(Ctx &c, int n, const int *agg, int nc, int *members);
 /* exclusive scan of int32 counts into out[0..n], out[n] = total; returns total */
-long long exclusive_scan(Ctx &c, const int *in, int *out, int n);
+long long exclusive_scan(Ctx &c, const int *in, int *out, int n, bool read_total = true);
 
 /* ------------------------------------------------------------ host algo */
 void build_boot(Ctx &c, DevCsr &A, const bcmg_params &p, Boot &B);

@@ -60,7 +60,11 @@ using namespace bcmg;
 
 namespace {
 
-void use_device(bcmg_ctx ctx) { BCMG_HIP(hipSetDevice(ctx->c.device)); }
+/* device of the context current, its pool serving the call's allocations */
+struct CtxScope {
+    bcmg::PoolScope pool;
+    explicit CtxScope(bcmg_ctx ctx) : pool(&ctx->c.pool) { BCMG_HIP(hipSetDevice(ctx->c.device)); }
+};
 
 void check_csr_host(int n, int nnz, const int *row, const int *col)
 {
@@ -198,7 +202,7 @@ int bcmg_ctx_create(int device, bcmg_ctx *out)
         BCMG_HIP(hipSetDevice(device));
         BCMG_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
         BCMG_HIP(hipHostMalloc((void **)&ctx->c.h_scalars, 64 * sizeof(double), hipHostMallocDefault));
-        BCMG_HIP(hipHostMalloc((void **)&ctx->c.h_ints, 64 * sizeof(int), hipHostMallocDefault));
+        BCMG_HIP(hipHostMalloc((void **)&ctx->c.h_ints, 256 * sizeof(int), hipHostMallocDefault));
         ctx->c.red.alloc(1024 * 3 + 16);
     } catch (...) {
         delete ctx;
@@ -212,7 +216,7 @@ int bcmg_ctx_destroy(bcmg_ctx ctx)
 {
     if (!ctx) return BCMG_OK;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     (void)hipStreamSynchronize(ctx->c.stream);
     delete ctx;
     API_END
@@ -222,7 +226,7 @@ int bcmg_sync(bcmg_ctx ctx)
 {
     if (!ctx) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
     API_END
 }
@@ -231,7 +235,7 @@ int bcmg_dmalloc(bcmg_ctx ctx, size_t bytes, void **ptr)
 {
     if (!ctx || !ptr) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     *ptr = nullptr;
     if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
         (void)hipGetLastError();
@@ -244,7 +248,7 @@ int bcmg_dfree(bcmg_ctx ctx, void *ptr)
 {
     if (!ctx) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
     if (ptr) BCMG_HIP(hipFree(ptr));
     API_END
@@ -254,7 +258,7 @@ int bcmg_memcpy_h2d(bcmg_ctx ctx, void *dst, const void *src, size_t bytes)
 {
     if (!ctx || (bytes && (!dst || !src))) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     if (bytes) {
         BCMG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
         BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
@@ -266,7 +270,7 @@ int bcmg_memcpy_d2h(bcmg_ctx ctx, void *dst, const void *src, size_t bytes)
 {
     if (!ctx || (bytes && (!dst || !src))) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     d2h(ctx, dst, src, bytes);
     API_END
 }
@@ -275,7 +279,7 @@ int bcmg_memcpy_d2d(bcmg_ctx ctx, void *dst, const void *src, size_t bytes)
 {
     if (!ctx || (bytes && (!dst || !src))) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     if (bytes) {
         BCMG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->c.stream));
         BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
@@ -290,7 +294,7 @@ int bcmg_csr_upload(bcmg_ctx ctx, int n, int nnz, const int *row, const int *col
     *out = nullptr;
     API_BEGIN
     check_csr_host(n, nnz, row, col);
-    use_device(ctx);
+    CtxScope scope_(ctx);
     auto *h = new bcmg_csr_s();
     try {
         h->ctx = ctx;
@@ -318,7 +322,7 @@ int bcmg_csr_download(bcmg_csr A, int *row, int *col, double *val)
 {
     if (!A) return BCMG_EINVAL;
     API_BEGIN
-    use_device(A->ctx);
+    CtxScope scope_(A->ctx);
     d2h(A->ctx, row, A->A.row.get(), sizeof(int) * (A->A.n + 1));
     d2h(A->ctx, col, A->A.col.get(), sizeof(int) * A->A.nnz);
     d2h(A->ctx, val, A->A.val.get(), sizeof(double) * A->A.nnz);
@@ -337,7 +341,7 @@ int bcmg_csr_free(bcmg_csr A)
 {
     if (!A) return BCMG_OK;
     API_BEGIN
-    use_device(A->ctx);
+    CtxScope scope_(A->ctx);
     BCMG_HIP(hipStreamSynchronize(A->ctx->c.stream));
     delete A;
     API_END
@@ -354,7 +358,7 @@ int bcmg_setup(bcmg_ctx ctx, bcmg_csr A, const bcmg_params *p, bcmg_boot *out)
         return rc;
     }
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     auto *h = new bcmg_boot_s();
     try {
         h->ctx = ctx;
@@ -372,7 +376,7 @@ int bcmg_boot_free(bcmg_boot B)
 {
     if (!B) return BCMG_OK;
     API_BEGIN
-    use_device(B->ctx);
+    CtxScope scope_(B->ctx);
     BCMG_HIP(hipStreamSynchronize(B->ctx->c.stream));
     delete B;
     API_END
@@ -382,7 +386,7 @@ int bcmg_fcg(bcmg_ctx ctx, bcmg_boot B, const double *b, double *x, int *iters, 
 {
     if (!ctx || !B || !b || !x) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     fcg_solve(ctx->c, B->B, b, x, iters, resid, nullptr);
     API_END
 }
@@ -450,7 +454,7 @@ int bcmg_level_A(bcmg_boot B, int k, int l, int *row, int *col, double *val)
 {
     API_BEGIN
     const Level &lv = get_level(B, k, l);
-    use_device(B->ctx);
+    CtxScope scope_(B->ctx);
     const DevCsr &A = lv.mat();
     d2h(B->ctx, row, A.row.get(), sizeof(int) * (A.n + 1));
     d2h(B->ctx, col, A.col.get(), sizeof(int) * A.nnz);
@@ -463,7 +467,7 @@ int bcmg_level_P(bcmg_boot B, int k, int l, int *agg, double *p)
     API_BEGIN
     const Level &lv = get_level(B, k, l);
     REQUIRE(l + 1 < B->B.H[k].num_levels(), "the coarsest level has no prolongator");
-    use_device(B->ctx);
+    CtxScope scope_(B->ctx);
     d2h(B->ctx, agg, lv.agg.get(), sizeof(int) * lv.mat().n);
     d2h(B->ctx, p, lv.p.get(), sizeof(double) * lv.mat().n);
     API_END
@@ -473,7 +477,7 @@ int bcmg_level_DM(bcmg_boot B, int k, int l, double *D, double *M, int *hasD, in
 {
     API_BEGIN
     const Level &lv = get_level(B, k, l);
-    use_device(B->ctx);
+    CtxScope scope_(B->ctx);
     if (hasD) *hasD = lv.hasD;
     if (hasM) *hasM = lv.hasM;
     if (lv.hasD) d2h(B->ctx, D, lv.D.get(), sizeof(double) * lv.mat().n);
@@ -485,7 +489,7 @@ int bcmg_spmv(bcmg_ctx ctx, bcmg_csr A, const double *x, double *y)
 {
     if (!ctx || !A || !x || !y) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     launch_rows(ctx->c.stream, A->A, OP_SPMV, x, nullptr, nullptr, y);
     BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
     API_END
@@ -495,7 +499,7 @@ int bcmg_residual(bcmg_ctx ctx, bcmg_csr A, const double *x, const double *f, do
 {
     if (!ctx || !A || !x || !f || !y) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     launch_rows(ctx->c.stream, A->A, OP_RESID, x, f, nullptr, y);
     BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
     API_END
@@ -506,7 +510,7 @@ int bcmg_relax(bcmg_ctx ctx, bcmg_csr A, const double *d, const double *f, doubl
 {
     if (!ctx || !A || !d || !f || !u || !u2 || sweeps < 0) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     double *x = u, *y = u2;
     for (int k = 0; k < sweeps; k++) {
         launch_rows(ctx->c.stream, A->A, OP_JACOBI, x, f, d, y);
@@ -521,7 +525,7 @@ int bcmg_weights(bcmg_ctx ctx, bcmg_csr A, const double *w, double *W)
 {
     if (!ctx || !A || !w || !W) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     match_weights(ctx->c, A->A, w, W);
     API_END
 }
@@ -531,8 +535,8 @@ int bcmg_matching(bcmg_ctx ctx, bcmg_csr A, const double *W, int *M_raw, int *M_
 {
     if (!ctx || !A || !W || !M_raw || !M_sym) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
-    const int nu = match_suitor(ctx->c, A->A, W, M_raw, M_sym);
+    CtxScope scope_(ctx);
+    const int nu = match_suitor(ctx->c, A->A, W, M_raw, M_sym, true);
     BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
     if (n_unsym) *n_unsym = nu;
     API_END
@@ -543,7 +547,7 @@ int bcmg_aggregate(bcmg_ctx ctx, int n, const int *M_sym, const double *w, int *
 {
     if (!ctx || n <= 0 || !M_sym || !w || !agg || !p || !n_coarse) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     DBuf<int> members(2 * (size_t)n);
     *n_coarse = aggregate_pairs(ctx->c, n, M_sym, w, agg, p, members.get());
     API_END
@@ -555,13 +559,14 @@ int bcmg_galerkin(bcmg_ctx ctx, bcmg_csr A, const int *agg, const double *p, int
     if (!ctx || !A || !agg || !p || n_coarse <= 0 || !Ac) return BCMG_EINVAL;
     *Ac = nullptr;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     DBuf<int> members(2 * (size_t)n_coarse);
     members_from_agg(ctx->c, A->A.n, agg, n_coarse, members.get());
     auto *h = new bcmg_csr_s();
     try {
         h->ctx = ctx;
         galerkin_pairwise(ctx->c, A->A, agg, p, members.get(), n_coarse, h->A);
+        BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
     } catch (...) {
         delete h;
         throw;
@@ -574,7 +579,7 @@ int bcmg_precond_apply(bcmg_ctx ctx, bcmg_boot B, const double *r, double *x)
 {
     if (!ctx || !B || !r || !x) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     precond_apply(ctx->c, B->B, r, x, false);
     BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
     API_END
@@ -592,7 +597,7 @@ int bcmg_smoother_timing(bcmg_ctx ctx, double *total_ms, int *launches)
 {
     if (!ctx) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
     double tot = 0.0;
     for (size_t i = 0; i + 1 < ctx->c.ev_used; i += 2) {
@@ -611,7 +616,7 @@ int bcmg_time_spmv(bcmg_ctx ctx, bcmg_csr A, const double *x, double *y, int rep
 {
     if (!ctx || !A || !x || !y || reps < 1 || !avg_ms) return BCMG_EINVAL;
     API_BEGIN
-    use_device(ctx);
+    CtxScope scope_(ctx);
     hipEvent_t e0, e1;
     BCMG_HIP(hipEventCreate(&e0));
     BCMG_HIP(hipEventCreate(&e1));

@@ -66,7 +66,7 @@ void matching_aggregation(Ctx &c, const bcmg_params &p, BuildState &bs, const De
         DBuf<double> W(Ai->nnz > 0 ? Ai->nnz : 1);
         match_weights(c, *Ai, wptr, W.get());
         DBuf<int> M(n), Ms(n);
-        match_suitor(c, *Ai, W.get(), M.get(), Ms.get());
+        match_suitor(c, *Ai, W.get(), M.get(), Ms.get(), false);
         W.release();
         DBuf<int> agg(n), members(2 * (size_t)n);
         DBuf<double> pv(n);
@@ -81,7 +81,6 @@ void matching_aggregation(Ctx &c, const bcmg_params &p, BuildState &bs, const De
         } else {
             compose_P(c.stream, n0, aggP.get(), pP.get(), agg.get(), pv.get());
         }
-        BCMG_HIP(hipStreamSynchronize(c.stream));
         const double coarse_ratio = (double)n / (double)nc;
         cur = std::move(Ac);
         Ai = &cur;
@@ -129,7 +128,9 @@ void adaptive_coarsening(Ctx &c, const bcmg_params &p, BuildState &bs, DevCsr &A
     }
     if (p.coarse_solver != p.relax_type)
         relax_prepare(c, H.L[level - 1], H.L[level - 1].mat(), p.coarse_solver);
-    BCMG_HIP(hipStreamSynchronize(c.stream));
+    std::vector<DevCsr *> pending;
+    for (int i = 0; i + 1 < level; i++) pending.push_back(&H.L[i].R);
+    resolve_tiles(c, pending);
     double cx = 0, cw = 0;
     for (int i = 0; i < level; i++) {
         cx += H.L[i].mat().nnz;

@@ -232,27 +232,47 @@ __global__ __launch_bounds__(BS) void k_suitor(int n, const int *__restrict__ rp
                                                const double *__restrict__ W,
                                                unsigned long long *S)
 {
-    const int v = blockIdx.x * BS + threadIdx.x;
-    if (v >= n) return;
-    int cur = v;
+    constexpr int SB = 8; /* neighbours examined per batch of loads */
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    /* blocks start in index order, so thread t takes vertex n-1-t: the high
+       ids (which win every tie) propose first and fewer offers get undone */
+    int cur = n - 1 - t;
     while (cur >= 0) {
         int best = -1, bj = -1;
         double bw = -1.0;
-        const int j1 = rp[cur + 1];
-        for (int j = rp[cur]; j < j1; j++) {
-            const double wv = W[j];
-            if (!(wv > 0.0)) continue;
-            const int y = ci[j];
-            if (y == cur) continue;
-            if (wv < bw || (wv == bw && y < best)) continue;
-            const unsigned long long sv =
-                __hip_atomic_load(&S[y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const double wy = key_w(sv, W);
-            const int uy = key_u(sv);
-            if (wv < wy || (wv == wy && cur < uy)) continue;
-            bw = wv;
-            best = y;
-            bj = j;
+        const int j0 = rp[cur], j1 = rp[cur + 1];
+        for (int jb = j0; jb < j1; jb += SB) {
+            /* issue all loads of the batch before any use: one round trip for
+               W and the columns, one for the neighbours' offers, one for the
+               weights of those offers */
+            double wv[SB];
+            int y[SB];
+            unsigned long long sv[SB];
+            double wy[SB];
+#pragma unroll
+            for (int k = 0; k < SB; k++) {
+                const int j = jb + k;
+                wv[k] = j < j1 ? W[j] : 0.0;
+                y[k] = j < j1 ? ci[j] : cur;
+            }
+#pragma unroll
+            for (int k = 0; k < SB; k++)
+                sv[k] = (wv[k] > 0.0 && y[k] != cur)
+                            ? __hip_atomic_load(&S[y[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                            : ~0ull;
+#pragma unroll
+            for (int k = 0; k < SB; k++) wy[k] = key_w(sv[k], W);
+#pragma unroll
+            for (int k = 0; k < SB; k++) {
+                const double w = wv[k];
+                if (!(w > 0.0) || y[k] == cur) continue;
+                if (w < bw || (w == bw && y[k] < best)) continue;
+                if (w < wy[k] || (w == wy[k] && cur < key_u(sv[k]))) continue;
+                bw = w;
+                best = y[k];
+                bj = jb + k;
+            }
         }
         if (best < 0) break;
         const unsigned long long mine =
@@ -315,11 +335,13 @@ __global__ __launch_bounds__(BS) void k_compact(int n, const int *__restrict__ f
     if (v < n && flag[v]) list[pos[v]] = v;
 }
 
-/* the non-mutual candidates in ascending v, one at a time */
-__global__ void k_unsym_serial(int cnt, const int *__restrict__ list, const int *__restrict__ M,
-                               int *__restrict__ Ms)
+/* the non-mutual candidates in ascending v, one at a time; the count is
+   read on the device (pos[n] of the scan), so no host round trip */
+__global__ void k_unsym_serial(const int *__restrict__ cntp, const int *__restrict__ list,
+                               const int *__restrict__ M, int *__restrict__ Ms)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int cnt = *cntp;
     for (int t = 0; t < cnt; t++) {
         const int v = list[t];
         const int u = M[v];
@@ -622,7 +644,7 @@ int group_for(double avg)
 
 /* ================================================================ host */
 
-long long exclusive_scan(Ctx &c, const int *in, int *out, int n)
+long long exclusive_scan(Ctx &c, const int *in, int *out, int n, bool read_total)
 {
     if (n <= 0) {
         BCMG_HIP(hipMemsetAsync(out, 0, sizeof(int), c.stream));
@@ -635,6 +657,7 @@ long long exclusive_scan(Ctx &c, const int *in, int *out, int n)
     k_scan_top<<<1, BS, 0, c.stream>>>(nb, boff);
     k_scan_apply<<<nb, BS, 0, c.stream>>>(n, in, boff, out);
     BCMG_CHECK_LAUNCH();
+    if (!read_total) return -1;
     BCMG_HIP(hipMemcpyAsync(c.h_ints, out + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     BCMG_HIP(hipStreamSynchronize(c.stream));
     return c.h_ints[0];
@@ -657,36 +680,33 @@ void match_weights(Ctx &c, const DevCsr &A, const double *w, double *W)
     const int n = A.n;
     DBuf<double> C(n);
     DBuf<unsigned long long> mb(1);
-    const unsigned long long init = 0x7FEFFFFFFFFFFFFFull;
-    BCMG_HIP(hipMemcpyAsync(mb.get(), &init, sizeof(init), hipMemcpyHostToDevice, c.stream));
+    k_fill_u64<<<1, BS, 0, c.stream>>>(1, 0x7FEFFFFFFFFFFFFFull, mb.get()); /* DBL_MAX */
     k_weights_C<<<blocks_for(n), BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(), w,
                                                      C.get());
     k_weights_AH<<<blocks_for(n), BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(), w,
                                                       C.get(), W, mb.get());
     k_weights_log<<<blocks_for(n), BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), mb.get(), W);
     BCMG_CHECK_LAUNCH();
-    BCMG_HIP(hipStreamSynchronize(c.stream));
 }
 
-int match_suitor(Ctx &c, const DevCsr &A, const double *W, int *M, int *Msym)
+int match_suitor(Ctx &c, const DevCsr &A, const double *W, int *M, int *Msym, bool want_count)
 {
     const int n = A.n;
     DBuf<unsigned long long> S(n);
-    DBuf<int> flag(n), pos(n + 1);
+    DBuf<int> flag(n), pos(n + 1), list(n);
     k_fill_u64<<<blocks_for(n), BS, 0, c.stream>>>(n, ~0ull, S.get());
     k_suitor<<<blocks_for(n), BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), W, S.get());
     k_suitor_extract<<<blocks_for(n), BS, 0, c.stream>>>(n, S.get(), M);
     k_sym_pairs<<<blocks_for(n), BS, 0, c.stream>>>(n, M, Msym, flag.get());
     BCMG_CHECK_LAUNCH();
-    const long long cnt = exclusive_scan(c, flag.get(), pos.get(), n);
-    if (cnt > 0) {
-        DBuf<int> list(cnt);
-        k_compact<<<blocks_for(n), BS, 0, c.stream>>>(n, flag.get(), pos.get(), list.get());
-        k_unsym_serial<<<1, 64, 0, c.stream>>>((int)cnt, list.get(), M, Msym);
-        BCMG_CHECK_LAUNCH();
-        BCMG_HIP(hipStreamSynchronize(c.stream));
-    }
-    return (int)cnt;
+    exclusive_scan(c, flag.get(), pos.get(), n, false);
+    k_compact<<<blocks_for(n), BS, 0, c.stream>>>(n, flag.get(), pos.get(), list.get());
+    k_unsym_serial<<<1, 64, 0, c.stream>>>(pos.get() + n, list.get(), M, Msym);
+    BCMG_CHECK_LAUNCH();
+    if (!want_count) return -1;
+    BCMG_HIP(hipMemcpyAsync(c.h_ints, pos.get() + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+    return c.h_ints[0];
 }
 
 int aggregate_pairs(Ctx &c, int n, const int *Msym, const double *w, int *agg, double *p,
@@ -699,7 +719,6 @@ int aggregate_pairs(Ctx &c, int n, const int *Msym, const double *w, int *agg, d
     k_agg_assign<<<blocks_for(n), BS, 0, c.stream>>>(n, Msym, w, flag.get(), ids.get(), agg, p,
                                                       members);
     BCMG_CHECK_LAUNCH();
-    BCMG_HIP(hipStreamSynchronize(c.stream));
     return (int)nc;
 }
 
@@ -733,8 +752,9 @@ void galerkin_pairwise(Ctx &c, const DevCsr &A, const int *agg, const double *p,
     DBuf<int> len(nc), stoff(nc + 1);
     k_stage_len<<<blocks_for(nc), BS, 0, c.stream>>>(nc, A.row.get(), members, len.get());
     BCMG_CHECK_LAUNCH();
-    const long long tot = exclusive_scan(c, len.get(), stoff.get(), nc);
-    if (tot != A.nnz) throw Error(BCMG_EINVAL, "galerkin: staged length differs from nnz");
+    /* the segments of all aggregates together hold every entry of A once */
+    exclusive_scan(c, len.get(), stoff.get(), nc, false);
+    const long long tot = A.nnz;
     DBuf<int> Jst(tot);
     DBuf<double> tst(tot);
     DBuf<unsigned char> first(tot);
@@ -748,15 +768,24 @@ void galerkin_pairwise(Ctx &c, const DevCsr &A, const int *agg, const double *p,
     BCMG_CHECK_LAUNCH();
     Ac.n = Ac.m = nc;
     Ac.row.alloc(nc + 1);
-    const long long nnzc = exclusive_scan(c, cnt.get(), Ac.row.get(), nc);
+    exclusive_scan(c, cnt.get(), Ac.row.get(), nc, false);
+    /* one round trip for the coarse nnz and the row-block tile */
+    launch_block_nnz(c, Ac);
+    BCMG_HIP(hipMemcpyAsync(c.h_ints, Ac.row.get() + nc, sizeof(int), hipMemcpyDeviceToHost,
+                            c.stream));
+    BCMG_HIP(hipMemcpyAsync(c.h_ints + 1, Ac.tile_dev.get(), sizeof(int), hipMemcpyDeviceToHost,
+                            c.stream));
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+    const long long nnzc = c.h_ints[0];
     Ac.nnz = (int)nnzc;
+    Ac.tile = c.h_ints[1] > 0 ? c.h_ints[1] : 1;
+    Ac.group = lanes_for(nc, nnzc);
     Ac.col.alloc(nnzc);
     Ac.val.alloc(nnzc);
     BCMG_GROUP_SWITCH(Gr, k_rap_fill, blocks_for((long long)nc * Gr), nc, A.row.get(), members, p,
                       stoff.get(), Jst.get(), tst.get(), first.get(), Ac.row.get(), Ac.col.get(),
                       Ac.val.get());
     BCMG_CHECK_LAUNCH();
-    analyze_rows(c, Ac);
 }
 
 void restrict_w(hipStream_t s, int nc, const int *members, const double *p, const double *w,
@@ -784,13 +813,14 @@ void transpose_P(Ctx &c, int n, int nc, const int *agg, const double *p, DevCsr 
     R.row.alloc(nc + 1);
     R.col.alloc(n);
     R.val.alloc(n);
-    exclusive_scan(c, cnt.get(), R.row.get(), nc);
+    exclusive_scan(c, cnt.get(), R.row.get(), nc, false);
     BCMG_HIP(hipMemcpyAsync(cursor.get(), R.row.get(), sizeof(int) * (nc + 1),
                             hipMemcpyDeviceToDevice, c.stream));
     k_tfill<<<blocks_for(n), BS, 0, c.stream>>>(n, agg, p, cursor.get(), R.col.get(), R.val.get());
     k_sort_rows<<<blocks_for(nc), BS, 0, c.stream>>>(nc, R.row.get(), R.col.get(), R.val.get());
     BCMG_CHECK_LAUNCH();
-    analyze_rows(c, R);
+    /* tile: read later together with the other pending tiles */
+    launch_block_nnz(c, R);
 }
 
 } // namespace bcmg

@@ -346,21 +346,37 @@ int lanes_for(int n, long long nnz)
     return G;
 }
 
-void analyze_rows(Ctx &c, DevCsr &A)
+void launch_block_nnz(Ctx &c, DevCsr &A)
 {
     A.group = lanes_for(A.n, A.nnz);
-    if (A.n <= 0) {
-        A.tile = 0;
-        return;
-    }
-    DBuf<int> mx(1);
-    BCMG_HIP(hipMemsetAsync(mx.get(), 0, sizeof(int), c.stream));
+    A.tile_dev.ensure(1);
+    BCMG_HIP(hipMemsetAsync(A.tile_dev.get(), 0, sizeof(int), c.stream));
     const int nblk = (A.n + BS - 1) / BS;
-    k_block_nnz<<<blocks_for(nblk), BS, 0, c.stream>>>(A.n, A.row.get(), mx.get());
+    if (A.n > 0)
+        k_block_nnz<<<blocks_for(nblk), BS, 0, c.stream>>>(A.n, A.row.get(), A.tile_dev.get());
     BCMG_CHECK_LAUNCH();
-    BCMG_HIP(hipMemcpyAsync(c.h_ints, mx.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    BCMG_HIP(hipStreamSynchronize(c.stream));
-    A.tile = c.h_ints[0] > 0 ? c.h_ints[0] : 1;
+    A.tile = -1; /* pending until resolve_tiles */
+}
+
+void resolve_tiles(Ctx &c, const std::vector<DevCsr *> &mats)
+{
+    for (size_t b = 0; b < mats.size(); b += 64) {
+        const size_t e = std::min(mats.size(), b + 64);
+        for (size_t k = b; k < e; k++)
+            BCMG_HIP(hipMemcpyAsync(c.h_ints + 64 + (k - b), mats[k]->tile_dev.get(), sizeof(int),
+                                    hipMemcpyDeviceToHost, c.stream));
+        BCMG_HIP(hipStreamSynchronize(c.stream));
+        for (size_t k = b; k < e; k++) {
+            const int t = c.h_ints[64 + (k - b)];
+            mats[k]->tile = t > 0 ? t : 1;
+        }
+    }
+}
+
+void analyze_rows(Ctx &c, DevCsr &A)
+{
+    launch_block_nnz(c, A);
+    resolve_tiles(c, {&A});
 }
 
 void launch_rows(hipStream_t s, const DevCsr &A, RowOp op, const double *x, const double *f,
