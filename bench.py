@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every FCG kernel from the host instead of replaying "
+                         "captured HIP graphs (used for kernel-trace profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path has no multi-GPU form here "
@@ -67,6 +70,8 @@ def main():
     prm = bc.make_params(**SETTINGS)
 
     ctx = bc.Context(0)
+    if args.no_graphs:
+        bc.use_graphs(ctx, False)
     dA = bc.Csr(ctx, A)
     db = bc.DeviceArray.from_numpy(ctx, b)
     dx = bc.DeviceArray(ctx, n)

@@ -340,6 +340,12 @@ def aggregate(ctx, n, Msym_dev, w_dev, agg_dev, p_dev):
     return nc.value
 
 
+def use_graphs(ctx, enable):
+    """Replay FCG iterations from captured HIP graphs (default) or launch
+    every kernel from the host."""
+    _check(lib().bcmg_use_graphs(ctx.h, 1 if enable else 0))
+
+
 def time_smoother(ctx, enable):
     """Bracket every level-0 Jacobi sweep (from a non-zero iterate) with
     HIP events until disabled; read with smoother_timing()."""

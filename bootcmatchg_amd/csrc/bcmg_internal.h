@@ -118,7 +118,8 @@ struct DevCsr {
     DBuf<double> val;
     int group = 8;          /* lanes per row of the row-group kernels */
     int tile = 0;           /* max nnz of a 256-row block; 0 = not analysed, -1 pending */
-    DBuf<int> tile_dev;     /* device copy while pending */
+    int maxrow = 0;         /* longest row (valid with tile) */
+    DBuf<int> tile_dev;     /* device copy while pending: {tile, maxrow} */
 };
 
 /* Lanes per row for an average row length (a power of two in [1, 64]). */
@@ -144,9 +145,13 @@ struct Ctx {
     DBuf<int> iscratch;
     /* timing of the dominant kernel (optional) */
     bool time_smoother = false;
+    bool use_graphs = true;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     hipEvent_t next_event();
+    /* grow the reduction scratch for an n-row operator (outside any graph
+       capture) */
+    void ensure_reductions(int n);
     ~Ctx();
 };
 
@@ -189,7 +194,16 @@ struct Boot {
     DBuf<double> scal;          /* FCG scalars on device */
     /* setup timing (ms) */
     double t_setup_ms = 0;
+    /* captured FCG iterations (even, odd) and the output vector they use */
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    const double *graph_x = nullptr;
+    const double *graph_red = nullptr;
+    void drop_graphs();
     void ensure_work();
+    Boot() = default;
+    Boot(const Boot &) = delete;
+    Boot &operator=(const Boot &) = delete;
+    ~Boot();
 };
 
 /* ----------------------------------------------------- launch wrappers */
@@ -213,9 +227,19 @@ void launch_fcg_scalars(hipStream_t s, double *scal);
 /* z = -theta zo + z; x += theta2 z; t = -theta to + t; r += -theta2 t; scal[6] = ||r|| */
 void launch_fcg_update(Ctx &c, int n, double *scal, const double *zo, double *z, double *x,
                        const double *to, double *t, double *r);
-/* first FCG step: x += theta v; r -= theta w; ||r|| */
+/* first FCG step: x += theta v; r -= theta w; ||r|| (theta from scal[4]) */
 void launch_fcg_first(Ctx &c, int n, double *scal, const double *v, double *x,
                       const double *w, double *r);
+/* t = A z with the FCG products of z (k = 2: r.z, t.z; k = 3: also to.z) and
+   the scalar recurrence, all on the device */
+void launch_spmv_dots(Ctx &c, const DevCsr &A, int k, const double *z, double *t,
+                      const double *r, const double *to, double *scal);
+/* doubles of reduction scratch needed for an operator of n rows */
+size_t red_size_for(int n);
+/* all sweeps of a small level in one workgroup (when relax_block_ok(A)) */
+bool relax_block_ok(const DevCsr &A);
+void launch_relax_block(hipStream_t s, const DevCsr &A, const double *d, const double *f,
+                        const double *xin, double *xout, int sweeps, bool zero);
 
 /* setup-phase kernels (kernels_setup.hip) */
 void launch_diag(hipStream_t s, const DevCsr &A, double *D);

@@ -203,7 +203,8 @@ int bcmg_ctx_create(int device, bcmg_ctx *out)
         BCMG_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
         BCMG_HIP(hipHostMalloc((void **)&ctx->c.h_scalars, 64 * sizeof(double), hipHostMallocDefault));
         BCMG_HIP(hipHostMalloc((void **)&ctx->c.h_ints, 256 * sizeof(int), hipHostMallocDefault));
-        ctx->c.red.alloc(1024 * 3 + 16);
+        ctx->c.ensure_reductions(1);
+        BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
     } catch (...) {
         delete ctx;
         throw;
@@ -583,6 +584,13 @@ int bcmg_precond_apply(bcmg_ctx ctx, bcmg_boot B, const double *r, double *x)
     precond_apply(ctx->c, B->B, r, x, false);
     BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
     API_END
+}
+
+int bcmg_use_graphs(bcmg_ctx ctx, int enable)
+{
+    if (!ctx) return BCMG_EINVAL;
+    ctx->c.use_graphs = enable != 0;
+    return BCMG_OK;
 }
 
 int bcmg_time_smoother(bcmg_ctx ctx, int enable)

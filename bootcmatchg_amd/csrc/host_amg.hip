@@ -159,37 +159,45 @@ struct CycleRun {
 
 const double *divisor(const Level &lv, int type) { return type == 4 ? lv.M.get() : lv.D.get(); }
 
-/* `sweeps` Jacobi sweeps on level lv, ping-pong between st.x and st.x2;
-   a sweep from a known-zero x is f / d (the SpMV of a zero vector is skipped) */
+/* `sweeps` Jacobi sweeps on level lv (src/AMG/relaxation.cu:211-285),
+   ping-pong between st.x and st.x2. A sweep from a known-zero x is f / d
+   (the SpMV of a zero vector is skipped; same values). A small level runs
+   all its sweeps in one workgroup (same values, one launch). */
 void relax_level(Ctx &c, const Level &lv, int type, int sweeps, const double *f, LevelState &s,
                  double *final_out, bool level0)
 {
     const DevCsr &A = lv.mat();
     const double *d = divisor(lv, type);
+    if (sweeps == 0) return;
+    const bool timing = level0 && c.time_smoother;
+    if (relax_block_ok(A) && !timing) {
+        double *out = (final_out && s.x != final_out) ? final_out : s.x;
+        launch_relax_block(c.stream, A, d, f, s.x, out, sweeps, s.zero);
+        if (out != s.x) {
+            s.x2 = s.x;
+            s.x = out;
+        }
+        s.zero = false;
+        return;
+    }
     for (int k = 0; k < sweeps; k++) {
         double *out = s.x2;
         if (final_out && k == sweeps - 1 && s.x != final_out) out = final_out;
-        const bool timed = level0 && c.time_smoother && !s.zero;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (timed) {
-            e0 = c.next_event();
-            e1 = c.next_event();
-            BCMG_HIP(hipEventRecord(e0, c.stream));
-        }
-        if (s.zero)
+        if (s.zero) {
             launch_jacobi_zero(c.stream, A.n, f, d, out);
-        else
-            launch_rows(c.stream, A, OP_JACOBI, s.x, f, d, out);
-        if (timed) BCMG_HIP(hipEventRecord(e1, c.stream));
-        s.zero = false;
-        if (out == s.x2) {
-            s.x2 = s.x;
-            s.x = out;
         } else {
-            /* wrote straight into final_out: the old x becomes scratch */
-            s.x2 = s.x;
-            s.x = out;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (timing) {
+                e0 = c.next_event();
+                e1 = c.next_event();
+                BCMG_HIP(hipEventRecord(e0, c.stream));
+            }
+            launch_rows(c.stream, A, OP_JACOBI, s.x, f, d, out);
+            if (timing) BCMG_HIP(hipEventRecord(e1, c.stream));
         }
+        s.x2 = s.x;
+        s.x = out;
+        s.zero = false;
     }
 }
 
@@ -247,6 +255,17 @@ void run_hierarchy(Ctx &c, Boot &B, int k, std::vector<LevelState> &st, double *
 }
 
 } // namespace
+
+void Boot::drop_graphs()
+{
+    for (auto &g : gexec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    graph_x = nullptr;
+}
+
+Boot::~Boot() { drop_graphs(); }
 
 void Boot::ensure_work()
 {
@@ -380,6 +399,7 @@ void build_boot(Ctx &c, DevCsr &A, const bcmg_params &p, Boot &B)
     B.q.ensure(n);
     B.r.ensure(n);
     B.scal.ensure(16);
+    c.ensure_reductions(n);
     BCMG_HIP(hipStreamSynchronize(c.stream));
     B.t_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -392,6 +412,8 @@ int fcg_solve(Ctx &c, Boot &B, const double *b, double *x, int *iters, double *r
     const int n = A.n;
     double *scal = B.scal.get();
     double *v = B.v.get(), *w = B.w.get(), *d = B.d.get(), *q = B.q.get(), *r = B.r.get();
+    /* scratch of the fused reductions; graphs hold its address */
+    c.ensure_reductions(n);
     /* x0 = 0: r = b - A 0 = b */
     launch_copy(c.stream, n, b, r);
     launch_fill(c.stream, n, 0.0, x);
@@ -406,31 +428,63 @@ int fcg_solve(Ctx &c, Boot &B, const double *b, double *x, int *iters, double *r
         throw Error(BCMG_EZERORHS, "initial residual is zero (the reference exits)");
     }
     precond_apply(c, B, r, v, true);
-    launch_rows(c.stream, A, OP_SPMV, v, nullptr, nullptr, w);
-    launch_dots(c, n, 2, r, w, nullptr, v, scal);
+    launch_spmv_dots(c, A, 2, v, w, r, nullptr, scal);
     launch_fcg_first(c, n, scal, v, x, w, r);
     /* d, q take the first direction; the next one goes to the other pair */
     std::swap(v, d);
     std::swap(w, q);
     const double rtol = B.prm.rtol;
     const int max_iter = B.prm.itnlim;
-    int iter = 0;
-    double l2 = 0.0;
-    do {
-        const bool even = (iter % 2) == 0;
+    /* one FCG iteration (src/SOLVER/FCG.cu:497-565): preconditioner into the
+       free direction slot, A z, the three products, the scalar recurrence,
+       both merged axpys and ||r||, which is copied to pinned host memory */
+    auto enqueue_iteration = [&](bool even) {
         double *z = even ? v : d;
         double *t = even ? w : q;
         const double *zo = even ? d : v;
         const double *to = even ? q : w;
         precond_apply(c, B, r, z, true);
-        launch_rows(c.stream, A, OP_SPMV, z, nullptr, nullptr, t);
-        launch_dots(c, n, 3, r, t, to, z, scal);
-        launch_fcg_scalars(c.stream, scal);
+        launch_spmv_dots(c, A, 3, z, t, r, to, scal);
         launch_fcg_update(c, n, scal, zo, z, x, to, t, r);
-        BCMG_HIP(hipMemcpyAsync(c.h_scalars, scal + 6, sizeof(double), hipMemcpyDeviceToHost,
+        BCMG_HIP(hipMemcpyAsync(c.h_scalars + 8, scal + 6, sizeof(double), hipMemcpyDeviceToHost,
                                 c.stream));
+    };
+    /* The iteration is the same launch sequence every time (two variants for
+       the alternating direction slots), so it is captured once per composite
+       and output vector as a HIP graph and replayed: the kernels then go out
+       without per-launch host work. Event timing of the smoother needs the
+       plain launches. */
+    const bool graphs = c.use_graphs && !c.time_smoother;
+    if (graphs && (B.graph_x != x || B.graph_red != c.red.get())) B.drop_graphs();
+    int iter = 0;
+    double l2 = 0.0;
+    do {
+        const bool even = (iter % 2) == 0;
+        if (graphs) {
+            hipGraphExec_t &ge = B.gexec[even ? 0 : 1];
+            if (!ge) {
+                hipGraph_t g = nullptr;
+                BCMG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    enqueue_iteration(even);
+                } catch (...) {
+                    (void)hipStreamEndCapture(c.stream, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                BCMG_HIP(hipStreamEndCapture(c.stream, &g));
+                const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                BCMG_HIP(e);
+                B.graph_x = x;
+                B.graph_red = c.red.get();
+            }
+            BCMG_HIP(hipGraphLaunch(ge, c.stream));
+        } else {
+            enqueue_iteration(even);
+        }
         BCMG_HIP(hipStreamSynchronize(c.stream));
-        l2 = c.h_scalars[0];
+        l2 = c.h_scalars[8];
         iter++;
     } while (l2 > rtol * delta0 && iter < max_iter);
     if (iters) *iters = iter + 1;
@@ -438,6 +492,8 @@ int fcg_solve(Ctx &c, Boot &B, const double *b, double *x, int *iters, double *r
     if (!std::isfinite(l2)) throw Error(BCMG_EBREAKDOWN, "non-finite residual");
     return BCMG_OK;
 }
+
+void Ctx::ensure_reductions(int n) { red.ensure(red_size_for(n)); }
 
 hipEvent_t Ctx::next_event()
 {

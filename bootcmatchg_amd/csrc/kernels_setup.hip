@@ -773,12 +773,13 @@ void galerkin_pairwise(Ctx &c, const DevCsr &A, const int *agg, const double *p,
     launch_block_nnz(c, Ac);
     BCMG_HIP(hipMemcpyAsync(c.h_ints, Ac.row.get() + nc, sizeof(int), hipMemcpyDeviceToHost,
                             c.stream));
-    BCMG_HIP(hipMemcpyAsync(c.h_ints + 1, Ac.tile_dev.get(), sizeof(int), hipMemcpyDeviceToHost,
-                            c.stream));
+    BCMG_HIP(hipMemcpyAsync(c.h_ints + 1, Ac.tile_dev.get(), 2 * sizeof(int),
+                            hipMemcpyDeviceToHost, c.stream));
     BCMG_HIP(hipStreamSynchronize(c.stream));
     const long long nnzc = c.h_ints[0];
     Ac.nnz = (int)nnzc;
     Ac.tile = c.h_ints[1] > 0 ? c.h_ints[1] : 1;
+    Ac.maxrow = c.h_ints[2];
     Ac.group = lanes_for(nc, nnzc);
     Ac.col.alloc(nnzc);
     Ac.val.alloc(nnzc);

@@ -21,7 +21,12 @@ namespace {
 
 constexpr int BS = 256;          /* threads per block */
 constexpr int RED_MAX_BLOCKS = 1024;
+/* layout of the context's reduction scratch (doubles) */
+constexpr size_t RED_UPD = 3 * RED_MAX_BLOCKS + 16;   /* update/norm partials */
+constexpr size_t RED_SPMV = 4 * RED_MAX_BLOCKS + 32;  /* SpMV+dots partials */
 constexpr int STREAM_TILE_MAX = 4096;  /* doubles of LDS per block (32 KB) */
+/* matrices above this size are streamed with non-temporal loads */
+constexpr size_t NT_BYTES = (size_t)192 << 20;
 
 /* Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
    Return the logical block index so that XCD x owns a contiguous range. */
@@ -34,11 +39,16 @@ __device__ __forceinline__ int xcd_block(int bid, int nb)
 
 /* Row-block kernel: block b owns rows [256 b, 256 b + 256). Phase 1 streams
    the block's slice of val/col with consecutive lanes on consecutive entries
-   (fully coalesced) and stores the products a_ij x_j in LDS; phase 2 gives one
-   row to each thread, which sums its products in column order and applies the
-   epilogue with coalesced loads of f, d, x and a coalesced store. The sum
-   order (products rounded, then added left to right) is the serial one. */
-template <int OP>
+   (fully coalesced; non-temporal when the matrix is larger than the 256 MB
+   last-level cache, so that the once-read stream does not push the gathered
+   x out of the caches, while a smaller matrix stays cached for the next
+   sweep) and stores the products a_ij x_j in LDS;
+   phase 2 gives one row to each thread, which sums its products in column
+   order and applies the epilogue. The row's own operands (row pointers, f,
+   d, x) are loaded before phase 1 so that their latency overlaps the stream.
+   The sum order (products rounded, then added left to right) is the serial
+   one. */
+template <int OP, bool NT>
 __global__ __launch_bounds__(BS) void k_rows_stream(int n, const int *__restrict__ rp,
                                                     const int *__restrict__ ci,
                                                     const double *__restrict__ av,
@@ -51,6 +61,19 @@ __global__ __launch_bounds__(BS) void k_rows_stream(int n, const int *__restrict
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int r0 = b * BS;
     const int r1 = min(r0 + BS, n);
+    const int r = r0 + threadIdx.x;
+    const bool own = r < r1;
+    int ra = 0, re = 0;
+    double fr = 0.0, dr = 1.0, xr = 0.0;
+    if (own) {
+        ra = rp[r];
+        re = rp[r + 1];
+        if (OP != OP_SPMV) fr = f[r];
+        if (OP == OP_JACOBI) {
+            dr = d[r];
+            xr = x[r];
+        }
+    }
     const int e0 = rp[r0];
     const int cnt = rp[r1] - e0;
     constexpr int U = 4;
@@ -61,8 +84,13 @@ __global__ __launch_bounds__(BS) void k_rows_stream(int n, const int *__restrict
         for (int u = 0; u < U; u++) {
             const int k = k0 + u * BS;
             if (k < cnt) {
-                c[u] = ci[e0 + k];
-                v[u] = av[e0 + k];
+                if (NT) {
+                    c[u] = __builtin_nontemporal_load(ci + e0 + k);
+                    v[u] = __builtin_nontemporal_load(av + e0 + k);
+                } else {
+                    c[u] = ci[e0 + k];
+                    v[u] = av[e0 + k];
+                }
             }
         }
 #pragma unroll
@@ -72,20 +100,19 @@ __global__ __launch_bounds__(BS) void k_rows_stream(int n, const int *__restrict
         }
     }
     __syncthreads();
-    const int r = r0 + threadIdx.x;
-    if (r >= r1) return;
-    const int a = rp[r] - e0, e = rp[r + 1] - e0;
+    if (!own) return;
     double t = 0.0;
-    for (int k = a; k < e; k++) t = t + prod[k];
+    for (int k = ra - e0; k < re - e0; k++) t = t + prod[k];
     if (OP == OP_SPMV)
         y[r] = t;
     else if (OP == OP_RESID)
-        y[r] = f[r] - t;
+        y[r] = fr - t;
     else
-        y[r] = (f[r] - t) / d[r] + x[r];
+        y[r] = (fr - t) / dr + xr;
 }
 
-/* largest nnz of any 256-row block (the LDS tile of k_rows_stream) */
+/* largest nnz of any 256-row block (the LDS tile of k_rows_stream) into
+   maxv[0], longest row into maxv[1] */
 __global__ __launch_bounds__(BS) void k_block_nnz(int n, const int *__restrict__ rp,
                                                   int *__restrict__ maxv)
 {
@@ -94,6 +121,9 @@ __global__ __launch_bounds__(BS) void k_block_nnz(int n, const int *__restrict__
     if (r0 >= n) return;
     const int r1 = (int)min(r0 + BS, (long long)n);
     atomicMax(maxv, rp[r1] - rp[r0]);
+    int m = 0;
+    for (int r = (int)r0; r < r1; r++) m = max(m, rp[r + 1] - rp[r]);
+    atomicMax(maxv + 1, m);
 }
 
 template <int G, int OP>
@@ -249,9 +279,149 @@ __global__ void k_fcg_scalars(double *scal)
     }
 }
 
-/* fused double_merged_axpy pair + ||r||^2 partials (src/SOLVER/FCG.cu:88-98,
-   536-558): z = -theta zo + z; x += theta2 z; t = -theta to + t; r += -theta2 t */
-__global__ __launch_bounds__(BS) void k_fcg_update(int n, const double *__restrict__ scal,
+/* first step (src/SOLVER/FCG.cu:475-478): delta = beta, theta = alpha / delta */
+__global__ void k_fcg_first_scalars(double *scal)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        scal[3] = scal[1];
+        scal[4] = scal[0] / scal[1];
+    }
+}
+
+/* Final pass of the FCG reductions (one block of RED_FINAL_BS threads): sums
+   the per-block partials part[b*K + k] in a fixed order and applies the
+   scalar recurrence, so the scalars never leave the device.
+   K = 2 (first step, src/SOLVER/FCG.cu:475-478): delta = beta, theta = alpha / delta.
+   K = 3 (src/SOLVER/FCG.cu:531-534): theta = gamma / delta,
+   delta = beta - gamma^2 / delta, theta2 = alpha / delta.
+   K = 1: scal[6] = sqrt(sum) (the residual norm). */
+constexpr int RED_FINAL_BS = 1024;
+
+template <int K>
+__global__ __launch_bounds__(RED_FINAL_BS) void k_fcg_final(int nb, const double *__restrict__ part,
+                                                           double *__restrict__ scal)
+{
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += RED_FINAL_BS) {
+#pragma unroll
+        for (int k = 0; k < K; k++) s[k] += part[(size_t)b * K + k];
+    }
+    __shared__ double sh[K][RED_FINAL_BS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s[k] += __shfl_down(s[k], o, 64);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) sh[k][wid] = s[k];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double res[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double t = 0.0;
+        for (int w = 0; w < RED_FINAL_BS / 64; w++) t += sh[k][w];
+        res[k] = t;
+    }
+    if (K == 1) {
+        scal[6] = sqrt(res[0]);
+    } else if (K == 2) {
+        scal[0] = res[0];
+        scal[1] = res[1];
+        scal[3] = res[1];
+        scal[4] = res[0] / res[1];
+    } else {
+        const double alpha = res[0], beta = res[1], gamma = res[K - 1];
+        double delta = scal[3];
+        scal[0] = alpha;
+        scal[1] = beta;
+        scal[2] = gamma;
+        const double theta = gamma / delta;
+        delta = beta - (gamma * gamma) / delta;
+        scal[3] = delta;
+        scal[4] = theta;
+        scal[5] = alpha / delta;
+    }
+}
+
+/* FCG SpMV fused with its inner products (src/SOLVER/FCG.cu:508-528 and the
+   triple inner product :6-83): t = A z, and per block the partial sums of
+   alpha = r.z, beta = t.z (K = 2, first step) and gamma = to.z (K = 3),
+   finished by k_fcg_final. The products use the z, r, to values the
+   epilogue loads anyway, so the separate triple-product pass is saved. */
+template <int K, bool NT>
+__global__ __launch_bounds__(BS) void k_spmv_dots(int n, const int *__restrict__ rp,
+                                                  const int *__restrict__ ci,
+                                                  const double *__restrict__ av,
+                                                  const double *__restrict__ z,
+                                                  double *__restrict__ t,
+                                                  const double *__restrict__ rv,
+                                                  const double *__restrict__ to,
+                                                  double *__restrict__ part)
+{
+    extern __shared__ double prod[];
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = b * BS;
+    const int r1 = min(r0 + BS, n);
+    const int r = r0 + threadIdx.x;
+    const bool own = r < r1;
+    int ra = 0, re = 0;
+    double zr = 0.0, rr = 0.0, tor = 0.0;
+    if (own) {
+        ra = rp[r];
+        re = rp[r + 1];
+        zr = z[r];
+        rr = rv[r];
+        if (K > 2) tor = to[r];
+    }
+    const int e0 = rp[r0];
+    const int cnt = rp[r1] - e0;
+    constexpr int U = 4;
+    for (int k0 = threadIdx.x; k0 < cnt; k0 += BS * U) {
+        int c[U];
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k0 + u * BS;
+            if (k < cnt) {
+                if (NT) {
+                    c[u] = __builtin_nontemporal_load(ci + e0 + k);
+                    v[u] = __builtin_nontemporal_load(av + e0 + k);
+                } else {
+                    c[u] = ci[e0 + k];
+                    v[u] = av[e0 + k];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k0 + u * BS;
+            if (k < cnt) prod[k] = v[u] * z[c[u]];
+        }
+    }
+    __syncthreads();
+    double sdot[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) sdot[k] = 0.0;
+    if (own) {
+        double tr = 0.0;
+        for (int k = ra - e0; k < re - e0; k++) tr = tr + prod[k];
+        t[r] = tr;
+        sdot[0] = rr * zr;
+        sdot[1] = tr * zr;
+        if (K > 2) sdot[K - 1] = tor * zr;
+    }
+    block_reduce_store<K>(sdot, part + (size_t)b * K);
+}
+
+/* fused double_merged_axpy pair + ||r|| (src/SOLVER/FCG.cu:88-98, 536-558):
+   z = -theta zo + z; x += theta2 z; t = -theta to + t; r += -theta2 t */
+__global__ __launch_bounds__(BS) void k_fcg_update(int n, double *__restrict__ scal,
                                                    const double *__restrict__ zo,
                                                    double *__restrict__ z,
                                                    double *__restrict__ x,
@@ -275,8 +445,7 @@ __global__ __launch_bounds__(BS) void k_fcg_update(int n, const double *__restri
     block_reduce_store<1>(s, part + blockIdx.x);
 }
 
-/* first FCG step (src/SOLVER/FCG.cu:475-486): delta = beta, theta = alpha/delta,
-   x += theta v, r -= theta w */
+/* first FCG step (src/SOLVER/FCG.cu:475-486): x += theta v, r -= theta w, ||r|| */
 __global__ __launch_bounds__(BS) void k_fcg_first(int n, double *__restrict__ scal,
                                                   const double *__restrict__ v,
                                                   double *__restrict__ x,
@@ -284,12 +453,7 @@ __global__ __launch_bounds__(BS) void k_fcg_first(int n, double *__restrict__ sc
                                                   double *__restrict__ r,
                                                   double *__restrict__ part)
 {
-    const double alpha = scal[0], beta = scal[1];
-    const double theta = alpha / beta;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        scal[3] = beta;
-        scal[4] = theta;
-    }
+    const double theta = scal[4];
     double s[1] = {0.0};
     for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
         x[i] = fma(theta, v[i], x[i]);
@@ -300,7 +464,79 @@ __global__ __launch_bounds__(BS) void k_fcg_first(int n, double *__restrict__ sc
     block_reduce_store<1>(s, part + blockIdx.x);
 }
 
+/* All `sweeps` Jacobi sweeps of a small level in one workgroup of 1024
+   threads: thread t owns rows t + 1024 q (q < RPT) and keeps their entries
+   (at most MAXNZ per row) in registers for all sweeps; both iterates live in
+   LDS and a barrier separates the sweeps. Row sums are sequential in column
+   order (the order of k_rows_stream), so the result equals `sweeps` separate
+   launches bit for bit. */
+template <int RPT, int MAXNZ>
+__global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restrict__ rp,
+                                                      const int *__restrict__ ci,
+                                                      const double *__restrict__ av,
+                                                      const double *__restrict__ d,
+                                                      const double *__restrict__ f,
+                                                      const double *__restrict__ xin,
+                                                      double *__restrict__ xout, int sweeps,
+                                                      int zero)
+{
+    extern __shared__ double xs[];
+    double *cur = xs, *nxt = xs + n;
+    double a[RPT][MAXNZ];
+    int cj[RPT][MAXNZ];
+    int len[RPT];
+    double fi[RPT], di[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int i = threadIdx.x + 1024 * q;
+        len[q] = 0;
+        fi[q] = 0.0;
+        di[q] = 1.0;
+        if (i < n) {
+            const int j0 = rp[i];
+            len[q] = rp[i + 1] - j0;
+            fi[q] = f[i];
+            di[q] = d[i];
+#pragma unroll
+            for (int k = 0; k < MAXNZ; k++) {
+                a[q][k] = k < len[q] ? av[j0 + k] : 0.0;
+                cj[q][k] = k < len[q] ? ci[j0 + k] : 0;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int i = threadIdx.x + 1024 * q;
+        if (i < n) cur[i] = zero ? fi[q] / di[q] : xin[i];
+    }
+    __syncthreads();
+    for (int s = zero ? 1 : 0; s < sweeps; s++) {
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int i = threadIdx.x + 1024 * q;
+            if (i < n) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < MAXNZ; k++)
+                    if (k < len[q]) t = t + a[q][k] * cur[cj[q][k]];
+                nxt[i] = (fi[q] - t) / di[q] + cur[i];
+            }
+        }
+        __syncthreads();
+        double *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int i = threadIdx.x + 1024 * q;
+        if (i < n) xout[i] = cur[i];
+    }
+}
+
 inline int blocks_for(long long work) { return (int)((work + BS - 1) / BS); }
+
+inline bool stream_ok(const DevCsr &A) { return A.tile > 0 && A.tile <= STREAM_TILE_MAX; }
 
 inline int red_blocks(int n)
 {
@@ -311,14 +547,27 @@ inline int red_blocks(int n)
 }
 
 template <int OP>
+void stream_dispatch(hipStream_t s, const DevCsr &A, const double *x, const double *f,
+                     const double *d, double *y)
+{
+    const int n = A.n;
+    const int nb = (n + BS - 1) / BS;
+    const size_t lds = (size_t)A.tile * sizeof(double);
+    if ((size_t)A.nnz * 12 > NT_BYTES)
+        k_rows_stream<OP, true><<<nb, BS, lds, s>>>(n, A.row.get(), A.col.get(), A.val.get(), x,
+                                                     f, d, y);
+    else
+        k_rows_stream<OP, false><<<nb, BS, lds, s>>>(n, A.row.get(), A.col.get(), A.val.get(), x,
+                                                      f, d, y);
+}
+
+template <int OP>
 void rows_dispatch(hipStream_t s, const DevCsr &A, const double *x, const double *f,
                    const double *d, double *y)
 {
     const int n = A.n;
-    if (A.tile > 0 && A.tile <= STREAM_TILE_MAX) {
-        const int nb = (n + BS - 1) / BS;
-        k_rows_stream<OP><<<nb, BS, (size_t)A.tile * sizeof(double), s>>>(
-            n, A.row.get(), A.col.get(), A.val.get(), x, f, d, y);
+    if (stream_ok(A)) {
+        stream_dispatch<OP>(s, A, x, f, d, y);
         return;
     }
     const int G = A.group;
@@ -349,8 +598,8 @@ int lanes_for(int n, long long nnz)
 void launch_block_nnz(Ctx &c, DevCsr &A)
 {
     A.group = lanes_for(A.n, A.nnz);
-    A.tile_dev.ensure(1);
-    BCMG_HIP(hipMemsetAsync(A.tile_dev.get(), 0, sizeof(int), c.stream));
+    A.tile_dev.ensure(2);
+    BCMG_HIP(hipMemsetAsync(A.tile_dev.get(), 0, 2 * sizeof(int), c.stream));
     const int nblk = (A.n + BS - 1) / BS;
     if (A.n > 0)
         k_block_nnz<<<blocks_for(nblk), BS, 0, c.stream>>>(A.n, A.row.get(), A.tile_dev.get());
@@ -363,12 +612,13 @@ void resolve_tiles(Ctx &c, const std::vector<DevCsr *> &mats)
     for (size_t b = 0; b < mats.size(); b += 64) {
         const size_t e = std::min(mats.size(), b + 64);
         for (size_t k = b; k < e; k++)
-            BCMG_HIP(hipMemcpyAsync(c.h_ints + 64 + (k - b), mats[k]->tile_dev.get(), sizeof(int),
-                                    hipMemcpyDeviceToHost, c.stream));
+            BCMG_HIP(hipMemcpyAsync(c.h_ints + 64 + 2 * (k - b), mats[k]->tile_dev.get(),
+                                    2 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
         BCMG_HIP(hipStreamSynchronize(c.stream));
         for (size_t k = b; k < e; k++) {
-            const int t = c.h_ints[64 + (k - b)];
+            const int t = c.h_ints[64 + 2 * (k - b)];
             mats[k]->tile = t > 0 ? t : 1;
+            mats[k]->maxrow = c.h_ints[64 + 2 * (k - b) + 1];
         }
     }
 }
@@ -464,9 +714,8 @@ void launch_fcg_update(Ctx &c, int n, double *scal, const double *zo, double *z,
                        const double *to, double *t, double *r)
 {
     const int nb = red_blocks(n);
-    c.red.ensure((size_t)RED_MAX_BLOCKS * 3);
-    k_fcg_update<<<nb, BS, 0, c.stream>>>(n, scal, zo, z, x, to, t, r, c.red.get());
-    k_reduce_final<1, true><<<1, BS, 0, c.stream>>>(nb, c.red.get(), scal + 6);
+    k_fcg_update<<<nb, BS, 0, c.stream>>>(n, scal, zo, z, x, to, t, r, c.red.get() + RED_UPD);
+    k_fcg_final<1><<<1, RED_FINAL_BS, 0, c.stream>>>(nb, c.red.get() + RED_UPD, scal);
     BCMG_CHECK_LAUNCH();
 }
 
@@ -474,9 +723,79 @@ void launch_fcg_first(Ctx &c, int n, double *scal, const double *v, double *x, c
                       double *r)
 {
     const int nb = red_blocks(n);
-    c.red.ensure((size_t)RED_MAX_BLOCKS * 3);
-    k_fcg_first<<<nb, BS, 0, c.stream>>>(n, scal, v, x, w, r, c.red.get());
-    k_reduce_final<1, true><<<1, BS, 0, c.stream>>>(nb, c.red.get(), scal + 6);
+    k_fcg_first<<<nb, BS, 0, c.stream>>>(n, scal, v, x, w, r, c.red.get() + RED_UPD);
+    k_fcg_final<1><<<1, RED_FINAL_BS, 0, c.stream>>>(nb, c.red.get() + RED_UPD, scal);
+    BCMG_CHECK_LAUNCH();
+}
+
+size_t red_size_for(int n) { return (size_t)RED_SPMV + 3 * (size_t)((n + BS - 1) / BS) + 64; }
+
+void launch_spmv_dots(Ctx &c, const DevCsr &A, int k, const double *z, double *t,
+                      const double *r, const double *to, double *scal)
+{
+    const int n = A.n;
+    if (!stream_ok(A)) {
+        /* long rows: separate SpMV, products and recurrence */
+        launch_rows(c.stream, A, OP_SPMV, z, nullptr, nullptr, t);
+        launch_dots(c, n, k, r, t, to, z, scal);
+        if (k == 2) {
+            k_fcg_first_scalars<<<1, 64, 0, c.stream>>>(scal);
+        } else {
+            k_fcg_scalars<<<1, 64, 0, c.stream>>>(scal);
+        }
+        BCMG_CHECK_LAUNCH();
+        return;
+    }
+    if (c.red.n < red_size_for(n))
+        throw Error(BCMG_EINVAL, "reduction scratch not sized for this operator");
+    const int nb = (n + BS - 1) / BS;
+    const size_t lds = (size_t)A.tile * sizeof(double);
+    double *part = c.red.get() + RED_SPMV;
+    const bool nt = (size_t)A.nnz * 12 > NT_BYTES;
+    const int *rp = A.row.get(), *ci = A.col.get();
+    const double *av = A.val.get();
+    if (k == 2) {
+        if (nt)
+            k_spmv_dots<2, true><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part);
+        else
+            k_spmv_dots<2, false><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part);
+        k_fcg_final<2><<<1, RED_FINAL_BS, 0, c.stream>>>(nb, part, scal);
+    } else {
+        if (nt)
+            k_spmv_dots<3, true><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part);
+        else
+            k_spmv_dots<3, false><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part);
+        k_fcg_final<3><<<1, RED_FINAL_BS, 0, c.stream>>>(nb, part, scal);
+    }
+    BCMG_CHECK_LAUNCH();
+}
+
+bool relax_block_ok(const DevCsr &A)
+{
+    if (A.maxrow <= 0) return false;
+    return (A.n <= 4096 && A.maxrow <= 8) || (A.n <= 2048 && A.maxrow <= 16) ||
+           (A.n <= 1024 && A.maxrow <= 32);
+}
+
+void launch_relax_block(hipStream_t s, const DevCsr &A, const double *d, const double *f,
+                        const double *xin, double *xout, int sweeps, bool zero)
+{
+    const size_t lds = (size_t)2 * A.n * sizeof(double);
+    const int z = zero ? 1 : 0;
+    if (A.n <= 1024 && A.maxrow <= 8)
+        k_relax_block<1, 8><<<1, 1024, lds, s>>>(A.n, A.row.get(), A.col.get(), A.val.get(), d,
+                                                  f, xin, xout, sweeps, z);
+    else if (A.n <= 4096 && A.maxrow <= 8)
+        k_relax_block<4, 8><<<1, 1024, lds, s>>>(A.n, A.row.get(), A.col.get(), A.val.get(), d,
+                                                  f, xin, xout, sweeps, z);
+    else if (A.n <= 2048 && A.maxrow <= 16)
+        k_relax_block<2, 16><<<1, 1024, lds, s>>>(A.n, A.row.get(), A.col.get(), A.val.get(), d,
+                                                   f, xin, xout, sweeps, z);
+    else if (A.n <= 1024 && A.maxrow <= 32)
+        k_relax_block<1, 32><<<1, 1024, lds, s>>>(A.n, A.row.get(), A.col.get(), A.val.get(), d,
+                                                   f, xin, xout, sweeps, z);
+    else
+        throw Error(BCMG_EINVAL, "relax_block: level too large");
     BCMG_CHECK_LAUNCH();
 }
 

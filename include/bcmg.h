@@ -154,6 +154,10 @@ int bcmg_galerkin(bcmg_ctx ctx, bcmg_csr A, const int *agg, const double *p, int
    the initial guess. Replaces preconditionApply (src/SOLVER/FCG.cu:112-187). */
 int bcmg_precond_apply(bcmg_ctx ctx, bcmg_boot B, const double *r, double *x);
 
+/* Replay the FCG iteration as a captured HIP graph (default on). Results are
+   identical either way; off runs every launch from the host. */
+int bcmg_use_graphs(bcmg_ctx ctx, int enable);
+
 /* ---- measurement -------------------------------------------------------- */
 /* When enabled, every fine-level (level 0) smoother sweep launched by
    bcmg_fcg is bracketed by HIP events on the context stream; the totals are

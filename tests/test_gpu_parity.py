@@ -229,3 +229,21 @@ def test_solve_host_api_and_errors(ctx):
     with pytest.raises(bc.BcmgError) as e:
         bc.solve(A, bc.make_params(aggrtype=1), np.ones(n))
     assert e.value.code == bc.EUNSUP
+
+
+def test_graph_replay_matches_eager_launches(ctx):
+    """FCG replayed from captured graphs gives the same bits as host launches,
+    over several solves (graph reuse) and a change of output vector."""
+    A = OPERATORS["lognormal3d_14"]()
+    n = len(A[0]) - 1
+    B = bc.Boot(ctx, bc.Csr(ctx, A), bc.make_params(**SETTINGS["V_L1_2sweeps"]))
+    b = dev(ctx, ops.uniform_pm1(9, n))
+    bc.use_graphs(ctx, False)
+    x_e = bc.DeviceArray(ctx, n)
+    it_e, r_e = B.fcg(b, x_e)
+    bc.use_graphs(ctx, True)
+    for rep in range(3):
+        x_g = bc.DeviceArray(ctx, n)
+        it_g, r_g = B.fcg(b, x_g)
+        assert it_g == it_e and r_g == r_e
+        assert np.array_equal(bits(x_g.numpy()), bits(x_e.numpy()))
