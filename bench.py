@@ -32,6 +32,13 @@ SETTINGS = dict(solver_type=0, max_hrc=1, conv_ratio=0.8, aggrsweeps=2, aggrtype
                 rtol=1e-8)
 
 
+# Measured HBM bytes per level-0 sweep launch at 160^3 (rocprofv3 FETCH_SIZE
+# x 1.999 + WRITE_SIZE x 1.000, factors from a plain copy kernel of known
+# size; summary: profiles/jacobi_sweep_poisson160_traffic.md). Re-measured
+# whenever the sweep kernel changes; reported only for the 160^3 workload.
+SWEEP_TRAFFIC_BYTES_160 = 502.8e6
+
+
 def sweep_bytes(n, nnz):
     """Algorithmic HBM bytes of one Jacobi/L1 sweep u' = u + (f - A u) / d:
     val + col (12 B/nnz), row pointer, u (gathered, counted once), f, d, u'."""
@@ -169,8 +176,9 @@ def main():
         "spmv_gbs": spmv_gbs,
         "spmv_frac_of_hbm": spmv_gbs / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "level-0 Jacobi/L1 sweep (k_rows<8, OP_JACOBI>)",
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": SWEEP_TRAFFIC_BYTES_160 if N == GRID else None,
+                     "kernel": "level-0 Jacobi/L1 sweep (k_rows_stream<OP_JACOBI, non-temporal>)",
                      "bytes_per_launch": sweep_b, "avg_launch_ms": sweep_avg_ms,
                      "launches": sm_launches},
         "cpu_baseline": cpu,
