@@ -7,10 +7,14 @@ src/SOLVER/solver.cu:20-129): 3D 7-point Poisson on a 160^3 grid
 from SplitMix64 stream 0, settings 0,1,0.8,2,0,39,0,4,4,16,4,4,2000,1e-8.
 The operator and b are resident in HBM before the timed region.
 
-Printed JSON line: value = solves/s of the whole step; extras give the
-setup/solve split, the rate with a cached hierarchy, the dominant kernel's
-roofline (level-0 Jacobi/L1 sweep, timed with HIP events inside the timed
-region) and the serial CPU oracle on the same workload.
+Printed JSON line: value = solves/s of the whole FP64 drop-in step; extras
+give the setup/solve split, the rate with a cached hierarchy, the dominant
+kernel's roofline (level-0 Jacobi/L1 sweep, timed with HIP events inside the
+timed region), the serial CPU oracle on the same workload, and the precision
+form: the same step with the hierarchy converted to FP32 (or bfloat16 values
+on levels >= 1) inside FP64 FCG with a true-residual check, on this workload
+and on the bootstrap composite workload (cell-coefficient diffusion on 128^3,
+k = exp(g), g ~ N(0, 4), max_hrc 3, rho 0.5).
 """
 import argparse
 import json
@@ -39,6 +43,81 @@ SETTINGS = dict(solver_type=0, max_hrc=1, conv_ratio=0.8, aggrsweeps=2, aggrtype
 SWEEP_TRAFFIC_BYTES_160 = 502.8e6
 
 
+C2_GRID = 128
+C2_SETTINGS = dict(SETTINGS, max_hrc=3, conv_ratio=0.5)
+PREC_NAMES = {0: "fp64", 1: "fp32", 2: "bf16"}
+
+
+def run_form(bc, ctx, dA, db, dx, prm, prec, steps, warmup):
+    """`steps` timed full solves (setup, precision conversion, FCG, free)
+    after `warmup` untimed ones, then `steps` FCG solves on one cached
+    hierarchy. Returns the figures and the last solution."""
+    lat, setup, solve = [], [], []
+    it = restarts = 0
+    for k in range(warmup + steps):
+        ctx.sync()
+        t0 = time.perf_counter()
+        B = bc.Boot(ctx, dA, prm)
+        if prec:
+            B.set_precision(prec)
+        t1 = time.perf_counter()
+        it, _ = B.fcg(db, dx)
+        t2 = time.perf_counter()
+        restarts = B.restarts()
+        B.free()
+        if k >= warmup:
+            lat.append((t2 - t0) * 1e3)
+            setup.append((t1 - t0) * 1e3)
+            solve.append((t2 - t1) * 1e3)
+    B = bc.Boot(ctx, dA, prm)
+    if prec:
+        B.set_precision(prec)
+    B.fcg(db, dx)
+    ctx.sync()
+    tc0 = time.perf_counter()
+    for _ in range(steps):
+        B.fcg(db, dx)
+    ctx.sync()
+    tc1 = time.perf_counter()
+    B.free()
+    lat = np.array(lat)
+    return {"solves_per_s": 1e3 / float(np.mean(lat)),
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "setup_ms": float(np.mean(setup)), "solve_ms": float(np.mean(solve)),
+            "iterations": it, "restarts": restarts,
+            "cached_hierarchy_solves_per_s": steps / (tc1 - tc0)}, dx.numpy()
+
+
+def precision_forms(bc, ctx, A, b, settings, steps, warmup, x_ref=None, ref_name=None):
+    """FP64, FP32 and BF16 forms on one workload; errors against x_ref (or
+    the FP64 GPU solution when x_ref is None)."""
+    from bootcmatchg_amd import operators as ops
+    n = len(A[0]) - 1
+    prm = bc.make_params(**settings)
+    dA = bc.Csr(ctx, A)
+    db = bc.DeviceArray.from_numpy(ctx, b)
+    dx = bc.DeviceArray(ctx, n)
+    S = ops.to_scipy(A)
+    nb = np.linalg.norm(b)
+    out = {}
+    xs = {}
+    for prec in (0, 1, 2):
+        r, x = run_form(bc, ctx, dA, db, dx, prm, prec, steps, warmup)
+        r["true_rel_residual"] = float(np.linalg.norm(b - S @ x) / nb)
+        xs[prec] = x
+        out[PREC_NAMES[prec]] = r
+    ref = x_ref if x_ref is not None else xs[0]
+    for prec in (0, 1, 2):
+        out[PREC_NAMES[prec]]["rel_err_vs_" + (ref_name or "fp64_gpu")] = float(
+            np.linalg.norm(xs[prec] - ref) / np.linalg.norm(ref))
+    for k in ("fp32", "bf16"):
+        out[k]["speedup_vs_fp64"] = out[k]["solves_per_s"] / out["fp64"]["solves_per_s"]
+        out[k]["cached_speedup_vs_fp64"] = (out[k]["cached_hierarchy_solves_per_s"]
+                                            / out["fp64"]["cached_hierarchy_solves_per_s"])
+    del dA, db, dx
+    return out
+
+
 def sweep_bytes(n, nnz):
     """Algorithmic HBM bytes of one Jacobi/L1 sweep u' = u + (f - A u) / d:
     val + col (12 B/nnz), row pointer, u (gathered, counted once), f, d, u'."""
@@ -57,6 +136,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the FP32/BF16 precision-form measurements")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every FCG kernel from the host instead of replaying "
                          "captured HIP graphs (used for kernel-trace profiling)")
@@ -139,7 +220,7 @@ def main():
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
         tcpu0 = time.perf_counter()
-        rc, xo, ores = orc.solve(A, b, orc.params(**SETTINGS))
+        rc, xo_cpu, ores = orc.solve(A, b, orc.params(**SETTINGS))
         tcpu1 = time.perf_counter()
         cpu = {"value": 1.0 / (tcpu1 - tcpu0), "unit": "solves/s", "cores": 1,
                "kind": "port",
@@ -147,7 +228,23 @@ def main():
                          "same 160^3 workload by the serial C++ oracle, 1 thread"
                          % (ores.setup_seconds, ores.solve_seconds, ores.iterations),
                "iterations": ores.iterations,
-               "solution_rel_diff_vs_gpu": float(np.linalg.norm(x - xo) / np.linalg.norm(xo))}
+               "solution_rel_diff_vs_gpu": float(np.linalg.norm(x - xo_cpu)
+                                                 / np.linalg.norm(xo_cpu))}
+
+    precision = None
+    if not args.no_precision:
+        xo = None if cpu is None else xo_cpu
+        precision = {"c1_poisson160": precision_forms(
+            bc, ctx, A, b, SETTINGS, args.steps, 1, x_ref=xo,
+            ref_name="fp64_oracle" if xo is not None else None)}
+        from bootcmatchg_amd import operators as ops2
+        A2 = ops2.lognormal3d(C2_GRID, 4.0, seed=0)
+        b2 = ops2.uniform_pm1(0, len(A2[0]) - 1)
+        precision["c2_lognormal128_boot3"] = precision_forms(
+            bc, ctx, A2, b2, C2_SETTINGS, max(2, args.steps // 2), 1)
+        precision["c2_lognormal128_boot3"]["workload"] = (
+            "cell-coefficient diffusion 128^3, k=exp(g), g~N(0,4) seed 0, harmonic faces; "
+            "b uniform[-1,1] seed 0; settings 0,3,0.5,2,0,39,0,4,4,16,4,4,2000,1e-8")
 
     out = {
         "metric": METRIC,
@@ -182,6 +279,7 @@ def main():
                      "bytes_per_launch": sweep_b, "avg_launch_ms": sweep_avg_ms,
                      "launches": sm_launches},
         "cpu_baseline": cpu,
+        "precision_form": precision,
     }
     print(json.dumps(out))
 

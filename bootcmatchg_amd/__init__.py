@@ -98,6 +98,9 @@ def lib():
     return _lib
 
 
+PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
+
+
 def _check(rc):
     if rc != OK:
         raise BcmgError(rc, lib().bcmg_last_error().decode(errors="replace"))
@@ -284,6 +287,18 @@ class Boot:
 
     def precond(self, r_dev, x_dev):
         _check(lib().bcmg_precond_apply(self.ctx.h, self.h, r_dev.ptr, x_dev.ptr))
+
+    def set_precision(self, precision):
+        """Precision of the preconditioner: PREC_FP64 (the drop-in), PREC_FP32
+        or PREC_BF16 (bfloat16 operator values on levels >= 1); FCG stays
+        FP64 with a true-residual check in the reduced precisions."""
+        _check(lib().bcmg_set_precision(self.h, int(precision)))
+
+    def restarts(self):
+        """True-residual restarts made by the last fcg()."""
+        r = ctypes.c_int()
+        _check(lib().bcmg_fcg_restarts(self.h, ctypes.byref(r)))
+        return r.value
 
 
 def solve(A, p, rhs):

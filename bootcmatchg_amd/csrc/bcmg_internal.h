@@ -14,6 +14,7 @@
 #include <map>
 #include <unordered_map>
 #include <stdexcept>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -110,12 +111,19 @@ struct DBuf {
 
 /* -------------------------------------------------------------- matrix */
 
+/* which stored values a row kernel reads */
+enum MatType { MT_F64 = 0, MT_F32 = 1, MT_BF16 = 2 };
+
 /* CSR, 0-based, int32 indices, FP64 values; columns ascending with a diagonal
    entry in every row (the layout of src/matrix/CSR.h:13-28). */
 struct DevCsr {
     int n = 0, m = 0, nnz = 0;
     DBuf<int> row, col;
     DBuf<double> val;
+    DBuf<float> val32;           /* reduced-precision copies of val (optional) */
+    DBuf<unsigned short> val16;  /* bfloat16 bit patterns */
+    /* fill val32 (bf16 = false) or val16 (bf16 = true) from val */
+    void make_lowp(hipStream_t s, bool bf16);
     int group = 8;          /* lanes per row of the row-group kernels */
     int tile = 0;           /* max nnz of a 256-row block; 0 = not analysed, -1 pending */
     int maxrow = 0;         /* longest row (valid with tile) */
@@ -163,6 +171,7 @@ struct Level {
     const DevCsr &mat() const { return Aext ? *Aext : Aown; }
     DBuf<double> D, M;          /* diag(A), sum_j |a_ij| (may be empty) */
     bool hasD = false, hasM = false;
+    DBuf<float> D32, M32, p32;  /* FP32 copies for the reduced-precision cycle */
     /* prolongator to the next level: one entry per fine row */
     DBuf<int> agg;
     DBuf<double> p;
@@ -176,6 +185,15 @@ struct Hierarchy {
     int num_levels() const { return (int)L.size(); }
 };
 
+/* work vectors of the cycles in one compute type, per level, sized to the
+   largest hierarchy */
+template <typename VT>
+struct CycleBufs {
+    std::vector<DBuf<VT>> xa, xb, f;
+    DBuf<VT> res;               /* residual buffer, size n0 */
+    DBuf<VT> xsave, xadd;       /* additive composite */
+};
+
 struct Boot {
     bcmg_params prm{};
     int coarse_solver = 4;
@@ -183,12 +201,11 @@ struct Boot {
     std::vector<Hierarchy> H;
     double estimated_ratio = 1.0;
     DevCsr *A0 = nullptr;       /* fine operator (owned by the caller's handle) */
-    const double *f0 = nullptr; /* level-0 right-hand side of the running cycle */
-    /* cycle work vectors, per level, sized to the max over hierarchies */
-    std::vector<DBuf<double>> xa, xb, f;
-    DBuf<double> res;           /* residual buffer, size n0 */
-    DBuf<double> xsave;         /* additive composite: input x */
-    DBuf<double> xadd;
+    int prec = BCMG_PREC_FP64;  /* storage/compute precision of the cycles */
+    int restarts = 0;           /* true-residual restarts of the last FCG */
+    CycleBufs<double> b64;
+    CycleBufs<float> b32;
+    DBuf<float> r32, z32;       /* level-0 input/output of an FP32 cycle */
     /* FCG vectors */
     DBuf<double> v, w, d, q, r;
     DBuf<double> scal;          /* FCG scalars on device */
@@ -207,18 +224,30 @@ struct Boot {
 };
 
 /* ----------------------------------------------------- launch wrappers */
-/* solve-phase kernels (kernels_solve.hip) */
+/* solve-phase kernels (kernels_solve.hip). VT is the vector/compute type
+   (double or float); MatType picks which stored values of a DevCsr a row
+   kernel reads. */
 enum RowOp { OP_SPMV = 0, OP_RESID = 1, OP_JACOBI = 2 };
+template <typename VT>
+void launch_rows(hipStream_t s, const DevCsr &A, MatType mt, RowOp op, const VT *x, const VT *f,
+                 const VT *d, VT *y);
 void launch_rows(hipStream_t s, const DevCsr &A, RowOp op, const double *x, const double *f,
                  const double *d, double *y);
-void launch_jacobi_zero(hipStream_t s, int n, const double *f, const double *d, double *y);
-void launch_prolong(hipStream_t s, int n, const int *agg, const double *p, const double *xc,
-                    double *x);
-void launch_fill(hipStream_t s, int n, double v, double *x);
-void launch_copy(hipStream_t s, int n, const double *a, double *b);
+template <typename VT>
+void launch_jacobi_zero(hipStream_t s, int n, const VT *f, const VT *d, VT *y);
+template <typename VT>
+void launch_prolong(hipStream_t s, int n, const int *agg, const VT *p, const VT *xc, VT *x);
+template <typename VT>
+void launch_fill(hipStream_t s, int n, VT v, VT *x);
+template <typename VT>
+void launch_copy(hipStream_t s, int n, const VT *a, VT *b);
 /* y = alpha x + y */
-void launch_axpy(hipStream_t s, int n, double alpha, const double *x, double *y);
-void launch_scale(hipStream_t s, int n, double alpha, double *x);
+template <typename VT>
+void launch_axpy(hipStream_t s, int n, VT alpha, const VT *x, VT *y);
+template <typename VT>
+void launch_scale(hipStream_t s, int n, VT alpha, VT *x);
+void launch_d2f(hipStream_t s, size_t n, const double *a, float *b);
+void launch_f2d(hipStream_t s, size_t n, const float *a, double *b);
 /* deterministic dot products: up to 3 dots of (a_k, b) ; results to out[k] (device) */
 void launch_dots(Ctx &c, int n, int k, const double *a0, const double *a1, const double *a2,
                  const double *b, double *out);
@@ -230,6 +259,9 @@ void launch_fcg_update(Ctx &c, int n, double *scal, const double *zo, double *z,
 /* first FCG step: x += theta v; r -= theta w; ||r|| (theta from scal[4]) */
 void launch_fcg_first(Ctx &c, int n, double *scal, const double *v, double *x,
                       const double *w, double *r);
+/* r = b - A x (FP64, y is scratch), scal[6] = ||r|| */
+void launch_true_residual(Ctx &c, const DevCsr &A, const double *b, const double *x, double *y,
+                          double *r, double *scal);
 /* t = A z with the FCG products of z (k = 2: r.z, t.z; k = 3: also to.z) and
    the scalar recurrence, all on the device */
 void launch_spmv_dots(Ctx &c, const DevCsr &A, int k, const double *z, double *t,
@@ -237,9 +269,11 @@ void launch_spmv_dots(Ctx &c, const DevCsr &A, int k, const double *z, double *t
 /* doubles of reduction scratch needed for an operator of n rows */
 size_t red_size_for(int n);
 /* all sweeps of a small level in one workgroup (when relax_block_ok(A)) */
+template <typename VT>
 bool relax_block_ok(const DevCsr &A);
-void launch_relax_block(hipStream_t s, const DevCsr &A, const double *d, const double *f,
-                        const double *xin, double *xout, int sweeps, bool zero);
+template <typename VT>
+void launch_relax_block(hipStream_t s, const DevCsr &A, MatType mt, const VT *d, const VT *f,
+                        const VT *xin, VT *xout, int sweeps, bool zero);
 
 /* setup-phase kernels (kernels_setup.hip) */
 void launch_diag(hipStream_t s, const DevCsr &A, double *D);
@@ -267,6 +301,8 @@ void members_from_agg(Ctx &c, int n, const int *agg, int nc, int *members);
 long long exclusive_scan(Ctx &c, const int *in, int *out, int n, bool read_total = true);
 
 /* ------------------------------------------------------------ host algo */
+/* switch a built composite to BCMG_PREC_* (converts the hierarchy copies) */
+void set_precision(Ctx &c, Boot &B, int prec);
 void build_boot(Ctx &c, DevCsr &A, const bcmg_params &p, Boot &B);
 void precond_apply(Ctx &c, Boot &B, const double *r, double *x, bool x_is_zero);
 int fcg_solve(Ctx &c, Boot &B, const double *b, double *x, int *iters, double *resid,

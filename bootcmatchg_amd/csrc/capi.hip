@@ -373,6 +373,20 @@ int bcmg_setup(bcmg_ctx ctx, bcmg_csr A, const bcmg_params *p, bcmg_boot *out)
     API_END
 }
 
+int bcmg_set_precision(bcmg_boot B, int precision)
+{
+    if (!B) return BCMG_EINVAL;
+    if (precision != BCMG_PREC_FP64 && precision != BCMG_PREC_FP32 &&
+        precision != BCMG_PREC_BF16) {
+        set_last_error("precision must be BCMG_PREC_FP64, BCMG_PREC_FP32 or BCMG_PREC_BF16");
+        return BCMG_EINVAL;
+    }
+    API_BEGIN
+    CtxScope scope_(B->ctx);
+    set_precision(B->ctx->c, B->B, precision);
+    API_END
+}
+
 int bcmg_boot_free(bcmg_boot B)
 {
     if (!B) return BCMG_OK;
@@ -433,6 +447,13 @@ int bcmg_boot_info(bcmg_boot B, int *n_hrc, double *estimated_ratio)
     if (!B) return BCMG_EINVAL;
     if (n_hrc) *n_hrc = (int)B->B.H.size();
     if (estimated_ratio) *estimated_ratio = B->B.estimated_ratio;
+    return BCMG_OK;
+}
+
+int bcmg_fcg_restarts(bcmg_boot B, int *restarts)
+{
+    if (!B || !restarts) return BCMG_EINVAL;
+    *restarts = B->B.restarts;
     return BCMG_OK;
 }
 

@@ -143,36 +143,72 @@ void adaptive_coarsening(Ctx &c, const bcmg_params &p, BuildState &bs, DevCsr &A
 
 /* ------------------------------------------------------------ cycles */
 
+template <typename VT>
 struct LevelState {
-    double *x = nullptr, *x2 = nullptr;
+    VT *x = nullptr, *x2 = nullptr;
     bool zero = true;
 };
 
+/* per-precision views of a level's data */
+template <typename VT>
+const VT *divisor(const Level &lv, int type);
+template <>
+const double *divisor<double>(const Level &lv, int type)
+{
+    return type == 4 ? lv.M.get() : lv.D.get();
+}
+template <>
+const float *divisor<float>(const Level &lv, int type)
+{
+    return type == 4 ? lv.M32.get() : lv.D32.get();
+}
+template <typename VT>
+const VT *pvals(const Level &lv);
+template <>
+const double *pvals<double>(const Level &lv) { return lv.p.get(); }
+template <>
+const float *pvals<float>(const Level &lv) { return lv.p32.get(); }
+template <typename VT>
+CycleBufs<VT> &bufs(Boot &B);
+template <>
+CycleBufs<double> &bufs<double>(Boot &B) { return B.b64; }
+template <>
+CycleBufs<float> &bufs<float>(Boot &B) { return B.b32; }
+
+/* stored values a level's kernels read: FP64 for the drop-in; FP32, or
+   bfloat16 on levels >= 1, for the reduced-precision preconditioner */
+MatType level_mt(const Boot &B, int l)
+{
+    if (B.prec == BCMG_PREC_FP64) return MT_F64;
+    if (B.prec == BCMG_PREC_BF16 && l > 0) return MT_BF16;
+    return MT_F32;
+}
+
+template <typename VT>
 struct CycleRun {
     Ctx &c;
     Boot &B;
     const Hierarchy &H;
-    std::vector<LevelState> &st;
-    double *final_out;          /* where the very last level-0 sweep should write */
-    bool last_hierarchy;
+    std::vector<LevelState<VT>> &st;
+    const VT *f0;               /* level-0 right-hand side */
+    VT *final_out;              /* where the very last level-0 sweep should write */
 };
-
-const double *divisor(const Level &lv, int type) { return type == 4 ? lv.M.get() : lv.D.get(); }
 
 /* `sweeps` Jacobi sweeps on level lv (src/AMG/relaxation.cu:211-285),
    ping-pong between st.x and st.x2. A sweep from a known-zero x is f / d
    (the SpMV of a zero vector is skipped; same values). A small level runs
    all its sweeps in one workgroup (same values, one launch). */
-void relax_level(Ctx &c, const Level &lv, int type, int sweeps, const double *f, LevelState &s,
-                 double *final_out, bool level0)
+template <typename VT>
+void relax_level(Ctx &c, const Level &lv, MatType mt, int type, int sweeps, const VT *f,
+                 LevelState<VT> &s, VT *final_out, bool level0)
 {
     const DevCsr &A = lv.mat();
-    const double *d = divisor(lv, type);
+    const VT *d = divisor<VT>(lv, type);
     if (sweeps == 0) return;
     const bool timing = level0 && c.time_smoother;
-    if (relax_block_ok(A) && !timing) {
-        double *out = (final_out && s.x != final_out) ? final_out : s.x;
-        launch_relax_block(c.stream, A, d, f, s.x, out, sweeps, s.zero);
+    if (relax_block_ok<VT>(A) && !timing) {
+        VT *out = (final_out && s.x != final_out) ? final_out : s.x;
+        launch_relax_block<VT>(c.stream, A, mt, d, f, s.x, out, sweeps, s.zero);
         if (out != s.x) {
             s.x2 = s.x;
             s.x = out;
@@ -181,10 +217,10 @@ void relax_level(Ctx &c, const Level &lv, int type, int sweeps, const double *f,
         return;
     }
     for (int k = 0; k < sweeps; k++) {
-        double *out = s.x2;
+        VT *out = s.x2;
         if (final_out && k == sweeps - 1 && s.x != final_out) out = final_out;
         if (s.zero) {
-            launch_jacobi_zero(c.stream, A.n, f, d, out);
+            launch_jacobi_zero<VT>(c.stream, A.n, f, d, out);
         } else {
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (timing) {
@@ -192,7 +228,7 @@ void relax_level(Ctx &c, const Level &lv, int type, int sweeps, const double *f,
                 e1 = c.next_event();
                 BCMG_HIP(hipEventRecord(e0, c.stream));
             }
-            launch_rows(c.stream, A, OP_JACOBI, s.x, f, d, out);
+            launch_rows<VT>(c.stream, A, mt, OP_JACOBI, s.x, f, d, out);
             if (timing) BCMG_HIP(hipEventRecord(e1, c.stream));
         }
         s.x2 = s.x;
@@ -202,56 +238,132 @@ void relax_level(Ctx &c, const Level &lv, int type, int sweeps, const double *f,
 }
 
 /* recursive cycle (src/SOLVER/GAMG_cycle.cu:25-140); l is 1-based */
-void cycle(CycleRun &R, int l)
+template <typename VT>
+void cycle(CycleRun<VT> &R, int l)
 {
     Ctx &c = R.c;
     Boot &B = R.B;
+    CycleBufs<VT> &W = bufs<VT>(B);
     const Hierarchy &H = R.H;
     const int L = H.num_levels();
     const Level &lv = H.L[l - 1];
-    LevelState &s = R.st[l - 1];
-    const double *f = (l == 1) ? R.B.f0 : B.f[l - 1].get();
+    const MatType mt = level_mt(B, l - 1);
+    LevelState<VT> &s = R.st[l - 1];
+    const VT *f = (l == 1) ? R.f0 : W.f[l - 1].get();
     const bool top = (l == 1);
     if (l == L) {
-        relax_level(c, lv, B.coarse_solver, B.prm.relaxnumber_coarse, f, s,
-                    top ? R.final_out : nullptr, top);
+        relax_level<VT>(c, lv, mt, B.coarse_solver, B.prm.relaxnumber_coarse, f, s,
+                        top ? R.final_out : nullptr, top);
         return;
     }
-    relax_level(c, lv, B.prm.relax_type, B.prm.prerelax_sweeps, f, s, nullptr, top);
+    relax_level<VT>(c, lv, mt, B.prm.relax_type, B.prm.prerelax_sweeps, f, s, nullptr, top);
     const DevCsr &A = lv.mat();
-    const double *rsrc = f;
+    const VT *rsrc = f;
     if (!s.zero) {
-        launch_rows(c.stream, A, OP_RESID, s.x, f, nullptr, B.res.get());
-        rsrc = B.res.get();
+        launch_rows<VT>(c.stream, A, mt, OP_RESID, s.x, f, nullptr, W.res.get());
+        rsrc = W.res.get();
     }
-    launch_rows(c.stream, lv.R, OP_SPMV, rsrc, nullptr, nullptr, B.f[l].get());
-    LevelState &sc = R.st[l];
+    /* R holds the prolongator values: FP32 whenever the cycle runs in FP32 */
+    launch_rows<VT>(c.stream, lv.R, std::is_same<VT, double>::value ? MT_F64 : MT_F32, OP_SPMV,
+                    rsrc, nullptr, nullptr, W.f[l].get());
+    LevelState<VT> &sc = R.st[l];
     sc.zero = true;
     for (int i = 1; i <= B.num_grid_sweeps[l - 1]; i++) {
-        cycle(R, l + 1);
+        cycle<VT>(R, l + 1);
         if (l == L - 1) break;
     }
     if (!sc.zero) {
         if (s.zero) {
-            launch_fill(c.stream, A.n, 0.0, s.x);
+            launch_fill<VT>(c.stream, A.n, VT(0), s.x);
             s.zero = false;
         }
-        launch_prolong(c.stream, A.n, lv.agg.get(), lv.p.get(), sc.x, s.x);
+        launch_prolong<VT>(c.stream, A.n, lv.agg.get(), pvals<VT>(lv), sc.x, s.x);
     }
-    relax_level(c, lv, B.prm.relax_type, B.prm.postrelax_sweeps, f, s, top ? R.final_out : nullptr,
-                top);
+    relax_level<VT>(c, lv, mt, B.prm.relax_type, B.prm.postrelax_sweeps, f, s,
+                    top ? R.final_out : nullptr, top);
 }
 
-void run_hierarchy(Ctx &c, Boot &B, int k, std::vector<LevelState> &st, double *final_out)
+template <typename VT>
+void run_hierarchy(Ctx &c, Boot &B, int k, std::vector<LevelState<VT>> &st, const VT *f0,
+                   VT *final_out)
 {
+    CycleBufs<VT> &W = bufs<VT>(B);
     const Hierarchy &H = B.H[k];
     for (int l = 1; l < H.num_levels(); l++) {
-        st[l].x = B.xa[l].get();
-        st[l].x2 = B.xb[l].get();
+        st[l].x = W.xa[l].get();
+        st[l].x2 = W.xb[l].get();
         st[l].zero = true;
     }
-    CycleRun R{c, B, H, st, final_out, true};
-    cycle(R, 1);
+    CycleRun<VT> R{c, B, H, st, f0, final_out};
+    cycle<VT>(R, 1);
+}
+
+/* composite preconditioner x <- B(r; x) (src/SOLVER/FCG.cu:112-187) in the
+   compute type VT. When x_is_zero the content of x is treated as zero. */
+template <typename VT>
+void precond_apply_t(Ctx &c, Boot &B, const VT *r, VT *x, bool x_is_zero)
+{
+    CycleBufs<VT> &W = bufs<VT>(B);
+    const int n = B.H[0].L[0].mat().n;
+    const int nh = (int)B.H.size();
+    const size_t maxL = W.xa.size();
+    std::vector<LevelState<VT>> st(maxL);
+    if (B.prm.solver_type == 0 || B.prm.solver_type == 1) {
+        st[0].x = x;
+        st[0].x2 = W.xa[0].get();
+        st[0].zero = x_is_zero;
+        std::vector<int> order;
+        for (int k = 0; k < nh; k++) order.push_back(k);
+        if (B.prm.solver_type == 1)
+            for (int k = nh - 1; k >= 0; k--) order.push_back(k);
+        for (size_t t = 0; t < order.size(); t++) {
+            const bool last = (t + 1 == order.size());
+            run_hierarchy<VT>(c, B, order[t], st, r, last ? x : nullptr);
+        }
+        if (st[0].zero) {
+            launch_fill<VT>(c.stream, n, VT(0), x);
+        } else if (st[0].x != x) {
+            launch_copy<VT>(c.stream, n, st[0].x, x);
+        }
+    } else {
+        /* additive: every hierarchy starts from the same x; mean of results */
+        W.xadd.ensure(n);
+        W.xsave.ensure(n);
+        launch_fill<VT>(c.stream, n, VT(0), W.xadd.get());
+        if (!x_is_zero) launch_copy<VT>(c.stream, n, x, W.xsave.get());
+        for (int k = 0; k < nh; k++) {
+            st[0].x = W.xb[0].get();
+            st[0].x2 = W.xa[0].get();
+            st[0].zero = x_is_zero;
+            if (!x_is_zero) launch_copy<VT>(c.stream, n, W.xsave.get(), st[0].x);
+            run_hierarchy<VT>(c, B, k, st, r, nullptr);
+            if (st[0].zero) launch_fill<VT>(c.stream, n, VT(0), st[0].x);
+            launch_axpy<VT>(c.stream, n, VT(1), st[0].x, W.xadd.get());
+        }
+        launch_scale<VT>(c.stream, n, VT(1) / (VT)nh, W.xadd.get());
+        launch_copy<VT>(c.stream, n, W.xadd.get(), x);
+    }
+}
+
+template <typename VT>
+void ensure_bufs(Boot &B, CycleBufs<VT> &W)
+{
+    size_t maxL = 0;
+    for (auto &h : B.H) maxL = std::max(maxL, h.L.size());
+    if (W.xa.size() < maxL) {
+        W.xa.resize(maxL);
+        W.xb.resize(maxL);
+        W.f.resize(maxL);
+    }
+    for (size_t l = 0; l < maxL; l++) {
+        size_t nl = 0;
+        for (auto &h : B.H)
+            if (l < h.L.size()) nl = std::max(nl, (size_t)h.L[l].mat().n);
+        W.xa[l].ensure(nl);
+        W.xb[l].ensure(nl);
+        W.f[l].ensure(nl);
+    }
+    W.res.ensure(B.H[0].L[0].mat().n);
 }
 
 } // namespace
@@ -269,23 +381,13 @@ Boot::~Boot() { drop_graphs(); }
 
 void Boot::ensure_work()
 {
-    size_t maxL = 0;
-    for (auto &h : H) maxL = std::max(maxL, h.L.size());
-    if (xa.size() < maxL) {
-        xa.resize(maxL);
-        xb.resize(maxL);
-        f.resize(maxL);
+    ensure_bufs<double>(*this, b64);
+    if (prec != BCMG_PREC_FP64) {
+        ensure_bufs<float>(*this, b32);
+        const size_t n0 = H[0].L[0].mat().n;
+        r32.ensure(n0);
+        z32.ensure(n0);
     }
-    for (size_t l = 0; l < maxL; l++) {
-        size_t nl = 0;
-        for (auto &h : H)
-            if (l < h.L.size()) nl = std::max(nl, (size_t)h.L[l].mat().n);
-        xa[l].ensure(nl);
-        xb[l].ensure(nl);
-        f[l].ensure(nl);
-    }
-    const size_t n0 = H[0].L[0].mat().n;
-    res.ensure(n0);
 }
 
 double a_norm(Ctx &c, const DevCsr &A, const double *x, double *tmp)
@@ -299,50 +401,56 @@ double a_norm(Ctx &c, const DevCsr &A, const double *x, double *tmp)
     return std::sqrt(c.h_scalars[0]);
 }
 
-/* composite preconditioner x <- B(r; x) (src/SOLVER/FCG.cu:112-187). When
-   x_is_zero the content of x is ignored and treated as zero. */
+/* composite preconditioner on FP64 vectors; with a reduced-precision
+   hierarchy r is rounded to FP32, the cycles run in FP32 and the result is
+   widened back to FP64 */
 void precond_apply(Ctx &c, Boot &B, const double *r, double *x, bool x_is_zero)
 {
-    const int n = B.H[0].L[0].mat().n;
-    const int nh = (int)B.H.size();
-    size_t maxL = B.xa.size();
-    std::vector<LevelState> st(maxL);
-    B.f0 = r;
-    if (B.prm.solver_type == 0 || B.prm.solver_type == 1) {
-        st[0].x = x;
-        st[0].x2 = B.xa[0].get();
-        st[0].zero = x_is_zero;
-        std::vector<int> order;
-        for (int k = 0; k < nh; k++) order.push_back(k);
-        if (B.prm.solver_type == 1)
-            for (int k = nh - 1; k >= 0; k--) order.push_back(k);
-        for (size_t t = 0; t < order.size(); t++) {
-            const bool last = (t + 1 == order.size());
-            run_hierarchy(c, B, order[t], st, last ? x : nullptr);
-        }
-        if (st[0].zero) {
-            launch_fill(c.stream, n, 0.0, x);
-        } else if (st[0].x != x) {
-            launch_copy(c.stream, n, st[0].x, x);
-        }
-    } else {
-        /* additive: every hierarchy starts from the same x; mean of results */
-        B.xadd.ensure(n);
-        B.xsave.ensure(n);
-        launch_fill(c.stream, n, 0.0, B.xadd.get());
-        if (!x_is_zero) launch_copy(c.stream, n, x, B.xsave.get());
-        for (int k = 0; k < nh; k++) {
-            st[0].x = B.xb[0].get();
-            st[0].x2 = B.xa[0].get();
-            st[0].zero = x_is_zero;
-            if (!x_is_zero) launch_copy(c.stream, n, B.xsave.get(), st[0].x);
-            run_hierarchy(c, B, k, st, nullptr);
-            if (st[0].zero) launch_fill(c.stream, n, 0.0, st[0].x);
-            launch_axpy(c.stream, n, 1.0, st[0].x, B.xadd.get());
-        }
-        launch_scale(c.stream, n, 1.0 / (double)nh, B.xadd.get());
-        launch_copy(c.stream, n, B.xadd.get(), x);
+    if (B.prec == BCMG_PREC_FP64) {
+        precond_apply_t<double>(c, B, r, x, x_is_zero);
+        return;
     }
+    const int n = B.H[0].L[0].mat().n;
+    launch_d2f(c.stream, n, r, B.r32.get());
+    if (!x_is_zero) launch_d2f(c.stream, n, x, B.z32.get());
+    precond_apply_t<float>(c, B, B.r32.get(), B.z32.get(), x_is_zero);
+    launch_f2d(c.stream, n, B.z32.get(), x);
+}
+
+/* Convert the built hierarchy to the reduced-precision form: FP32 copies of
+   every level's values (bfloat16 on levels >= 1 for BCMG_PREC_BF16), of the
+   smoother divisors and of the prolongator values. */
+void set_precision(Ctx &c, Boot &B, int prec)
+{
+    B.prec = prec;
+    B.drop_graphs();
+    if (prec == BCMG_PREC_FP64) return;
+    for (auto &H : B.H) {
+        for (int l = 0; l < H.num_levels(); l++) {
+            Level &lv = H.L[l];
+            DevCsr &A = l == 0 ? *B.A0 : lv.Aown;
+            if (level_mt(B, l) == MT_BF16)
+                A.make_lowp(c.stream, true);
+            else
+                A.make_lowp(c.stream, false);
+            const size_t n = A.n;
+            if (lv.hasD) {
+                lv.D32.ensure(n);
+                launch_d2f(c.stream, n, lv.D.get(), lv.D32.get());
+            }
+            if (lv.hasM) {
+                lv.M32.ensure(n);
+                launch_d2f(c.stream, n, lv.M.get(), lv.M32.get());
+            }
+            if (l + 1 < H.num_levels()) {
+                lv.p32.ensure(n);
+                launch_d2f(c.stream, n, lv.p.get(), lv.p32.get());
+                lv.R.make_lowp(c.stream, false);
+            }
+        }
+    }
+    B.ensure_work();
+    BCMG_HIP(hipStreamSynchronize(c.stream));
 }
 
 namespace {
@@ -456,37 +564,68 @@ int fcg_solve(Ctx &c, Boot &B, const double *b, double *x, int *iters, double *r
        plain launches. */
     const bool graphs = c.use_graphs && !c.time_smoother;
     if (graphs && (B.graph_x != x || B.graph_red != c.red.get())) B.drop_graphs();
-    int iter = 0;
+    B.restarts = 0;
+    int iter = 0;  /* iterations after the first step, restarts included */
+    int phase = 0; /* iterations since the last (re)start: selects the slot pair */
     double l2 = 0.0;
-    do {
-        const bool even = (iter % 2) == 0;
-        if (graphs) {
-            hipGraphExec_t &ge = B.gexec[even ? 0 : 1];
-            if (!ge) {
-                hipGraph_t g = nullptr;
-                BCMG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-                try {
-                    enqueue_iteration(even);
-                } catch (...) {
-                    (void)hipStreamEndCapture(c.stream, &g);
-                    if (g) (void)hipGraphDestroy(g);
-                    throw;
+    bool run_loop = true;
+    for (;;) {
+        if (run_loop) do {
+            const bool even = (phase % 2) == 0;
+            if (graphs) {
+                hipGraphExec_t &ge = B.gexec[even ? 0 : 1];
+                if (!ge) {
+                    hipGraph_t g = nullptr;
+                    BCMG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+                    try {
+                        enqueue_iteration(even);
+                    } catch (...) {
+                        (void)hipStreamEndCapture(c.stream, &g);
+                        if (g) (void)hipGraphDestroy(g);
+                        throw;
+                    }
+                    BCMG_HIP(hipStreamEndCapture(c.stream, &g));
+                    const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    BCMG_HIP(e);
+                    B.graph_x = x;
+                    B.graph_red = c.red.get();
                 }
-                BCMG_HIP(hipStreamEndCapture(c.stream, &g));
-                const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(g);
-                BCMG_HIP(e);
-                B.graph_x = x;
-                B.graph_red = c.red.get();
+                BCMG_HIP(hipGraphLaunch(ge, c.stream));
+            } else {
+                enqueue_iteration(even);
             }
-            BCMG_HIP(hipGraphLaunch(ge, c.stream));
-        } else {
-            enqueue_iteration(even);
-        }
+            BCMG_HIP(hipStreamSynchronize(c.stream));
+            l2 = c.h_scalars[8];
+            iter++;
+            phase++;
+        } while (l2 > rtol * delta0 && iter < max_iter);
+        /* With a reduced-precision hierarchy the recursively updated r can
+           drift from b - A x; the stopping test is then confirmed on the
+           true residual, and FCG restarts from it (first step again, same
+           slot pairs) while it is above the tolerance. FP64 keeps the
+           reference's recursive test unchanged. */
+        if (B.prec == BCMG_PREC_FP64 || !std::isfinite(l2)) break;
+        launch_true_residual(c, A, b, x, v, r, scal);
+        BCMG_HIP(hipMemcpyAsync(c.h_scalars + 8, scal + 6, sizeof(double), hipMemcpyDeviceToHost,
+                                c.stream));
+        BCMG_HIP(hipStreamSynchronize(c.stream));
+        l2 = c.h_scalars[8];
+        if (l2 <= rtol * delta0 || iter >= max_iter || !std::isfinite(l2)) break;
+        B.restarts++;
+        precond_apply(c, B, r, d, true);
+        launch_spmv_dots(c, A, 2, d, q, r, nullptr, scal);
+        launch_fcg_first(c, n, scal, d, x, q, r);
+        BCMG_HIP(hipMemcpyAsync(c.h_scalars + 8, scal + 6, sizeof(double), hipMemcpyDeviceToHost,
+                                c.stream));
         BCMG_HIP(hipStreamSynchronize(c.stream));
         l2 = c.h_scalars[8];
         iter++;
-    } while (l2 > rtol * delta0 && iter < max_iter);
+        phase = 0;
+        /* a restart that already meets the test, or the limit, goes straight
+           to the true-residual check */
+        run_loop = l2 > rtol * delta0 && iter < max_iter;
+    }
     if (iters) *iters = iter + 1;
     if (resid) *resid = l2;
     if (!std::isfinite(l2)) throw Error(BCMG_EBREAKDOWN, "non-finite residual");

@@ -2,11 +2,10 @@
  * kernels_solve.hip - solve-phase kernels for gfx950 (wave64, CDNA4).
  *
  * Everything here is HBM-bound CSR/vector streaming; there is no matrix-core
- * work. Rows are processed by groups of G lanes (G a power of two chosen from
- * the average row length, so a 7-point row uses 8 lanes and eight rows share
- * one wave64); the group reduces with xor-shuffles. Blocks are remapped so
- * that each of the 8 XCDs streams one contiguous slice of the rows and keeps
- * the gathered x entries of its slice in its own L2.
+ * work. The row kernels are templated on the stored matrix value type MT
+ * (double, float, or bfloat16 kept as its 16-bit pattern) and on the compute
+ * and vector type VT (double or float), so the same code serves the FP64
+ * drop-in and the reduced-precision preconditioner.
  *
  * Reductions (dot products, norms) are deterministic: a fixed grid that
  * depends only on n, a fixed in-block tree and a fixed final pass; no
@@ -24,8 +23,8 @@ constexpr int RED_MAX_BLOCKS = 1024;
 /* layout of the context's reduction scratch (doubles) */
 constexpr size_t RED_UPD = 3 * RED_MAX_BLOCKS + 16;   /* update/norm partials */
 constexpr size_t RED_SPMV = 4 * RED_MAX_BLOCKS + 32;  /* SpMV+dots partials */
-constexpr int STREAM_TILE_MAX = 4096;  /* doubles of LDS per block (32 KB) */
-/* matrices above this size are streamed with non-temporal loads */
+constexpr int STREAM_TILE_BYTES = 32768;              /* LDS per block */
+/* matrices whose value+column stream exceeds this are read non-temporally */
 constexpr size_t NT_BYTES = (size_t)192 << 20;
 
 /* Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
@@ -37,34 +36,49 @@ __device__ __forceinline__ int xcd_block(int bid, int nb)
     return x * q + (x < rm ? x : rm) + l;
 }
 
+/* stored matrix value -> compute type (bfloat16: upper half of a float) */
+template <typename VT>
+__device__ __forceinline__ VT mval(double v) { return (VT)v; }
+template <typename VT>
+__device__ __forceinline__ VT mval(float v) { return (VT)v; }
+template <typename VT>
+__device__ __forceinline__ VT mval(unsigned short h) { return (VT)__uint_as_float((unsigned)h << 16); }
+
+template <bool NT, typename T>
+__device__ __forceinline__ T ldm(const T *p)
+{
+    if (NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
 /* Row-block kernel: block b owns rows [256 b, 256 b + 256). Phase 1 streams
    the block's slice of val/col with consecutive lanes on consecutive entries
    (fully coalesced; non-temporal when the matrix is larger than the 256 MB
    last-level cache, so that the once-read stream does not push the gathered
    x out of the caches, while a smaller matrix stays cached for the next
-   sweep) and stores the products a_ij x_j in LDS;
-   phase 2 gives one row to each thread, which sums its products in column
-   order and applies the epilogue. The row's own operands (row pointers, f,
-   d, x) are loaded before phase 1 so that their latency overlaps the stream.
-   The sum order (products rounded, then added left to right) is the serial
-   one. */
-template <int OP, bool NT>
+   sweep) and stores the products a_ij x_j in LDS; phase 2 gives one row to
+   each thread, which sums its products in column order and applies the
+   epilogue. The row's own operands (row pointers, f, d, x) are loaded before
+   phase 1 so that their latency overlaps the stream. The sum order (products
+   rounded, then added left to right) is the serial one. */
+template <typename MT, typename VT, int OP, bool NT>
 __global__ __launch_bounds__(BS) void k_rows_stream(int n, const int *__restrict__ rp,
                                                     const int *__restrict__ ci,
-                                                    const double *__restrict__ av,
-                                                    const double *__restrict__ x,
-                                                    const double *__restrict__ f,
-                                                    const double *__restrict__ d,
-                                                    double *__restrict__ y)
+                                                    const MT *__restrict__ av,
+                                                    const VT *__restrict__ x,
+                                                    const VT *__restrict__ f,
+                                                    const VT *__restrict__ d,
+                                                    VT *__restrict__ y)
 {
-    extern __shared__ double prod[];
+    extern __shared__ double smem[];
+    VT *prod = reinterpret_cast<VT *>(smem);
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int r0 = b * BS;
     const int r1 = min(r0 + BS, n);
     const int r = r0 + threadIdx.x;
     const bool own = r < r1;
     int ra = 0, re = 0;
-    double fr = 0.0, dr = 1.0, xr = 0.0;
+    VT fr = 0, dr = 1, xr = 0;
     if (own) {
         ra = rp[r];
         re = rp[r + 1];
@@ -79,29 +93,24 @@ __global__ __launch_bounds__(BS) void k_rows_stream(int n, const int *__restrict
     constexpr int U = 4;
     for (int k0 = threadIdx.x; k0 < cnt; k0 += BS * U) {
         int c[U];
-        double v[U];
+        MT v[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int k = k0 + u * BS;
             if (k < cnt) {
-                if (NT) {
-                    c[u] = __builtin_nontemporal_load(ci + e0 + k);
-                    v[u] = __builtin_nontemporal_load(av + e0 + k);
-                } else {
-                    c[u] = ci[e0 + k];
-                    v[u] = av[e0 + k];
-                }
+                c[u] = ldm<NT>(ci + e0 + k);
+                v[u] = ldm<NT>(av + e0 + k);
             }
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int k = k0 + u * BS;
-            if (k < cnt) prod[k] = v[u] * x[c[u]];
+            if (k < cnt) prod[k] = mval<VT>(v[u]) * x[c[u]];
         }
     }
     __syncthreads();
     if (!own) return;
-    double t = 0.0;
+    VT t = 0;
     for (int k = ra - e0; k < re - e0; k++) t = t + prod[k];
     if (OP == OP_SPMV)
         y[r] = t;
@@ -126,14 +135,16 @@ __global__ __launch_bounds__(BS) void k_block_nnz(int n, const int *__restrict__
     atomicMax(maxv + 1, m);
 }
 
-template <int G, int OP>
+/* Fallback for rows too long for the LDS tile: G lanes per row (a power of
+   two from the average row length), xor-shuffle reduction. */
+template <int G, typename MT, typename VT, int OP>
 __global__ __launch_bounds__(BS) void k_rows(int n, const int *__restrict__ rp,
                                              const int *__restrict__ ci,
-                                             const double *__restrict__ av,
-                                             const double *__restrict__ x,
-                                             const double *__restrict__ f,
-                                             const double *__restrict__ d,
-                                             double *__restrict__ y)
+                                             const MT *__restrict__ av,
+                                             const VT *__restrict__ x,
+                                             const VT *__restrict__ f,
+                                             const VT *__restrict__ d,
+                                             VT *__restrict__ y)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int tid = b * BS + threadIdx.x;
@@ -141,8 +152,8 @@ __global__ __launch_bounds__(BS) void k_rows(int n, const int *__restrict__ rp,
     const int lane = threadIdx.x & (G - 1);
     if (row >= n) return;
     const int j0 = rp[row], j1 = rp[row + 1];
-    double t = 0.0;
-    for (int j = j0 + lane; j < j1; j += G) t = fma(av[j], x[ci[j]], t);
+    VT t = 0;
+    for (int j = j0 + lane; j < j1; j += G) t = fma(mval<VT>(av[j]), x[ci[j]], t);
 #pragma unroll
     for (int o = G >> 1; o > 0; o >>= 1) t += __shfl_xor(t, o, G);
     if (lane == 0) {
@@ -155,47 +166,77 @@ __global__ __launch_bounds__(BS) void k_rows(int n, const int *__restrict__ rp,
     }
 }
 
-__global__ __launch_bounds__(BS) void k_jacobi_zero(int n, const double *__restrict__ f,
-                                                    const double *__restrict__ d,
-                                                    double *__restrict__ y)
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_jacobi_zero(int n, const VT *__restrict__ f,
+                                                    const VT *__restrict__ d,
+                                                    VT *__restrict__ y)
 {
     const int i = blockIdx.x * BS + threadIdx.x;
     if (i < n) y[i] = f[i] / d[i];
 }
 
+template <typename VT>
 __global__ __launch_bounds__(BS) void k_prolong(int n, const int *__restrict__ agg,
-                                                const double *__restrict__ p,
-                                                const double *__restrict__ xc,
-                                                double *__restrict__ x)
+                                                const VT *__restrict__ p,
+                                                const VT *__restrict__ xc,
+                                                VT *__restrict__ x)
 {
     const int i = blockIdx.x * BS + threadIdx.x;
     if (i < n) x[i] = fma(p[i], xc[agg[i]], x[i]);
 }
 
-__global__ __launch_bounds__(BS) void k_fill(int n, double v, double *__restrict__ x)
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_fill(int n, VT v, VT *__restrict__ x)
 {
     const int i = blockIdx.x * BS + threadIdx.x;
     if (i < n) x[i] = v;
 }
 
-__global__ __launch_bounds__(BS) void k_copy(int n, const double *__restrict__ a,
-                                             double *__restrict__ b)
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_copy(int n, const VT *__restrict__ a,
+                                             VT *__restrict__ b)
 {
     const int i = blockIdx.x * BS + threadIdx.x;
     if (i < n) b[i] = a[i];
 }
 
-__global__ __launch_bounds__(BS) void k_axpy(int n, double alpha, const double *__restrict__ x,
-                                             double *__restrict__ y)
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_axpy(int n, VT alpha, const VT *__restrict__ x,
+                                             VT *__restrict__ y)
 {
     const int i = blockIdx.x * BS + threadIdx.x;
     if (i < n) y[i] = fma(alpha, x[i], y[i]);
 }
 
-__global__ __launch_bounds__(BS) void k_scale(int n, double alpha, double *__restrict__ x)
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_scale(int n, VT alpha, VT *__restrict__ x)
 {
     const int i = blockIdx.x * BS + threadIdx.x;
     if (i < n) x[i] = alpha * x[i];
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(BS) void k_convert(size_t n, const S *__restrict__ a,
+                                                D *__restrict__ b)
+{
+    const size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
+    if (i < n) b[i] = (D)a[i];
+}
+
+/* double -> bfloat16 (through float, round to nearest even; NaN kept quiet) */
+__global__ __launch_bounds__(BS) void k_to_bf16(size_t n, const double *__restrict__ a,
+                                                unsigned short *__restrict__ b)
+{
+    const size_t i = (size_t)blockIdx.x * BS + threadIdx.x;
+    if (i >= n) return;
+    const float f = (float)a[i];
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu)) {
+        b[i] = (unsigned short)((u >> 16) | 0x40u);
+        return;
+    }
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    b[i] = (unsigned short)(u >> 16);
 }
 
 /* fixed-order block reduction of K partial sums; thread 0 writes out[0..K) */
@@ -389,13 +430,8 @@ __global__ __launch_bounds__(BS) void k_spmv_dots(int n, const int *__restrict__
         for (int u = 0; u < U; u++) {
             const int k = k0 + u * BS;
             if (k < cnt) {
-                if (NT) {
-                    c[u] = __builtin_nontemporal_load(ci + e0 + k);
-                    v[u] = __builtin_nontemporal_load(av + e0 + k);
-                } else {
-                    c[u] = ci[e0 + k];
-                    v[u] = av[e0 + k];
-                }
+                c[u] = ldm<NT>(ci + e0 + k);
+                v[u] = ldm<NT>(av + e0 + k);
             }
         }
 #pragma unroll
@@ -419,9 +455,9 @@ __global__ __launch_bounds__(BS) void k_spmv_dots(int n, const int *__restrict__
     block_reduce_store<K>(sdot, part + (size_t)b * K);
 }
 
-/* fused double_merged_axpy pair + ||r|| (src/SOLVER/FCG.cu:88-98, 536-558):
-   z = -theta zo + z; x += theta2 z; t = -theta to + t; r += -theta2 t */
-__global__ __launch_bounds__(BS) void k_fcg_update(int n, double *__restrict__ scal,
+/* fused double_merged_axpy pair + ||r||^2 partials (src/SOLVER/FCG.cu:88-98,
+   536-558): z = -theta zo + z; x += theta2 z; t = -theta to + t; r += -theta2 t */
+__global__ __launch_bounds__(BS) void k_fcg_update(int n, const double *__restrict__ scal,
                                                    const double *__restrict__ zo,
                                                    double *__restrict__ z,
                                                    double *__restrict__ x,
@@ -445,8 +481,8 @@ __global__ __launch_bounds__(BS) void k_fcg_update(int n, double *__restrict__ s
     block_reduce_store<1>(s, part + blockIdx.x);
 }
 
-/* first FCG step (src/SOLVER/FCG.cu:475-486): x += theta v, r -= theta w, ||r|| */
-__global__ __launch_bounds__(BS) void k_fcg_first(int n, double *__restrict__ scal,
+/* first FCG step (src/SOLVER/FCG.cu:475-486): x += theta v, r -= theta w, ||r||^2 */
+__global__ __launch_bounds__(BS) void k_fcg_first(int n, const double *__restrict__ scal,
                                                   const double *__restrict__ v,
                                                   double *__restrict__ x,
                                                   const double *__restrict__ w,
@@ -464,34 +500,50 @@ __global__ __launch_bounds__(BS) void k_fcg_first(int n, double *__restrict__ sc
     block_reduce_store<1>(s, part + blockIdx.x);
 }
 
+/* r = b - y (the true residual, y = A x) and its squared norm partials */
+__global__ __launch_bounds__(BS) void k_true_residual(int n, const double *__restrict__ b,
+                                                      const double *__restrict__ y,
+                                                      double *__restrict__ r,
+                                                      double *__restrict__ part)
+{
+    double s[1] = {0.0};
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const double ri = b[i] - y[i];
+        r[i] = ri;
+        s[0] = fma(ri, ri, s[0]);
+    }
+    block_reduce_store<1>(s, part + blockIdx.x);
+}
+
 /* All `sweeps` Jacobi sweeps of a small level in one workgroup of 1024
    threads: thread t owns rows t + 1024 q (q < RPT) and keeps their entries
    (at most MAXNZ per row) in registers for all sweeps; both iterates live in
    LDS and a barrier separates the sweeps. Row sums are sequential in column
    order (the order of k_rows_stream), so the result equals `sweeps` separate
    launches bit for bit. */
-template <int RPT, int MAXNZ>
+template <typename MT, typename VT, int RPT, int MAXNZ>
 __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restrict__ rp,
                                                       const int *__restrict__ ci,
-                                                      const double *__restrict__ av,
-                                                      const double *__restrict__ d,
-                                                      const double *__restrict__ f,
-                                                      const double *__restrict__ xin,
-                                                      double *__restrict__ xout, int sweeps,
+                                                      const MT *__restrict__ av,
+                                                      const VT *__restrict__ d,
+                                                      const VT *__restrict__ f,
+                                                      const VT *__restrict__ xin,
+                                                      VT *__restrict__ xout, int sweeps,
                                                       int zero)
 {
-    extern __shared__ double xs[];
-    double *cur = xs, *nxt = xs + n;
-    double a[RPT][MAXNZ];
+    extern __shared__ double smem[];
+    VT *xs = reinterpret_cast<VT *>(smem);
+    VT *cur = xs, *nxt = xs + n;
+    VT a[RPT][MAXNZ];
     int cj[RPT][MAXNZ];
     int len[RPT];
-    double fi[RPT], di[RPT];
+    VT fi[RPT], di[RPT];
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
         const int i = threadIdx.x + 1024 * q;
         len[q] = 0;
-        fi[q] = 0.0;
-        di[q] = 1.0;
+        fi[q] = 0;
+        di[q] = 1;
         if (i < n) {
             const int j0 = rp[i];
             len[q] = rp[i + 1] - j0;
@@ -499,7 +551,7 @@ __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restri
             di[q] = d[i];
 #pragma unroll
             for (int k = 0; k < MAXNZ; k++) {
-                a[q][k] = k < len[q] ? av[j0 + k] : 0.0;
+                a[q][k] = k < len[q] ? mval<VT>(av[j0 + k]) : VT(0);
                 cj[q][k] = k < len[q] ? ci[j0 + k] : 0;
             }
         }
@@ -515,7 +567,7 @@ __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restri
         for (int q = 0; q < RPT; q++) {
             const int i = threadIdx.x + 1024 * q;
             if (i < n) {
-                double t = 0.0;
+                VT t = 0;
 #pragma unroll
                 for (int k = 0; k < MAXNZ; k++)
                     if (k < len[q]) t = t + a[q][k] * cur[cj[q][k]];
@@ -523,7 +575,7 @@ __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restri
             }
         }
         __syncthreads();
-        double *tmp = cur;
+        VT *tmp = cur;
         cur = nxt;
         nxt = tmp;
     }
@@ -536,8 +588,6 @@ __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restri
 
 inline int blocks_for(long long work) { return (int)((work + BS - 1) / BS); }
 
-inline bool stream_ok(const DevCsr &A) { return A.tile > 0 && A.tile <= STREAM_TILE_MAX; }
-
 inline int red_blocks(int n)
 {
     int nb = (int)(((long long)n + BS * 4 - 1) / (BS * 4));
@@ -546,46 +596,75 @@ inline int red_blocks(int n)
     return nb;
 }
 
-template <int OP>
-void stream_dispatch(hipStream_t s, const DevCsr &A, const double *x, const double *f,
-                     const double *d, double *y)
+template <typename VT>
+inline bool stream_ok(const DevCsr &A)
 {
-    const int n = A.n;
-    const int nb = (n + BS - 1) / BS;
-    const size_t lds = (size_t)A.tile * sizeof(double);
-    if ((size_t)A.nnz * 12 > NT_BYTES)
-        k_rows_stream<OP, true><<<nb, BS, lds, s>>>(n, A.row.get(), A.col.get(), A.val.get(), x,
-                                                     f, d, y);
-    else
-        k_rows_stream<OP, false><<<nb, BS, lds, s>>>(n, A.row.get(), A.col.get(), A.val.get(), x,
-                                                      f, d, y);
+    return A.tile > 0 && (size_t)A.tile * sizeof(VT) <= (size_t)STREAM_TILE_BYTES;
 }
 
-template <int OP>
-void rows_dispatch(hipStream_t s, const DevCsr &A, const double *x, const double *f,
-                   const double *d, double *y)
+inline size_t mt_bytes(MatType mt) { return mt == MT_F64 ? 8 : (mt == MT_F32 ? 4 : 2); }
+
+template <typename MT, typename VT, int OP>
+void rows_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x, const VT *f,
+                const VT *d, VT *y)
 {
     const int n = A.n;
-    if (stream_ok(A)) {
-        stream_dispatch<OP>(s, A, x, f, d, y);
+    const int *rp = A.row.get(), *ci = A.col.get();
+    if (stream_ok<VT>(A)) {
+        const int nb = (n + BS - 1) / BS;
+        const size_t lds = (size_t)A.tile * sizeof(VT);
+        if ((size_t)A.nnz * (4 + sizeof(MT)) > NT_BYTES)
+            k_rows_stream<MT, VT, OP, true><<<nb, BS, lds, s>>>(n, rp, ci, av, x, f, d, y);
+        else
+            k_rows_stream<MT, VT, OP, false><<<nb, BS, lds, s>>>(n, rp, ci, av, x, f, d, y);
         return;
     }
     const int G = A.group;
     const int nb = blocks_for((long long)n * G);
-    const int *rp = A.row.get(), *ci = A.col.get();
-    const double *av = A.val.get();
     switch (G) {
-    case 1: k_rows<1, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
-    case 2: k_rows<2, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
-    case 4: k_rows<4, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
-    case 8: k_rows<8, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
-    case 16: k_rows<16, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
-    case 32: k_rows<32, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
-    default: k_rows<64, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
+    case 1: k_rows<1, MT, VT, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
+    case 2: k_rows<2, MT, VT, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
+    case 4: k_rows<4, MT, VT, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
+    case 8: k_rows<8, MT, VT, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
+    case 16: k_rows<16, MT, VT, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
+    case 32: k_rows<32, MT, VT, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
+    default: k_rows<64, MT, VT, OP><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y); break;
     }
 }
 
+template <typename MT, typename VT>
+void rows_op(hipStream_t s, const DevCsr &A, const MT *av, RowOp op, const VT *x, const VT *f,
+             const VT *d, VT *y)
+{
+    if (op == OP_SPMV)
+        rows_typed<MT, VT, OP_SPMV>(s, A, av, x, f, d, y);
+    else if (op == OP_RESID)
+        rows_typed<MT, VT, OP_RESID>(s, A, av, x, f, d, y);
+    else
+        rows_typed<MT, VT, OP_JACOBI>(s, A, av, x, f, d, y);
+}
+
+template <typename MT, typename VT>
+void relax_block_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *d, const VT *f,
+                       const VT *xin, VT *xout, int sweeps, int z)
+{
+    const size_t lds = (size_t)2 * A.n * sizeof(VT);
+    const int *rp = A.row.get(), *ci = A.col.get();
+    if (A.n <= 1024 && A.maxrow <= 8)
+        k_relax_block<MT, VT, 1, 8><<<1, 1024, lds, s>>>(A.n, rp, ci, av, d, f, xin, xout, sweeps, z);
+    else if (A.n <= 4096 && A.maxrow <= 8)
+        k_relax_block<MT, VT, 4, 8><<<1, 1024, lds, s>>>(A.n, rp, ci, av, d, f, xin, xout, sweeps, z);
+    else if (A.n <= 2048 && A.maxrow <= 16)
+        k_relax_block<MT, VT, 2, 16><<<1, 1024, lds, s>>>(A.n, rp, ci, av, d, f, xin, xout, sweeps, z);
+    else if (A.n <= 1024 && A.maxrow <= 32)
+        k_relax_block<MT, VT, 1, 32><<<1, 1024, lds, s>>>(A.n, rp, ci, av, d, f, xin, xout, sweeps, z);
+    else
+        throw Error(BCMG_EINVAL, "relax_block: level too large");
+}
+
 } // namespace
+
+/* ============================================================== host */
 
 int lanes_for(int n, long long nnz)
 {
@@ -629,61 +708,149 @@ void analyze_rows(Ctx &c, DevCsr &A)
     resolve_tiles(c, {&A});
 }
 
+void DevCsr::make_lowp(hipStream_t s, bool bf16)
+{
+    if (nnz <= 0) return;
+    const size_t nb = ((size_t)nnz + BS - 1) / BS;
+    if (bf16) {
+        if (!val16.get()) val16.alloc(nnz);
+        k_to_bf16<<<nb, BS, 0, s>>>((size_t)nnz, val.get(), val16.get());
+    } else {
+        if (!val32.get()) val32.alloc(nnz);
+        k_convert<double, float><<<nb, BS, 0, s>>>((size_t)nnz, val.get(), val32.get());
+    }
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_rows(hipStream_t s, const DevCsr &A, MatType mt, RowOp op, const VT *x, const VT *f,
+                 const VT *d, VT *y)
+{
+    if (A.n <= 0) return;
+    if constexpr (std::is_same<VT, double>::value) {
+        if (mt != MT_F64) throw Error(BCMG_EINVAL, "FP64 vectors need FP64 matrix values");
+        rows_op<double, double>(s, A, A.val.get(), op, x, f, d, y);
+    } else {
+        if (mt == MT_F32)
+            rows_op<float, float>(s, A, A.val32.get(), op, x, f, d, y);
+        else if (mt == MT_BF16)
+            rows_op<unsigned short, float>(s, A, A.val16.get(), op, x, f, d, y);
+        else
+            rows_op<double, float>(s, A, A.val.get(), op, x, f, d, y);
+    }
+    BCMG_CHECK_LAUNCH();
+}
+
 void launch_rows(hipStream_t s, const DevCsr &A, RowOp op, const double *x, const double *f,
                  const double *d, double *y)
 {
-    if (A.n <= 0) return;
-    if (op == OP_SPMV)
-        rows_dispatch<OP_SPMV>(s, A, x, f, d, y);
-    else if (op == OP_RESID)
-        rows_dispatch<OP_RESID>(s, A, x, f, d, y);
-    else
-        rows_dispatch<OP_JACOBI>(s, A, x, f, d, y);
+    launch_rows<double>(s, A, MT_F64, op, x, f, d, y);
+}
+
+template <typename VT>
+bool relax_block_ok(const DevCsr &A)
+{
+    if (A.maxrow <= 0) return false;
+    return (A.n <= 4096 && A.maxrow <= 8) || (A.n <= 2048 && A.maxrow <= 16) ||
+           (A.n <= 1024 && A.maxrow <= 32);
+}
+
+template <typename VT>
+void launch_relax_block(hipStream_t s, const DevCsr &A, MatType mt, const VT *d, const VT *f,
+                        const VT *xin, VT *xout, int sweeps, bool zero)
+{
+    const int z = zero ? 1 : 0;
+    if constexpr (std::is_same<VT, double>::value) {
+        relax_block_typed<double, double>(s, A, A.val.get(), d, f, xin, xout, sweeps, z);
+    } else {
+        if (mt == MT_F32)
+            relax_block_typed<float, float>(s, A, A.val32.get(), d, f, xin, xout, sweeps, z);
+        else if (mt == MT_BF16)
+            relax_block_typed<unsigned short, float>(s, A, A.val16.get(), d, f, xin, xout, sweeps, z);
+        else
+            relax_block_typed<double, float>(s, A, A.val.get(), d, f, xin, xout, sweeps, z);
+    }
     BCMG_CHECK_LAUNCH();
 }
 
-void launch_jacobi_zero(hipStream_t s, int n, const double *f, const double *d, double *y)
+template <typename VT>
+void launch_jacobi_zero(hipStream_t s, int n, const VT *f, const VT *d, VT *y)
 {
     if (n <= 0) return;
-    k_jacobi_zero<<<blocks_for(n), BS, 0, s>>>(n, f, d, y);
+    k_jacobi_zero<VT><<<blocks_for(n), BS, 0, s>>>(n, f, d, y);
     BCMG_CHECK_LAUNCH();
 }
 
-void launch_prolong(hipStream_t s, int n, const int *agg, const double *p, const double *xc,
-                    double *x)
+template <typename VT>
+void launch_prolong(hipStream_t s, int n, const int *agg, const VT *p, const VT *xc, VT *x)
 {
     if (n <= 0) return;
-    k_prolong<<<blocks_for(n), BS, 0, s>>>(n, agg, p, xc, x);
+    k_prolong<VT><<<blocks_for(n), BS, 0, s>>>(n, agg, p, xc, x);
     BCMG_CHECK_LAUNCH();
 }
 
-void launch_fill(hipStream_t s, int n, double v, double *x)
+template <typename VT>
+void launch_fill(hipStream_t s, int n, VT v, VT *x)
 {
     if (n <= 0) return;
-    k_fill<<<blocks_for(n), BS, 0, s>>>(n, v, x);
+    k_fill<VT><<<blocks_for(n), BS, 0, s>>>(n, v, x);
     BCMG_CHECK_LAUNCH();
 }
 
-void launch_copy(hipStream_t s, int n, const double *a, double *b)
+template <typename VT>
+void launch_copy(hipStream_t s, int n, const VT *a, VT *b)
 {
     if (n <= 0 || a == b) return;
-    k_copy<<<blocks_for(n), BS, 0, s>>>(n, a, b);
+    k_copy<VT><<<blocks_for(n), BS, 0, s>>>(n, a, b);
     BCMG_CHECK_LAUNCH();
 }
 
-void launch_axpy(hipStream_t s, int n, double alpha, const double *x, double *y)
+template <typename VT>
+void launch_axpy(hipStream_t s, int n, VT alpha, const VT *x, VT *y)
 {
     if (n <= 0) return;
-    k_axpy<<<blocks_for(n), BS, 0, s>>>(n, alpha, x, y);
+    k_axpy<VT><<<blocks_for(n), BS, 0, s>>>(n, alpha, x, y);
     BCMG_CHECK_LAUNCH();
 }
 
-void launch_scale(hipStream_t s, int n, double alpha, double *x)
+template <typename VT>
+void launch_scale(hipStream_t s, int n, VT alpha, VT *x)
 {
     if (n <= 0) return;
-    k_scale<<<blocks_for(n), BS, 0, s>>>(n, alpha, x);
+    k_scale<VT><<<blocks_for(n), BS, 0, s>>>(n, alpha, x);
     BCMG_CHECK_LAUNCH();
 }
+
+void launch_d2f(hipStream_t s, size_t n, const double *a, float *b)
+{
+    if (!n) return;
+    k_convert<double, float><<<(n + BS - 1) / BS, BS, 0, s>>>(n, a, b);
+    BCMG_CHECK_LAUNCH();
+}
+
+void launch_f2d(hipStream_t s, size_t n, const float *a, double *b)
+{
+    if (!n) return;
+    k_convert<float, double><<<(n + BS - 1) / BS, BS, 0, s>>>(n, a, b);
+    BCMG_CHECK_LAUNCH();
+}
+
+#define BCMG_INSTANTIATE(VT)                                                                 \
+    template void launch_rows<VT>(hipStream_t, const DevCsr &, MatType, RowOp, const VT *,   \
+                                  const VT *, const VT *, VT *);                             \
+    template bool relax_block_ok<VT>(const DevCsr &);                                        \
+    template void launch_relax_block<VT>(hipStream_t, const DevCsr &, MatType, const VT *,   \
+                                         const VT *, const VT *, VT *, int, bool);           \
+    template void launch_jacobi_zero<VT>(hipStream_t, int, const VT *, const VT *, VT *);    \
+    template void launch_prolong<VT>(hipStream_t, int, const int *, const VT *, const VT *,  \
+                                     VT *);                                                  \
+    template void launch_fill<VT>(hipStream_t, int, VT, VT *);                               \
+    template void launch_copy<VT>(hipStream_t, int, const VT *, VT *);                       \
+    template void launch_axpy<VT>(hipStream_t, int, VT, const VT *, VT *);                   \
+    template void launch_scale<VT>(hipStream_t, int, VT, VT *);
+BCMG_INSTANTIATE(double)
+BCMG_INSTANTIATE(float)
+#undef BCMG_INSTANTIATE
 
 void launch_dots(Ctx &c, int n, int k, const double *a0, const double *a1, const double *a2,
                  const double *b, double *out)
@@ -728,21 +895,31 @@ void launch_fcg_first(Ctx &c, int n, double *scal, const double *v, double *x, c
     BCMG_CHECK_LAUNCH();
 }
 
+void launch_true_residual(Ctx &c, const DevCsr &A, const double *b, const double *x, double *y,
+                          double *r, double *scal)
+{
+    const int n = A.n;
+    launch_rows(c.stream, A, OP_SPMV, x, nullptr, nullptr, y);
+    const int nb = red_blocks(n);
+    k_true_residual<<<nb, BS, 0, c.stream>>>(n, b, y, r, c.red.get() + RED_UPD);
+    k_fcg_final<1><<<1, RED_FINAL_BS, 0, c.stream>>>(nb, c.red.get() + RED_UPD, scal);
+    BCMG_CHECK_LAUNCH();
+}
+
 size_t red_size_for(int n) { return (size_t)RED_SPMV + 3 * (size_t)((n + BS - 1) / BS) + 64; }
 
 void launch_spmv_dots(Ctx &c, const DevCsr &A, int k, const double *z, double *t,
                       const double *r, const double *to, double *scal)
 {
     const int n = A.n;
-    if (!stream_ok(A)) {
+    if (!stream_ok<double>(A)) {
         /* long rows: separate SpMV, products and recurrence */
         launch_rows(c.stream, A, OP_SPMV, z, nullptr, nullptr, t);
         launch_dots(c, n, k, r, t, to, z, scal);
-        if (k == 2) {
+        if (k == 2)
             k_fcg_first_scalars<<<1, 64, 0, c.stream>>>(scal);
-        } else {
+        else
             k_fcg_scalars<<<1, 64, 0, c.stream>>>(scal);
-        }
         BCMG_CHECK_LAUNCH();
         return;
     }
@@ -767,35 +944,6 @@ void launch_spmv_dots(Ctx &c, const DevCsr &A, int k, const double *z, double *t
             k_spmv_dots<3, false><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part);
         k_fcg_final<3><<<1, RED_FINAL_BS, 0, c.stream>>>(nb, part, scal);
     }
-    BCMG_CHECK_LAUNCH();
-}
-
-bool relax_block_ok(const DevCsr &A)
-{
-    if (A.maxrow <= 0) return false;
-    return (A.n <= 4096 && A.maxrow <= 8) || (A.n <= 2048 && A.maxrow <= 16) ||
-           (A.n <= 1024 && A.maxrow <= 32);
-}
-
-void launch_relax_block(hipStream_t s, const DevCsr &A, const double *d, const double *f,
-                        const double *xin, double *xout, int sweeps, bool zero)
-{
-    const size_t lds = (size_t)2 * A.n * sizeof(double);
-    const int z = zero ? 1 : 0;
-    if (A.n <= 1024 && A.maxrow <= 8)
-        k_relax_block<1, 8><<<1, 1024, lds, s>>>(A.n, A.row.get(), A.col.get(), A.val.get(), d,
-                                                  f, xin, xout, sweeps, z);
-    else if (A.n <= 4096 && A.maxrow <= 8)
-        k_relax_block<4, 8><<<1, 1024, lds, s>>>(A.n, A.row.get(), A.col.get(), A.val.get(), d,
-                                                  f, xin, xout, sweeps, z);
-    else if (A.n <= 2048 && A.maxrow <= 16)
-        k_relax_block<2, 16><<<1, 1024, lds, s>>>(A.n, A.row.get(), A.col.get(), A.val.get(), d,
-                                                   f, xin, xout, sweeps, z);
-    else if (A.n <= 1024 && A.maxrow <= 32)
-        k_relax_block<1, 32><<<1, 1024, lds, s>>>(A.n, A.row.get(), A.col.get(), A.val.get(), d,
-                                                   f, xin, xout, sweeps, z);
-    else
-        throw Error(BCMG_EINVAL, "relax_block: level too large");
     BCMG_CHECK_LAUNCH();
 }
 

@@ -97,6 +97,25 @@ int bcmg_setup(bcmg_ctx ctx, bcmg_csr A, const bcmg_params *p, bcmg_boot *out);
 /* Replaces AMG::Boot::free (src/AMG/AMG.cu:465-469). */
 int bcmg_boot_free(bcmg_boot B);
 
+/* Precision of the preconditioner (an extension: the reference is FP64
+   only). BCMG_PREC_FP64 is the drop-in. BCMG_PREC_FP32 stores every level's
+   operator, smoother divisors and prolongator in FP32 and runs the cycles in
+   FP32; BCMG_PREC_BF16 also stores the operator values of levels >= 1 as
+   bfloat16 (FP32 arithmetic). In both, FCG's SpMV, inner products, scalar
+   recurrence and updates stay FP64, and when the recursive residual reaches
+   rtol the FP64 true residual b - A x is checked; FCG restarts from it if it
+   is still above rtol ||b||. The hierarchy itself is built in FP64 (same
+   aggregates as the drop-in) and converted. */
+#define BCMG_PREC_FP64 0
+#define BCMG_PREC_FP32 1
+#define BCMG_PREC_BF16 2
+int bcmg_set_precision(bcmg_boot B, int precision);
+/* true-residual restarts made by the last bcmg_fcg on B (0 in FP64, where
+   the stopping test is the reference's recursive one); in a reduced
+   precision the resid bcmg_fcg returns is the checked true ||b - A x||.
+   No counterpart in the reference. */
+int bcmg_fcg_restarts(bcmg_boot B, int *restarts);
+
 /* FCG v3 from x0 = 0 with the composite preconditioner; b and x are device
    pointers of length n. Replaces flexibileConjugateGradients_v3
    (src/SOLVER/FCG.cu:433-585). iters follows the reference's count
