@@ -285,6 +285,21 @@ class Boot:
                               ctypes.byref(res)))
         return it.value, res.value
 
+    def fcg_multi(self, bs, xs):
+        """FCG on several right-hand sides at once (device arrays); returns
+        (iterations, residuals, status) arrays, column j equal to fcg(bs[j])."""
+        k = len(bs)
+        if len(xs) != k:
+            raise ValueError("need as many outputs as right-hand sides")
+        bp = (_vp * k)(*[b.ptr.value for b in bs])
+        xp = (_vp * k)(*[x.ptr.value for x in xs])
+        it = np.zeros(k, np.int32)
+        res = np.zeros(k)
+        st = np.zeros(k, np.int32)
+        _check(lib().bcmg_fcg_multi(self.ctx.h, self.h, k, bp, xp, it.ctypes.data_as(_i32p),
+                                    res.ctypes.data_as(_f64p), st.ctypes.data_as(_i32p)))
+        return it, res, st
+
     def precond(self, r_dev, x_dev):
         _check(lib().bcmg_precond_apply(self.ctx.h, self.h, r_dev.ptr, x_dev.ptr))
 

@@ -194,6 +194,10 @@ struct CycleBufs {
     DBuf<VT> xsave, xadd;       /* additive composite */
 };
 
+struct MultiWork;
+void destroy_multi(MultiWork *m);
+void drop_multi_graph(MultiWork *m);
+
 struct Boot {
     bcmg_params prm{};
     int coarse_solver = 4;
@@ -215,6 +219,8 @@ struct Boot {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     const double *graph_x = nullptr;
     const double *graph_red = nullptr;
+    /* multi-column work space (bcmg_fcg_multi and the serving loop) */
+    MultiWork *multi = nullptr;
     void drop_graphs();
     void ensure_work();
     Boot() = default;
@@ -222,6 +228,32 @@ struct Boot {
     Boot &operator=(const Boot &) = delete;
     ~Boot();
 };
+
+/* stored values a level's kernels read: FP64 for the drop-in; FP32, or
+   bfloat16 on levels >= 1, for the reduced-precision preconditioner */
+inline MatType level_mt(const Boot &B, int l)
+{
+    if (B.prec == BCMG_PREC_FP64) return MT_F64;
+    if (B.prec == BCMG_PREC_BF16 && l > 0) return MT_BF16;
+    return MT_F32;
+}
+/* per-precision views of a level's smoother divisor and prolongator values */
+template <typename VT>
+inline const VT *divisor(const Level &lv, int type)
+{
+    if constexpr (std::is_same<VT, double>::value)
+        return type == 4 ? lv.M.get() : lv.D.get();
+    else
+        return type == 4 ? lv.M32.get() : lv.D32.get();
+}
+template <typename VT>
+inline const VT *pvals(const Level &lv)
+{
+    if constexpr (std::is_same<VT, double>::value)
+        return lv.p.get();
+    else
+        return lv.p32.get();
+}
 
 /* ----------------------------------------------------- launch wrappers */
 /* solve-phase kernels (kernels_solve.hip). VT is the vector/compute type
@@ -268,12 +300,73 @@ void launch_spmv_dots(Ctx &c, const DevCsr &A, int k, const double *z, double *t
                       const double *r, const double *to, double *scal);
 /* doubles of reduction scratch needed for an operator of n rows */
 size_t red_size_for(int n);
-/* all sweeps of a small level in one workgroup (when relax_block_ok(A)) */
+/* all sweeps of a small level in one workgroup (when relax_block_ok(A));
+   with ncols > 1, workgroup j does column j of f/xin/xout (columns ld
+   elements apart), skipped when act[j] == 0 */
 template <typename VT>
 bool relax_block_ok(const DevCsr &A);
 template <typename VT>
 void launch_relax_block(hipStream_t s, const DevCsr &A, MatType mt, const VT *d, const VT *f,
-                        const VT *xin, VT *xout, int sweeps, bool zero);
+                        const VT *xin, VT *xout, int sweeps, bool zero, int ncols = 1,
+                        long ld = 0, const int *act = nullptr);
+
+/* ---------------------------------------- multi-column kernels (kernels_multi.hip)
+   Column blocks: column j of a level's block at base + j * ld. Each kernel
+   applies per column the arithmetic of its single-column counterpart (so a
+   column's bits do not depend on the other columns); act (device, one int
+   per column, nullptr = all) skips columns. */
+constexpr int MULTI_MAX_COLS = 32;
+template <typename VT>
+void launch_rows_multi(hipStream_t s, const DevCsr &A, MatType mt, RowOp op, const VT *x,
+                       long ldx, const VT *f, const VT *d, VT *y, long ldy, int nc,
+                       const int *act);
+template <typename VT>
+void launch_jacobi_zero_multi(hipStream_t s, int n, const VT *f, const VT *d, VT *y, long ld,
+                              int nc, const int *act);
+template <typename VT>
+void launch_prolong_multi(hipStream_t s, int n, const int *agg, const VT *p, const VT *xc,
+                          long ldc, VT *x, long ld, int nc, const int *act);
+template <typename VT>
+void launch_fill_multi(hipStream_t s, int n, VT v, VT *x, long ld, int nc, const int *act);
+template <typename S, typename D>
+void launch_copy_multi(hipStream_t s, int n, const S *a, long lda, D *b, long ldb, int nc,
+                       const int *act);
+template <typename VT>
+void launch_axpy_multi(hipStream_t s, int n, VT alpha, const VT *x, VT *y, long ld, int nc,
+                       const int *act);
+template <typename VT>
+void launch_scale_multi(hipStream_t s, int n, VT alpha, VT *x, long ld, int nc, const int *act);
+
+/* per-column FCG state on the device (FP64 blocks of leading dimension ld):
+   direction pairs (V[s], W[s]), the newest in pair par[j]; Z the
+   preconditioner output; R residual, X iterate, B right-hand side, Y
+   scratch; scal 8 doubles per column (the single-column layout, [7] = the
+   initial ||r||^2); act/it/tol the stopping test of each column. */
+struct MultiFcg {
+    double *V[2] = {nullptr, nullptr}, *W[2] = {nullptr, nullptr};
+    double *Z = nullptr, *R = nullptr, *X = nullptr, *B = nullptr, *Y = nullptr;
+    long ld = 0;
+    int nc = 0;
+    double *scal = nullptr;
+    int *act = nullptr, *par = nullptr, *it = nullptr;
+    double *tol = nullptr;
+    int maxit = 0;
+};
+/* doubles of partials scratch the multi-column FCG kernels need */
+size_t multi_part_size(int n, int nc);
+/* scal[8 j + 7] = ||R_j||^2 for masked columns */
+void launch_norm2_multi(Ctx &c, int n, MultiFcg &M, const int *mask, double *part);
+/* t = A Z with the products and recurrence (k = 2: first step into pair 0;
+   k = 3: iteration into pair par ^ 1) */
+void launch_spmv_dots_multi(Ctx &c, const DevCsr &A, int k, MultiFcg &M, const int *mask,
+                            double *part);
+/* iteration update of the active columns, then their stopping test */
+void launch_fcg_update_multi(Ctx &c, int n, MultiFcg &M, double *part);
+/* first step of the masked columns (direction Z moves to pair 0) */
+void launch_fcg_first_multi(Ctx &c, int n, MultiFcg &M, const int *mask, double *part);
+/* R = B - A X and scal[8 j + 6] = ||R_j|| for masked columns (Y scratch) */
+void launch_true_residual_multi(Ctx &c, const DevCsr &A, MultiFcg &M, const int *mask,
+                                double *part);
 
 /* setup-phase kernels (kernels_setup.hip) */
 void launch_diag(hipStream_t s, const DevCsr &A, double *D);
@@ -307,6 +400,11 @@ void build_boot(Ctx &c, DevCsr &A, const bcmg_params &p, Boot &B);
 void precond_apply(Ctx &c, Boot &B, const double *r, double *x, bool x_is_zero);
 int fcg_solve(Ctx &c, Boot &B, const double *b, double *x, int *iters, double *resid,
               double *delta0_out);
+/* k right-hand sides at once (device pointers b[j], x[j]); per column the
+   iterations, final residual and status (BCMG_OK, BCMG_EZERORHS,
+   BCMG_EBREAKDOWN) of fcg_solve on that column alone, bit for bit */
+void fcg_multi(Ctx &c, Boot &B, int k, const double *const *b, double *const *x, int *iters,
+               double *resid, int *status);
 double a_norm(Ctx &c, const DevCsr &A, const double *x, double *tmp);
 
 }  // namespace bcmg

@@ -406,6 +406,22 @@ int bcmg_fcg(bcmg_ctx ctx, bcmg_boot B, const double *b, double *x, int *iters, 
     API_END
 }
 
+int bcmg_fcg_multi(bcmg_ctx ctx, bcmg_boot B, int k, const double *const *b_dev,
+                   double *const *x_dev, int *iters, double *resid, int *status)
+{
+    if (!ctx || !B || k < 0 || (k > 0 && (!b_dev || !x_dev || !iters || !resid || !status)))
+        return BCMG_EINVAL;
+    for (int j = 0; j < k; j++)
+        if (!b_dev[j] || !x_dev[j]) {
+            set_last_error("bcmg_fcg_multi: null column pointer");
+            return BCMG_EINVAL;
+        }
+    API_BEGIN
+    CtxScope scope_(ctx);
+    fcg_multi(ctx->c, B->B, k, b_dev, x_dev, iters, resid, status);
+    API_END
+}
+
 int bcmg_solve(int n, int nnz, const int *row, const int *col, const double *val,
                const bcmg_params *p, const double *rhs, double *x, int *iters, double *resid,
                double *setup_ms, double *solve_ms)

@@ -149,40 +149,12 @@ struct LevelState {
     bool zero = true;
 };
 
-/* per-precision views of a level's data */
-template <typename VT>
-const VT *divisor(const Level &lv, int type);
-template <>
-const double *divisor<double>(const Level &lv, int type)
-{
-    return type == 4 ? lv.M.get() : lv.D.get();
-}
-template <>
-const float *divisor<float>(const Level &lv, int type)
-{
-    return type == 4 ? lv.M32.get() : lv.D32.get();
-}
-template <typename VT>
-const VT *pvals(const Level &lv);
-template <>
-const double *pvals<double>(const Level &lv) { return lv.p.get(); }
-template <>
-const float *pvals<float>(const Level &lv) { return lv.p32.get(); }
 template <typename VT>
 CycleBufs<VT> &bufs(Boot &B);
 template <>
 CycleBufs<double> &bufs<double>(Boot &B) { return B.b64; }
 template <>
 CycleBufs<float> &bufs<float>(Boot &B) { return B.b32; }
-
-/* stored values a level's kernels read: FP64 for the drop-in; FP32, or
-   bfloat16 on levels >= 1, for the reduced-precision preconditioner */
-MatType level_mt(const Boot &B, int l)
-{
-    if (B.prec == BCMG_PREC_FP64) return MT_F64;
-    if (B.prec == BCMG_PREC_BF16 && l > 0) return MT_BF16;
-    return MT_F32;
-}
 
 template <typename VT>
 struct CycleRun {
@@ -375,9 +347,14 @@ void Boot::drop_graphs()
         g = nullptr;
     }
     graph_x = nullptr;
+    drop_multi_graph(multi);
 }
 
-Boot::~Boot() { drop_graphs(); }
+Boot::~Boot()
+{
+    drop_graphs();
+    destroy_multi(multi);
+}
 
 void Boot::ensure_work()
 {

@@ -1,0 +1,540 @@
+/*
+ * host_multi.hip - the multi-column solve: several right-hand sides on one
+ * composite, each with its own FCG state, sharing every launch.
+ *
+ * Column slots: a MultiWork holds kmax slots; every vector of the cycles and
+ * of FCG is a column block (slot j at base + j * ld). A slot is admitted
+ * (b copied in, ||b||, first FCG step), iterates while its flag in `act` is
+ * set (the stopping test runs on the device), and is retired (x copied
+ * out). The kernels apply per column the arithmetic of the single-column
+ * path, so a slot's iterations, residual and x are those of fcg_solve on
+ * that right-hand side alone, whatever the other slots hold; only the
+ * traffic of the matrices and the launch overhead are shared.
+ *
+ * One iteration of all active slots is one HIP graph replay (the slot
+ * choices live on the device: act, par), followed by one read of the flags.
+ */
+#include "bcmg_internal.h"
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+namespace bcmg {
+
+struct MultiWork {
+    int kmax = 0;
+    int prec = -1;             /* precision the float buffers were sized for */
+    std::vector<long> ld;      /* per level (largest hierarchy) */
+    CycleBufs<double> b64;
+    CycleBufs<float> b32;
+    DBuf<float> r32, z32;      /* level-0 input/output of an FP32 cycle */
+    DBuf<double> vec;          /* V0 V1 W0 W1 Z R X B Y */
+    DBuf<double> scal, tol, part;
+    DBuf<int> flags;           /* act, par, it, mask: kmax each */
+    MultiFcg F;
+    int *mask = nullptr;       /* device mask of admission / check steps */
+    /* pinned staging */
+    double *h_scal = nullptr, *h_tol = nullptr;
+    int *h_flags = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int gnc = 0;
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+    }
+    ~MultiWork()
+    {
+        drop_graph();
+        if (h_scal) (void)hipHostFree(h_scal);
+        if (h_tol) (void)hipHostFree(h_tol);
+        if (h_flags) (void)hipHostFree(h_flags);
+    }
+};
+
+void destroy_multi(MultiWork *m) { delete m; }
+void drop_multi_graph(MultiWork *m)
+{
+    if (m) m->drop_graph();
+}
+
+namespace {
+
+inline long round_ld(size_t n) { return (long)((n + 63) / 64 * 64); }
+
+template <typename VT>
+void ensure_mbufs(const Boot &B, const MultiWork &M, CycleBufs<VT> &W)
+{
+    const size_t L = M.ld.size();
+    if (W.xa.size() < L) {
+        W.xa.resize(L);
+        W.xb.resize(L);
+        W.f.resize(L);
+    }
+    for (size_t l = 0; l < L; l++) {
+        const size_t sz = (size_t)M.kmax * M.ld[l];
+        W.xa[l].ensure(sz);
+        W.xb[l].ensure(sz);
+        W.f[l].ensure(sz);
+    }
+    const size_t s0 = (size_t)M.kmax * M.ld[0];
+    W.res.ensure(s0);
+    if (B.prm.solver_type == 2) {
+        W.xadd.ensure(s0);
+        W.xsave.ensure(s0);
+    }
+}
+
+/* work space for kmax slots (kept and grown across calls) */
+MultiWork &ensure_multi(Ctx &c, Boot &B, int kmax)
+{
+    if (kmax < 1 || kmax > MULTI_MAX_COLS)
+        throw Error(BCMG_EINVAL, "multi-column solve: 1..32 columns per block");
+    if (!B.multi) B.multi = new MultiWork();
+    MultiWork &M = *B.multi;
+    const int n = B.A0->n;
+    const bool grow = kmax > M.kmax;
+    if (grow) {
+        M.drop_graph();
+        M.kmax = kmax;
+        size_t maxL = 0;
+        for (auto &h : B.H) maxL = std::max(maxL, h.L.size());
+        M.ld.assign(maxL, 0);
+        for (size_t l = 0; l < maxL; l++) {
+            size_t nl = 0;
+            for (auto &h : B.H)
+                if (l < h.L.size()) nl = std::max(nl, (size_t)h.L[l].mat().n);
+            M.ld[l] = round_ld(nl);
+        }
+        const size_t blk = (size_t)kmax * M.ld[0];
+        M.vec.alloc(9 * blk);
+        M.scal.alloc((size_t)kmax * 8);
+        M.tol.alloc(kmax);
+        M.part.alloc(multi_part_size(n, kmax));
+        M.flags.alloc(4 * (size_t)kmax);
+        if (M.h_scal) (void)hipHostFree(M.h_scal);
+        if (M.h_tol) (void)hipHostFree(M.h_tol);
+        if (M.h_flags) (void)hipHostFree(M.h_flags);
+        M.h_scal = M.h_tol = nullptr;
+        M.h_flags = nullptr;
+        BCMG_HIP(hipHostMalloc((void **)&M.h_scal, (size_t)kmax * 8 * sizeof(double)));
+        BCMG_HIP(hipHostMalloc((void **)&M.h_tol, (size_t)kmax * sizeof(double)));
+        BCMG_HIP(hipHostMalloc((void **)&M.h_flags, 4 * (size_t)kmax * sizeof(int)));
+        double *v = M.vec.get();
+        MultiFcg &F = M.F;
+        F.V[0] = v;
+        F.V[1] = v + blk;
+        F.W[0] = v + 2 * blk;
+        F.W[1] = v + 3 * blk;
+        F.Z = v + 4 * blk;
+        F.R = v + 5 * blk;
+        F.X = v + 6 * blk;
+        F.B = v + 7 * blk;
+        F.Y = v + 8 * blk;
+        F.ld = M.ld[0];
+        F.scal = M.scal.get();
+        F.tol = M.tol.get();
+        F.act = M.flags.get();
+        F.par = F.act + kmax;
+        F.it = F.act + 2 * kmax;
+        M.mask = F.act + 3 * kmax;
+        /* no slot active, no stale flags */
+        BCMG_HIP(hipMemsetAsync(M.flags.get(), 0, 4 * (size_t)kmax * sizeof(int), c.stream));
+        std::memset(M.h_flags, 0, 4 * (size_t)kmax * sizeof(int));
+        ensure_mbufs<double>(B, M, M.b64);
+        M.prec = -1;
+    }
+    M.F.nc = M.kmax;
+    M.F.maxit = B.prm.itnlim;
+    if (B.prec != BCMG_PREC_FP64 && M.prec != B.prec) {
+        ensure_mbufs<float>(B, M, M.b32);
+        M.r32.ensure((size_t)M.kmax * M.ld[0]);
+        M.z32.ensure((size_t)M.kmax * M.ld[0]);
+        M.prec = B.prec;
+    }
+    return M;
+}
+
+template <typename VT>
+CycleBufs<VT> &mbufs(MultiWork &M);
+template <>
+CycleBufs<double> &mbufs<double>(MultiWork &M) { return M.b64; }
+template <>
+CycleBufs<float> &mbufs<float>(MultiWork &M) { return M.b32; }
+
+template <typename VT>
+struct MState {
+    VT *x = nullptr, *x2 = nullptr;
+    bool zero = true;
+};
+
+template <typename VT>
+struct MRun {
+    Ctx &c;
+    Boot &B;
+    MultiWork &M;
+    const Hierarchy &H;
+    std::vector<MState<VT>> &st;
+    const VT *f0;
+    VT *final_out;
+    int nc;
+    const int *act;
+};
+
+/* relax_level of host_amg.hip on a column block */
+template <typename VT>
+void relax_level_m(MRun<VT> &R, const Level &lv, MatType mt, int type, int sweeps, const VT *f,
+                   MState<VT> &s, VT *final_out, long ld)
+{
+    Ctx &c = R.c;
+    const DevCsr &A = lv.mat();
+    const VT *d = divisor<VT>(lv, type);
+    if (sweeps == 0) return;
+    if (relax_block_ok<VT>(A)) {
+        VT *out = (final_out && s.x != final_out) ? final_out : s.x;
+        launch_relax_block<VT>(c.stream, A, mt, d, f, s.x, out, sweeps, s.zero, R.nc, ld, R.act);
+        if (out != s.x) {
+            s.x2 = s.x;
+            s.x = out;
+        }
+        s.zero = false;
+        return;
+    }
+    for (int k = 0; k < sweeps; k++) {
+        VT *out = s.x2;
+        if (final_out && k == sweeps - 1 && s.x != final_out) out = final_out;
+        if (s.zero)
+            launch_jacobi_zero_multi<VT>(c.stream, A.n, f, d, out, ld, R.nc, R.act);
+        else
+            launch_rows_multi<VT>(c.stream, A, mt, OP_JACOBI, s.x, ld, f, d, out, ld, R.nc,
+                                  R.act);
+        s.x2 = s.x;
+        s.x = out;
+        s.zero = false;
+    }
+}
+
+/* cycle of host_amg.hip on a column block (src/SOLVER/GAMG_cycle.cu:25-140) */
+template <typename VT>
+void cycle_m(MRun<VT> &R, int l)
+{
+    Ctx &c = R.c;
+    Boot &B = R.B;
+    CycleBufs<VT> &W = mbufs<VT>(R.M);
+    const Hierarchy &H = R.H;
+    const int L = H.num_levels();
+    const Level &lv = H.L[l - 1];
+    const MatType mt = level_mt(B, l - 1);
+    MState<VT> &s = R.st[l - 1];
+    const long ld = R.M.ld[l - 1];
+    const VT *f = (l == 1) ? R.f0 : W.f[l - 1].get();
+    const bool top = (l == 1);
+    if (l == L) {
+        relax_level_m<VT>(R, lv, mt, B.coarse_solver, B.prm.relaxnumber_coarse, f, s,
+                          top ? R.final_out : nullptr, ld);
+        return;
+    }
+    relax_level_m<VT>(R, lv, mt, B.prm.relax_type, B.prm.prerelax_sweeps, f, s, nullptr, ld);
+    const DevCsr &A = lv.mat();
+    const long ldc = R.M.ld[l];
+    const VT *rsrc = f;
+    if (!s.zero) {
+        launch_rows_multi<VT>(c.stream, A, mt, OP_RESID, s.x, ld, f, nullptr, W.res.get(), ld,
+                              R.nc, R.act);
+        rsrc = W.res.get();
+    }
+    launch_rows_multi<VT>(c.stream, lv.R, std::is_same<VT, double>::value ? MT_F64 : MT_F32,
+                          OP_SPMV, rsrc, ld, nullptr, nullptr, W.f[l].get(), ldc, R.nc, R.act);
+    MState<VT> &sc = R.st[l];
+    sc.zero = true;
+    for (int i = 1; i <= B.num_grid_sweeps[l - 1]; i++) {
+        cycle_m<VT>(R, l + 1);
+        if (l == L - 1) break;
+    }
+    if (!sc.zero) {
+        if (s.zero) {
+            launch_fill_multi<VT>(c.stream, A.n, VT(0), s.x, ld, R.nc, R.act);
+            s.zero = false;
+        }
+        launch_prolong_multi<VT>(c.stream, A.n, lv.agg.get(), pvals<VT>(lv), sc.x, ldc, s.x, ld,
+                                 R.nc, R.act);
+    }
+    relax_level_m<VT>(R, lv, mt, B.prm.relax_type, B.prm.postrelax_sweeps, f, s,
+                      top ? R.final_out : nullptr, ld);
+}
+
+template <typename VT>
+void run_hierarchy_m(MRun<VT> &R, int k)
+{
+    CycleBufs<VT> &W = mbufs<VT>(R.M);
+    const Hierarchy &H = R.B.H[k];
+    for (int l = 1; l < H.num_levels(); l++) {
+        R.st[l].x = W.xa[l].get();
+        R.st[l].x2 = W.xb[l].get();
+        R.st[l].zero = true;
+    }
+    MRun<VT> Rk{R.c, R.B, R.M, H, R.st, R.f0, R.final_out, R.nc, R.act};
+    cycle_m<VT>(Rk, 1);
+}
+
+/* precond_apply_t of host_amg.hip from x = 0 on a column block */
+template <typename VT>
+void precond_m(Ctx &c, Boot &B, MultiWork &M, const VT *r, VT *x, const int *act)
+{
+    CycleBufs<VT> &W = mbufs<VT>(M);
+    const int n = B.A0->n;
+    const long ld = M.ld[0];
+    const int nc = M.kmax;
+    const int nh = (int)B.H.size();
+    std::vector<MState<VT>> st(M.ld.size());
+    if (B.prm.solver_type == 0 || B.prm.solver_type == 1) {
+        st[0].x = x;
+        st[0].x2 = W.xa[0].get();
+        st[0].zero = true;
+        std::vector<int> order;
+        for (int k = 0; k < nh; k++) order.push_back(k);
+        if (B.prm.solver_type == 1)
+            for (int k = nh - 1; k >= 0; k--) order.push_back(k);
+        for (size_t t = 0; t < order.size(); t++) {
+            const bool last = (t + 1 == order.size());
+            MRun<VT> R{c, B, M, B.H[order[t]], st, r, last ? x : nullptr, nc, act};
+            run_hierarchy_m<VT>(R, order[t]);
+        }
+        if (st[0].zero)
+            launch_fill_multi<VT>(c.stream, n, VT(0), x, ld, nc, act);
+        else if (st[0].x != x)
+            launch_copy_multi<VT, VT>(c.stream, n, st[0].x, ld, x, ld, nc, act);
+    } else {
+        launch_fill_multi<VT>(c.stream, n, VT(0), W.xadd.get(), ld, nc, act);
+        for (int k = 0; k < nh; k++) {
+            st[0].x = W.xb[0].get();
+            st[0].x2 = W.xa[0].get();
+            st[0].zero = true;
+            MRun<VT> R{c, B, M, B.H[k], st, r, nullptr, nc, act};
+            run_hierarchy_m<VT>(R, k);
+            if (st[0].zero) launch_fill_multi<VT>(c.stream, n, VT(0), st[0].x, ld, nc, act);
+            launch_axpy_multi<VT>(c.stream, n, VT(1), st[0].x, W.xadd.get(), ld, nc, act);
+        }
+        launch_scale_multi<VT>(c.stream, n, VT(1) / (VT)nh, W.xadd.get(), ld, nc, act);
+        launch_copy_multi<VT, VT>(c.stream, n, W.xadd.get(), ld, x, ld, nc, act);
+    }
+}
+
+/* Z = B(R) for the masked slots, FP64 or through the FP32 cycle */
+void precond_block(Ctx &c, Boot &B, MultiWork &M, const int *act)
+{
+    const int n = B.A0->n;
+    const long ld = M.ld[0];
+    if (B.prec == BCMG_PREC_FP64) {
+        precond_m<double>(c, B, M, M.F.R, M.F.Z, act);
+        return;
+    }
+    launch_copy_multi<double, float>(c.stream, n, M.F.R, ld, M.r32.get(), ld, M.kmax, act);
+    precond_m<float>(c, B, M, M.r32.get(), M.z32.get(), act);
+    launch_copy_multi<float, double>(c.stream, n, M.z32.get(), ld, M.F.Z, ld, M.kmax, act);
+}
+
+void upload_flags(Ctx &c, MultiWork &M)
+{
+    BCMG_HIP(hipMemcpyAsync(M.flags.get(), M.h_flags, 4 * (size_t)M.kmax * sizeof(int),
+                            hipMemcpyHostToDevice, c.stream));
+}
+
+void read_back(Ctx &c, MultiWork &M, bool flags, bool scal)
+{
+    if (flags)
+        BCMG_HIP(hipMemcpyAsync(M.h_flags, M.flags.get(), 3 * (size_t)M.kmax * sizeof(int),
+                                hipMemcpyDeviceToHost, c.stream));
+    if (scal)
+        BCMG_HIP(hipMemcpyAsync(M.h_scal, M.scal.get(), (size_t)M.kmax * 8 * sizeof(double),
+                                hipMemcpyDeviceToHost, c.stream));
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+}
+
+/* first FCG step of the slots whose host mask entry is set (the flags are
+   uploaded first; act/par/it of those slots come from h_flags) */
+void first_step(Ctx &c, Boot &B, MultiWork &M)
+{
+    const DevCsr &A = *B.A0;
+    upload_flags(c, M);
+    precond_block(c, B, M, M.mask);
+    launch_spmv_dots_multi(c, A, 2, M.F, M.mask, M.part.get());
+    launch_fcg_first_multi(c, A.n, M.F, M.mask, M.part.get());
+}
+
+/* one FCG iteration of every active slot (src/SOLVER/FCG.cu:497-565) */
+void enqueue_iteration_m(Ctx &c, Boot &B, MultiWork &M)
+{
+    const DevCsr &A = *B.A0;
+    precond_block(c, B, M, M.F.act);
+    launch_spmv_dots_multi(c, A, 3, M.F, M.F.act, M.part.get());
+    launch_fcg_update_multi(c, A.n, M.F, M.part.get());
+    BCMG_HIP(hipMemcpyAsync(M.h_flags, M.flags.get(), 3 * (size_t)M.kmax * sizeof(int),
+                            hipMemcpyDeviceToHost, c.stream));
+}
+
+void iterate(Ctx &c, Boot &B, MultiWork &M)
+{
+    if (c.use_graphs) {
+        if (M.gexec && M.gnc != M.kmax) M.drop_graph();
+        if (!M.gexec) {
+            hipGraph_t g = nullptr;
+            BCMG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_iteration_m(c, B, M);
+            } catch (...) {
+                (void)hipStreamEndCapture(c.stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            BCMG_HIP(hipStreamEndCapture(c.stream, &g));
+            const hipError_t e = hipGraphInstantiate(&M.gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            BCMG_HIP(e);
+            M.gnc = M.kmax;
+        }
+        BCMG_HIP(hipGraphLaunch(M.gexec, c.stream));
+    } else {
+        enqueue_iteration_m(c, B, M);
+    }
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+}
+
+} // namespace
+
+/* Per-slot state machine of fcg_solve: admit (||b||, first step), iterate
+   while active, and in a reduced precision check the true residual and
+   restart; every decision is the single-column one. */
+namespace {
+
+void fcg_block(Ctx &c, Boot &B, int k, const double *const *b, double *const *x, int *iters,
+               double *resid, int *status)
+{
+    const DevCsr &A = *B.A0;
+    const int n = A.n;
+    MultiWork &M = ensure_multi(c, B, k);
+    MultiFcg &F = M.F;
+    const long ld = F.ld;
+    const int km = M.kmax;
+    const double rtol = B.prm.rtol;
+    const int max_iter = B.prm.itnlim;
+    int *h_act = M.h_flags, *h_par = M.h_flags + km, *h_it = M.h_flags + 2 * km,
+        *h_mask = M.h_flags + 3 * km;
+    /* admission: B = b, R = b, X = 0, ||b|| */
+    for (int j = 0; j < km; j++) {
+        h_act[j] = 0;
+        h_par[j] = 0;
+        h_it[j] = 0;
+        h_mask[j] = j < k ? 1 : 0;
+    }
+    upload_flags(c, M);
+    for (int j = 0; j < k; j++) launch_copy<double>(c.stream, n, b[j], F.B + (size_t)j * ld);
+    launch_copy_multi<double, double>(c.stream, n, F.B, ld, F.R, ld, km, M.mask);
+    launch_fill_multi<double>(c.stream, n, 0.0, F.X, ld, km, M.mask);
+    launch_norm2_multi(c, n, F, M.mask, M.part.get());
+    read_back(c, M, false, true);
+    std::vector<double> delta0(k), l2(k, 0.0);
+    std::vector<int> state(k, 0); /* 0 running, 1 done */
+    for (int j = 0; j < k; j++) {
+        delta0[j] = std::sqrt(M.h_scal[(size_t)j * 8 + 7]);
+        status[j] = BCMG_OK;
+        iters[j] = 0;
+        resid[j] = 0.0;
+        if (delta0[j] <= DBL_EPSILON * delta0[j]) {
+            /* the reference exits on a zero initial residual (FCG.cu:459-462) */
+            status[j] = BCMG_EZERORHS;
+            state[j] = 1;
+            h_mask[j] = 0;
+        }
+        M.h_tol[j] = rtol * delta0[j];
+    }
+    for (int j = k; j < km; j++) M.h_tol[j] = 0.0;
+    BCMG_HIP(hipMemcpyAsync(M.tol.get(), M.h_tol, (size_t)km * sizeof(double),
+                            hipMemcpyHostToDevice, c.stream));
+    /* every admitted slot runs at least one iteration (FCG.cu:488-497); the
+       first step reads the mask, the iterations act (the pinned flags are
+       not touched again before the next synchronisation) */
+    for (int j = 0; j < k; j++) h_act[j] = h_mask[j];
+    first_step(c, B, M);
+    const bool lowp = B.prec != BCMG_PREC_FP64;
+    for (;;) {
+        bool any = false;
+        for (int j = 0; j < k; j++) any |= h_act[j] != 0;
+        while (any) {
+            iterate(c, B, M);
+            any = false;
+            for (int j = 0; j < k; j++) any |= h_act[j] != 0;
+        }
+        read_back(c, M, true, true);
+        /* every running slot has stopped: the recursive test (FP64: done) */
+        bool check = false;
+        for (int j = 0; j < km; j++) h_mask[j] = 0;
+        for (int j = 0; j < k; j++) {
+            if (state[j]) continue;
+            l2[j] = M.h_scal[(size_t)j * 8 + 6];
+            if (!lowp || !std::isfinite(l2[j])) {
+                state[j] = 1;
+                continue;
+            }
+            h_mask[j] = 1;
+            check = true;
+        }
+        if (!check) break;
+        /* true residual of the stopped slots */
+        upload_flags(c, M);
+        launch_true_residual_multi(c, A, F, M.mask, M.part.get());
+        read_back(c, M, false, true);
+        bool restart = false;
+        for (int j = 0; j < k; j++) {
+            if (!h_mask[j]) continue;
+            l2[j] = M.h_scal[(size_t)j * 8 + 6];
+            if (l2[j] <= M.h_tol[j] || h_it[j] >= max_iter || !std::isfinite(l2[j])) {
+                state[j] = 1;
+                h_mask[j] = 0;
+                continue;
+            }
+            restart = true;
+        }
+        if (!restart) break;
+        /* restart from the true residual: the first step again */
+        for (int j = 0; j < k; j++)
+            if (h_mask[j]) h_par[j] = 0;
+        first_step(c, B, M);
+        read_back(c, M, false, true);
+        for (int j = 0; j < k; j++) {
+            if (!h_mask[j]) continue;
+            B.restarts++;
+            h_it[j]++;
+            l2[j] = M.h_scal[(size_t)j * 8 + 6];
+            h_act[j] = (l2[j] > M.h_tol[j] && h_it[j] < max_iter) ? 1 : 0;
+            h_mask[j] = 0;
+        }
+        upload_flags(c, M);
+    }
+    for (int j = 0; j < k; j++) {
+        if (status[j] == BCMG_EZERORHS) {
+            launch_fill<double>(c.stream, n, 0.0, x[j]);
+            continue;
+        }
+        iters[j] = h_it[j] + 1;
+        resid[j] = l2[j];
+        if (!std::isfinite(l2[j])) status[j] = BCMG_EBREAKDOWN;
+        launch_copy<double>(c.stream, n, F.X + (size_t)j * ld, x[j]);
+    }
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+}
+
+} // namespace
+
+void fcg_multi(Ctx &c, Boot &B, int k, const double *const *b, double *const *x, int *iters,
+               double *resid, int *status)
+{
+    B.restarts = 0;
+    for (int j0 = 0; j0 < k; j0 += MULTI_MAX_COLS) {
+        const int kk = std::min(MULTI_MAX_COLS, k - j0);
+        fcg_block(c, B, kk, b + j0, x + j0, iters + j0, resid + j0, status + j0);
+    }
+}
+
+} // namespace bcmg

@@ -1,0 +1,268 @@
+/*
+ * kernels_common.h - device helpers shared by the single- and multi-column
+ * solve kernels (one copy per translation unit).
+ */
+#pragma once
+#include "bcmg_internal.h"
+
+namespace bcmg {
+
+namespace {
+
+constexpr int BS = 256;          /* threads per block */
+constexpr int RED_MAX_BLOCKS = 1024;
+constexpr int STREAM_TILE_BYTES = 32768;              /* LDS per block */
+/* matrices whose value+column stream exceeds this are read non-temporally */
+constexpr size_t NT_BYTES = (size_t)192 << 20;
+
+/* Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
+   Return the logical block index so that XCD x owns a contiguous range. */
+__device__ __forceinline__ int xcd_block(int bid, int nb)
+{
+    const int q = nb >> 3, rm = nb & 7;
+    const int x = bid & 7, l = bid >> 3;
+    return x * q + (x < rm ? x : rm) + l;
+}
+
+/* stored matrix value -> compute type (bfloat16: upper half of a float) */
+template <typename VT>
+__device__ __forceinline__ VT mval(double v) { return (VT)v; }
+template <typename VT>
+__device__ __forceinline__ VT mval(float v) { return (VT)v; }
+template <typename VT>
+__device__ __forceinline__ VT mval(unsigned short h) { return (VT)__uint_as_float((unsigned)h << 16); }
+
+template <bool NT, typename T>
+__device__ __forceinline__ T ldm(const T *p)
+{
+    if (NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
+constexpr int RED_FINAL_BS = 1024; /* threads of the one-block final reductions */
+
+/* fixed-order block reduction of K partial sums; thread 0 writes out[0..K) */
+template <int K>
+__device__ __forceinline__ void block_reduce_store(double (&s)[K], double *out)
+{
+    __shared__ double sh[K][BS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s[k] += __shfl_down(s[k], o, 64);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) sh[k][wid] = s[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double t = 0.0;
+            for (int w = 0; w < BS / 64; w++) t += sh[k][w];
+            out[k] = t;
+        }
+    }
+}
+
+/* final pass of the plain reductions (one block of BS threads):
+   out[k] = sum_b part[b*K + k], optionally square-rooted */
+template <int K, bool SQRT>
+__device__ __forceinline__ void reduce_final_body(int nb, const double *__restrict__ part,
+                                                  double *__restrict__ out)
+{
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += BS) {
+#pragma unroll
+        for (int k = 0; k < K; k++) s[k] += part[(size_t)b * K + k];
+    }
+    __shared__ double res[K];
+    block_reduce_store<K>(s, res);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) out[k] = SQRT ? sqrt(res[k]) : res[k];
+    }
+}
+
+/* FCG v3 scalar recurrence (src/SOLVER/FCG.cu:531-534) on
+   s = [alpha, beta, gamma, delta, theta, theta2, ...]:
+   theta = gamma / delta; delta = beta - gamma^2 / delta; theta2 = alpha / delta */
+__device__ __forceinline__ void fcg_recurrence(double *scal)
+{
+    const double alpha = scal[0], beta = scal[1], gamma = scal[2];
+    double delta = scal[3];
+    const double theta = gamma / delta;
+    delta = beta - (gamma * gamma) / delta;
+    scal[3] = delta;
+    scal[4] = theta;
+    scal[5] = alpha / delta;
+}
+
+/* first step (src/SOLVER/FCG.cu:475-478): delta = beta, theta = alpha / delta */
+__device__ __forceinline__ void fcg_first_recurrence(double *scal)
+{
+    scal[3] = scal[1];
+    scal[4] = scal[0] / scal[1];
+}
+
+/* Final pass of the FCG reductions (one block of RED_FINAL_BS threads): sums
+   the per-block partials part[b*K + k] in a fixed order into res (valid in
+   thread 0, which alone returns true). */
+template <int K>
+__device__ __forceinline__ bool fcg_final_sum(int nb, const double *__restrict__ part,
+                                              double (&res)[K])
+{
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += RED_FINAL_BS) {
+#pragma unroll
+        for (int k = 0; k < K; k++) s[k] += part[(size_t)b * K + k];
+    }
+    __shared__ double sh[K][RED_FINAL_BS / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s[k] += __shfl_down(s[k], o, 64);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) sh[k][wid] = s[k];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return false;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double t = 0.0;
+        for (int w = 0; w < RED_FINAL_BS / 64; w++) t += sh[k][w];
+        res[k] = t;
+    }
+    return true;
+}
+
+/* what the final pass stores:
+   K = 1: scal[6] = sqrt(sum) (the residual norm);
+   K = 2 (first step, src/SOLVER/FCG.cu:475-478): alpha, beta, delta = beta,
+          theta = alpha / delta;
+   K = 3 (src/SOLVER/FCG.cu:531-534): alpha, beta, gamma and the recurrence */
+template <int K>
+__device__ __forceinline__ void fcg_final_apply(const double (&res)[K], double *scal)
+{
+    if constexpr (K == 1) {
+        scal[6] = sqrt(res[0]);
+    } else if constexpr (K == 2) {
+        scal[0] = res[0];
+        scal[1] = res[1];
+        scal[3] = res[1];
+        scal[4] = res[0] / res[1];
+    } else {
+        scal[0] = res[0];
+        scal[1] = res[1];
+        scal[2] = res[2];
+        fcg_recurrence(scal);
+    }
+}
+
+inline int blocks_for(long long work) { return (int)((work + BS - 1) / BS); }
+
+inline int red_blocks(int n)
+{
+    int nb = (int)(((long long)n + BS * 4 - 1) / (BS * 4));
+    if (nb < 1) nb = 1;
+    if (nb > RED_MAX_BLOCKS) nb = RED_MAX_BLOCKS;
+    return nb;
+}
+
+template <typename VT>
+inline bool stream_ok(const DevCsr &A)
+{
+    return A.tile > 0 && (size_t)A.tile * sizeof(VT) <= (size_t)STREAM_TILE_BYTES;
+}
+
+/* All `sweeps` Jacobi sweeps of a small level in one workgroup of 1024
+   threads: thread t owns rows t + 1024 q (q < RPT) and keeps their entries
+   (at most MAXNZ per row) in registers for all sweeps; both iterates live in
+   LDS and a barrier separates the sweeps. Row sums are sequential in column
+   order (the order of k_rows_stream), so the result equals `sweeps` separate
+   launches bit for bit. */
+template <typename MT, typename VT, int RPT, int MAXNZ>
+__global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restrict__ rp,
+                                                      const int *__restrict__ ci,
+                                                      const MT *__restrict__ av,
+                                                      const VT *__restrict__ d,
+                                                      const VT *__restrict__ f,
+                                                      const VT *__restrict__ xin,
+                                                      VT *__restrict__ xout, int sweeps,
+                                                      int zero, long ld,
+                                                      const int *__restrict__ act)
+{
+    /* block j works on column j of (f, xin, xout), columns ld apart; columns
+       whose act flag is 0 are skipped (act == nullptr: one column) */
+    if (act && !act[blockIdx.x]) return;
+    f += (size_t)blockIdx.x * ld;
+    xin += (size_t)blockIdx.x * ld;
+    xout += (size_t)blockIdx.x * ld;
+    extern __shared__ double smem[];
+    VT *xs = reinterpret_cast<VT *>(smem);
+    VT *cur = xs, *nxt = xs + n;
+    VT a[RPT][MAXNZ];
+    int cj[RPT][MAXNZ];
+    int len[RPT];
+    VT fi[RPT], di[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int i = threadIdx.x + 1024 * q;
+        len[q] = 0;
+        fi[q] = 0;
+        di[q] = 1;
+        if (i < n) {
+            const int j0 = rp[i];
+            len[q] = rp[i + 1] - j0;
+            fi[q] = f[i];
+            di[q] = d[i];
+#pragma unroll
+            for (int k = 0; k < MAXNZ; k++) {
+                a[q][k] = k < len[q] ? mval<VT>(av[j0 + k]) : VT(0);
+                cj[q][k] = k < len[q] ? ci[j0 + k] : 0;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int i = threadIdx.x + 1024 * q;
+        if (i < n) cur[i] = zero ? fi[q] / di[q] : xin[i];
+    }
+    __syncthreads();
+    for (int s = zero ? 1 : 0; s < sweeps; s++) {
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int i = threadIdx.x + 1024 * q;
+            if (i < n) {
+                VT t = 0;
+#pragma unroll
+                for (int k = 0; k < MAXNZ; k++)
+                    if (k < len[q]) t = t + a[q][k] * cur[cj[q][k]];
+                nxt[i] = (fi[q] - t) / di[q] + cur[i];
+            }
+        }
+        __syncthreads();
+        VT *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int i = threadIdx.x + 1024 * q;
+        if (i < n) xout[i] = cur[i];
+    }
+}
+
+} // namespace
+
+} // namespace bcmg

@@ -1,0 +1,697 @@
+/*
+ * kernels_multi.hip - multi-column ("multi-RHS") forms of the solve kernels.
+ *
+ * A column block holds nc vectors of one level, column j at base + j * ld.
+ * Every kernel applies to column j exactly the arithmetic the single-column
+ * kernel of kernels_solve.hip applies to one vector (same products, same
+ * summation order, same reduction trees over the same block partition), so
+ * a right-hand side solved inside a block gets the same bits as solved on its
+ * own. What is shared is the traffic: a row's column indices and values are
+ * read once for all columns of a chunk.
+ *
+ * Columns whose flag in `act` is 0 are skipped (act == nullptr: all nc).
+ */
+#include "kernels_common.h"
+
+namespace bcmg {
+
+namespace {
+
+constexpr int CH = 8; /* columns per register chunk of the row kernels */
+
+__device__ __forceinline__ bool col_on(int j, int nc, const int *act)
+{
+    return j < nc && (act == nullptr || act[j] != 0);
+}
+
+/* Stream-order row kernel (the arithmetic of k_rows_stream): one row per
+   thread, p = a_ij * x_j rounded, then added left to right from 0.
+   OP_SPMV: y = A x; OP_RESID: y = f - A x; OP_JACOBI: y = (f - A x) / d + x.
+   x has leading dimension ldx, f and y ldy (restriction maps a level of ldx
+   rows onto one of ldy rows). */
+template <typename MT, typename VT, int OP>
+__global__ __launch_bounds__(BS) void k_rows_multi(int n, const int *__restrict__ rp,
+                                                   const int *__restrict__ ci,
+                                                   const MT *__restrict__ av,
+                                                   const VT *__restrict__ x, long ldx,
+                                                   const VT *__restrict__ f,
+                                                   const VT *__restrict__ d,
+                                                   VT *__restrict__ y, long ldy, int nc,
+                                                   const int *__restrict__ act)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r = b * BS + threadIdx.x;
+    if (r >= n) return;
+    const int ra = rp[r], re = rp[r + 1];
+    VT dr = 1;
+    if (OP == OP_JACOBI) dr = d[r];
+    for (int j0 = 0; j0 < nc; j0 += CH) {
+        bool on[CH];
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            on[u] = col_on(j0 + u, nc, act);
+            any |= on[u];
+        }
+        if (!any) continue;
+        VT t[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) t[u] = 0;
+        const VT *xj = x + (size_t)j0 * ldx;
+        for (int e = ra; e < re; e++) {
+            const int c = ci[e];
+            const VT v = mval<VT>(av[e]);
+#pragma unroll
+            for (int u = 0; u < CH; u++)
+                if (on[u]) t[u] = t[u] + v * xj[(size_t)u * ldx + c];
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            if (!on[u]) continue;
+            const size_t o = (size_t)(j0 + u) * ldy + r;
+            if (OP == OP_SPMV)
+                y[o] = t[u];
+            else if (OP == OP_RESID)
+                y[o] = f[o] - t[u];
+            else
+                y[o] = (f[o] - t[u]) / dr + x[(size_t)(j0 + u) * ldx + r];
+        }
+    }
+}
+
+/* Group-order row kernel (the arithmetic of k_rows<G>, used for matrices
+   whose rows are too long for the stream kernel's LDS tile): G lanes per
+   row, lane-strided fma sums, xor-shuffle reduction; one column at a time. */
+template <typename MT, typename VT, int OP>
+__global__ __launch_bounds__(BS) void k_rows_multi_g(int n, int G, const int *__restrict__ rp,
+                                                     const int *__restrict__ ci,
+                                                     const MT *__restrict__ av,
+                                                     const VT *__restrict__ x, long ldx,
+                                                     const VT *__restrict__ f,
+                                                     const VT *__restrict__ d,
+                                                     VT *__restrict__ y, long ldy, int nc,
+                                                     const int *__restrict__ act)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int tid = b * BS + threadIdx.x;
+    const int row = tid / G;
+    const int lane = threadIdx.x & (G - 1);
+    if (row >= n) return;
+    const int j0 = rp[row], j1 = rp[row + 1];
+    for (int j = 0; j < nc; j++) {
+        if (!col_on(j, nc, act)) continue;
+        const VT *xj = x + (size_t)j * ldx;
+        VT t = 0;
+        for (int e = j0 + lane; e < j1; e += G) t = fma(mval<VT>(av[e]), xj[ci[e]], t);
+        for (int o = G >> 1; o > 0; o >>= 1) t += __shfl_xor(t, o, G);
+        if (lane == 0) {
+            const size_t oy = (size_t)j * ldy + row;
+            if (OP == OP_SPMV)
+                y[oy] = t;
+            else if (OP == OP_RESID)
+                y[oy] = f[oy] - t;
+            else
+                y[oy] = (f[oy] - t) / d[row] + xj[row];
+        }
+    }
+}
+
+/* column j = blockIdx.y of the element-wise kernels */
+#define MCOL_BEGIN                                                      \
+    const int j = blockIdx.y;                                           \
+    if (act && !act[j]) return;                                         \
+    const int i = blockIdx.x * BS + threadIdx.x;                        \
+    if (i >= n) return;
+
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_jz_multi(int n, const VT *__restrict__ f,
+                                                 const VT *__restrict__ d, VT *__restrict__ y,
+                                                 long ld, const int *__restrict__ act)
+{
+    MCOL_BEGIN
+    const size_t o = (size_t)j * ld + i;
+    y[o] = f[o] / d[i];
+}
+
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_prolong_multi(int n, const int *__restrict__ agg,
+                                                      const VT *__restrict__ p,
+                                                      const VT *__restrict__ xc, long ldc,
+                                                      VT *__restrict__ x, long ld,
+                                                      const int *__restrict__ act)
+{
+    MCOL_BEGIN
+    const size_t o = (size_t)j * ld + i;
+    x[o] = fma(p[i], xc[(size_t)j * ldc + agg[i]], x[o]);
+}
+
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_fill_multi(int n, VT v, VT *__restrict__ x, long ld,
+                                                   const int *__restrict__ act)
+{
+    MCOL_BEGIN
+    x[(size_t)j * ld + i] = v;
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(BS) void k_copy_multi(int n, const S *__restrict__ a, long lda,
+                                                   D *__restrict__ b, long ldb,
+                                                   const int *__restrict__ act)
+{
+    MCOL_BEGIN
+    b[(size_t)j * ldb + i] = (D)a[(size_t)j * lda + i];
+}
+
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_axpy_multi(int n, VT alpha, const VT *__restrict__ x,
+                                                   VT *__restrict__ y, long ld,
+                                                   const int *__restrict__ act)
+{
+    MCOL_BEGIN
+    const size_t o = (size_t)j * ld + i;
+    y[o] = fma(alpha, x[o], y[o]);
+}
+
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_scale_multi(int n, VT alpha, VT *__restrict__ x, long ld,
+                                                    const int *__restrict__ act)
+{
+    MCOL_BEGIN
+    const size_t o = (size_t)j * ld + i;
+    x[o] = alpha * x[o];
+}
+
+/* ------------------------------------------------------------------ FCG */
+
+/* Per-column FCG state. Directions live in two slot pairs (V[s], W[s]); the
+   newest direction of column j is in pair par[j], an iteration writes the
+   next one into pair par[j] ^ 1. The preconditioner output is the fixed
+   block Z. scal holds 8 doubles per column (the single-column layout). */
+struct FcgCols {
+    double *V0, *V1, *W0, *W1;
+    const double *Z;
+    double *R, *X;
+    const double *B;
+    double *Y;
+    long ld;
+    double *scal;
+    int *act, *par, *it;
+    const double *tol;
+    int maxit;
+};
+
+__device__ __forceinline__ double *slot(double *s0, double *s1, int s, int j, long ld)
+{
+    return (s ? s1 : s0) + (size_t)j * ld;
+}
+
+/* the products of k_dots<K> (fma sums over a grid-stride partition),
+   column j = blockIdx.y, written to part[(j * nb + b) * K + k].
+   ROLE 0: ||r||^2. ROLE 1 (first step) / 2: (r.z, t.z[, to.z]) with t = A z
+   taken from the scratch block Y and stored on the way into the pair the
+   step writes (pair 0, or par ^ 1). */
+template <int K, int ROLE>
+__global__ __launch_bounds__(BS) void k_dots_multi(int n, FcgCols F,
+                                                   const int *__restrict__ mask,
+                                                   double *__restrict__ part)
+{
+    const int j = blockIdx.y;
+    if (mask && !mask[j]) return;
+    const long ld = F.ld;
+    const double *a0 = F.R + (size_t)j * ld, *a1 = nullptr, *a2 = nullptr, *bv = a0;
+    double *tdst = nullptr;
+    if (ROLE != 0) {
+        const int p = F.par[j];
+        a1 = F.Y + (size_t)j * ld;
+        tdst = slot(F.W0, F.W1, ROLE == 1 ? 0 : (p ^ 1), j, ld);
+        a2 = slot(F.W0, F.W1, p, j, ld);
+        bv = F.Z + (size_t)j * ld;
+    }
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = 0.0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const double bi = bv[i];
+        s[0] = fma(a0[i], bi, s[0]);
+        if constexpr (K > 1) {
+            const double ti = a1[i];
+            tdst[i] = ti;
+            s[1] = fma(ti, bi, s[1]);
+        }
+        if constexpr (K > 2) s[2] = fma(a2[i], bi, s[2]);
+    }
+    block_reduce_store<K>(s, part + ((size_t)j * gridDim.x + blockIdx.x) * K);
+}
+
+/* k_reduce_final<K, false> per column (block j), into scal + 8 j (+ off) */
+template <int K>
+__global__ __launch_bounds__(BS) void k_reduce_final_multi(int nb, const double *__restrict__ part,
+                                                           double *__restrict__ out, int off,
+                                                           const int *__restrict__ mask)
+{
+    const int j = blockIdx.x;
+    if (mask && !mask[j]) return;
+    reduce_final_body<K, false>(nb, part + (size_t)j * nb * K, out + (size_t)j * 8 + off);
+}
+
+/* unfused FCG scalar recurrences per column (thread j) */
+__global__ void k_fcg_scalars_multi(double *scal, int nc, int first, const int *__restrict__ mask)
+{
+    const int j = threadIdx.x;
+    if (j >= nc || (mask && !mask[j])) return;
+    if (first)
+        fcg_first_recurrence(scal + (size_t)j * 8);
+    else
+        fcg_recurrence(scal + (size_t)j * 8);
+}
+
+/* SpMV fused with the FCG products (the arithmetic of k_spmv_dots): t = A z,
+   per column the block partials of (r.z, t.z[, to.z]). K = 2: first step, t
+   into pair 0; K = 3: t into pair par ^ 1, to from pair par. One row per
+   thread; columns in register chunks of CH. */
+template <int K>
+__global__ __launch_bounds__(BS) void k_spmv_dots_multi(int n, const int *__restrict__ rp,
+                                                        const int *__restrict__ ci,
+                                                        const double *__restrict__ av, FcgCols F,
+                                                        int nc, const int *__restrict__ mask,
+                                                        double *__restrict__ part)
+{
+    __shared__ double sh[32][K][BS / 64];
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r = b * BS + threadIdx.x;
+    const bool own = r < n;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const long ld = F.ld;
+    int ra = 0, re = 0;
+    if (own) {
+        ra = rp[r];
+        re = rp[r + 1];
+    }
+    for (int j0 = 0; j0 < nc; j0 += CH) {
+        bool on[CH];
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            on[u] = col_on(j0 + u, nc, mask);
+            any |= on[u];
+        }
+        if (!any) continue;
+        double t[CH];
+#pragma unroll
+        for (int u = 0; u < CH; u++) t[u] = 0.0;
+        const double *zj = F.Z + (size_t)j0 * ld;
+        for (int e = ra; e < re; e++) {
+            const int c = ci[e];
+            const double v = av[e];
+#pragma unroll
+            for (int u = 0; u < CH; u++)
+                if (on[u]) t[u] = t[u] + v * zj[(size_t)u * ld + c];
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            if (!on[u]) continue;
+            const int j = j0 + u;
+            double sd[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) sd[k] = 0.0;
+            if (own) {
+                const size_t o = (size_t)j * ld + r;
+                const int p = F.par[j];
+                const double zr = F.Z[o];
+                const double rr = F.R[o];
+                double *tv = slot(F.W0, F.W1, K == 2 ? 0 : (p ^ 1), j, ld);
+                tv[r] = t[u];
+                sd[0] = rr * zr;
+                sd[1] = t[u] * zr;
+                if constexpr (K > 2) sd[K - 1] = slot(F.W0, F.W1, p, j, ld)[r] * zr;
+            }
+            /* the in-wave part of block_reduce_store<K> */
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) sd[k] += __shfl_down(sd[k], o, 64);
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int k = 0; k < K; k++) sh[j][k][wid] = sd[k];
+            }
+        }
+    }
+    __syncthreads();
+    /* the cross-wave part: thread j sums column j's wave results */
+    const int j = threadIdx.x;
+    if (j < nc && col_on(j, nc, mask)) {
+        double *out = part + ((size_t)j * gridDim.x + b) * K;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double t = 0.0;
+            for (int w = 0; w < BS / 64; w++) t += sh[j][k][w];
+            out[k] = t;
+        }
+    }
+}
+
+/* Final pass per column (block j of RED_FINAL_BS threads), the arithmetic
+   of k_fcg_final<K>. With COUNT (K = 1 at the end of an iteration) the
+   column's stopping test follows: it += 1; the column stays active while
+   ||r|| > tol and it < maxit (src/SOLVER/FCG.cu:558-565); its newest
+   direction moves to the other pair. */
+template <int K, bool COUNT>
+__global__ __launch_bounds__(RED_FINAL_BS) void k_fcg_final_multi(int nb,
+                                                                 const double *__restrict__ part,
+                                                                 FcgCols F,
+                                                                 const int *__restrict__ mask)
+{
+    const int j = blockIdx.x;
+    if (mask && !mask[j]) return;
+    double res[K];
+    if (!fcg_final_sum<K>(nb, part + (size_t)j * nb * K, res)) return;
+    double *scal = F.scal + (size_t)j * 8;
+    fcg_final_apply<K>(res, scal);
+    if constexpr (COUNT) {
+        const int it = F.it[j] + 1;
+        F.it[j] = it;
+        F.par[j] ^= 1;
+        if (!(scal[6] > F.tol[j] && it < F.maxit)) F.act[j] = 0;
+    }
+}
+
+/* FCG update (the arithmetic of k_fcg_update): z = -theta zo + Z into the
+   new pair, x += theta2 z; t = -theta to + t; r += -theta2 t; ||r||^2 */
+__global__ __launch_bounds__(BS) void k_fcg_update_multi(int n, FcgCols F,
+                                                         double *__restrict__ part)
+{
+    const int j = blockIdx.y;
+    if (!F.act[j]) return;
+    const long ld = F.ld;
+    const double *scal = F.scal + (size_t)j * 8;
+    const double theta = scal[4], theta2 = scal[5];
+    const int p = F.par[j], q = p ^ 1;
+    const double *zo = slot(F.V0, F.V1, p, j, ld);
+    double *zn = slot(F.V0, F.V1, q, j, ld);
+    const double *zc = F.Z + (size_t)j * ld;
+    const double *to = slot(F.W0, F.W1, p, j, ld);
+    double *t = slot(F.W0, F.W1, q, j, ld);
+    double *x = F.X + (size_t)j * ld;
+    double *r = F.R + (size_t)j * ld;
+    double s[1] = {0.0};
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const double zi = fma(-theta, zo[i], zc[i]);
+        zn[i] = zi;
+        x[i] = fma(theta2, zi, x[i]);
+        const double ti = fma(-theta, to[i], t[i]);
+        t[i] = ti;
+        const double ri = fma(-theta2, ti, r[i]);
+        r[i] = ri;
+        s[0] = fma(ri, ri, s[0]);
+    }
+    block_reduce_store<1>(s, part + (size_t)j * gridDim.x + blockIdx.x);
+}
+
+/* first FCG step (the arithmetic of k_fcg_first): the direction Z moves to
+   pair 0, x += theta v, r -= theta w, ||r||^2 */
+__global__ __launch_bounds__(BS) void k_fcg_first_multi(int n, FcgCols F,
+                                                        const int *__restrict__ mask,
+                                                        double *__restrict__ part)
+{
+    const int j = blockIdx.y;
+    if (!mask[j]) return;
+    const long ld = F.ld;
+    const double theta = F.scal[(size_t)j * 8 + 4];
+    const double *v = F.Z + (size_t)j * ld;
+    double *vs = F.V0 + (size_t)j * ld;
+    const double *w = F.W0 + (size_t)j * ld;
+    double *x = F.X + (size_t)j * ld;
+    double *r = F.R + (size_t)j * ld;
+    double s[1] = {0.0};
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const double vi = v[i];
+        vs[i] = vi;
+        x[i] = fma(theta, vi, x[i]);
+        const double ri = fma(-theta, w[i], r[i]);
+        r[i] = ri;
+        s[0] = fma(ri, ri, s[0]);
+    }
+    block_reduce_store<1>(s, part + (size_t)j * gridDim.x + blockIdx.x);
+}
+
+/* r = b - y (y = A x in F.Y) and ||r||^2 partials (k_true_residual) */
+__global__ __launch_bounds__(BS) void k_true_residual_multi(int n, FcgCols F,
+                                                            const int *__restrict__ mask,
+                                                            double *__restrict__ part)
+{
+    const int j = blockIdx.y;
+    if (!mask[j]) return;
+    const size_t o = (size_t)j * F.ld;
+    const double *b = F.B + o;
+    const double *y = F.Y + o;
+    double *r = F.R + o;
+    double s[1] = {0.0};
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const double ri = b[i] - y[i];
+        r[i] = ri;
+        s[0] = fma(ri, ri, s[0]);
+    }
+    block_reduce_store<1>(s, part + (size_t)j * gridDim.x + blockIdx.x);
+}
+
+/* ----------------------------------------------------------- dispatch */
+
+template <typename MT, typename VT, int OP>
+void rows_multi_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x, long ldx,
+                      const VT *f, const VT *d, VT *y, long ldy, int nc, const int *act)
+{
+    const int n = A.n;
+    const int *rp = A.row.get(), *ci = A.col.get();
+    if (stream_ok<VT>(A)) {
+        k_rows_multi<MT, VT, OP><<<blocks_for(n), BS, 0, s>>>(n, rp, ci, av, x, ldx, f, d, y,
+                                                              ldy, nc, act);
+    } else {
+        const int G = A.group;
+        k_rows_multi_g<MT, VT, OP><<<blocks_for((long long)n * G), BS, 0, s>>>(
+            n, G, rp, ci, av, x, ldx, f, d, y, ldy, nc, act);
+    }
+}
+
+template <typename MT, typename VT>
+void rows_multi_op(hipStream_t s, const DevCsr &A, const MT *av, RowOp op, const VT *x, long ldx,
+                   const VT *f, const VT *d, VT *y, long ldy, int nc, const int *act)
+{
+    if (op == OP_SPMV)
+        rows_multi_typed<MT, VT, OP_SPMV>(s, A, av, x, ldx, f, d, y, ldy, nc, act);
+    else if (op == OP_RESID)
+        rows_multi_typed<MT, VT, OP_RESID>(s, A, av, x, ldx, f, d, y, ldy, nc, act);
+    else
+        rows_multi_typed<MT, VT, OP_JACOBI>(s, A, av, x, ldx, f, d, y, ldy, nc, act);
+}
+
+inline dim3 grid2(int n, int nc) { return dim3((unsigned)blocks_for(n), (unsigned)nc); }
+
+inline FcgCols fcg_cols(const MultiFcg &M)
+{
+    FcgCols F;
+    F.V0 = M.V[0];
+    F.V1 = M.V[1];
+    F.W0 = M.W[0];
+    F.W1 = M.W[1];
+    F.Z = M.Z;
+    F.R = M.R;
+    F.X = M.X;
+    F.B = M.B;
+    F.Y = M.Y;
+    F.ld = M.ld;
+    F.scal = M.scal;
+    F.act = M.act;
+    F.par = M.par;
+    F.it = M.it;
+    F.tol = M.tol;
+    F.maxit = M.maxit;
+    return F;
+}
+
+} // namespace
+
+/* ============================================================== host */
+
+template <typename VT>
+void launch_rows_multi(hipStream_t s, const DevCsr &A, MatType mt, RowOp op, const VT *x,
+                       long ldx, const VT *f, const VT *d, VT *y, long ldy, int nc,
+                       const int *act)
+{
+    if (A.n <= 0 || nc <= 0) return;
+    if (nc > MULTI_MAX_COLS) throw Error(BCMG_EINVAL, "too many columns in one block");
+    if constexpr (std::is_same<VT, double>::value) {
+        if (mt != MT_F64) throw Error(BCMG_EINVAL, "FP64 vectors need FP64 matrix values");
+        rows_multi_op<double, double>(s, A, A.val.get(), op, x, ldx, f, d, y, ldy, nc, act);
+    } else {
+        if (mt == MT_F32)
+            rows_multi_op<float, float>(s, A, A.val32.get(), op, x, ldx, f, d, y, ldy, nc, act);
+        else if (mt == MT_BF16)
+            rows_multi_op<unsigned short, float>(s, A, A.val16.get(), op, x, ldx, f, d, y, ldy,
+                                                 nc, act);
+        else
+            rows_multi_op<double, float>(s, A, A.val.get(), op, x, ldx, f, d, y, ldy, nc, act);
+    }
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_jacobi_zero_multi(hipStream_t s, int n, const VT *f, const VT *d, VT *y, long ld,
+                              int nc, const int *act)
+{
+    if (n <= 0 || nc <= 0) return;
+    k_jz_multi<VT><<<grid2(n, nc), BS, 0, s>>>(n, f, d, y, ld, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_prolong_multi(hipStream_t s, int n, const int *agg, const VT *p, const VT *xc,
+                          long ldc, VT *x, long ld, int nc, const int *act)
+{
+    if (n <= 0 || nc <= 0) return;
+    k_prolong_multi<VT><<<grid2(n, nc), BS, 0, s>>>(n, agg, p, xc, ldc, x, ld, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_fill_multi(hipStream_t s, int n, VT v, VT *x, long ld, int nc, const int *act)
+{
+    if (n <= 0 || nc <= 0) return;
+    k_fill_multi<VT><<<grid2(n, nc), BS, 0, s>>>(n, v, x, ld, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename S, typename D>
+void launch_copy_multi(hipStream_t s, int n, const S *a, long lda, D *b, long ldb, int nc,
+                       const int *act)
+{
+    if (n <= 0 || nc <= 0) return;
+    k_copy_multi<S, D><<<grid2(n, nc), BS, 0, s>>>(n, a, lda, b, ldb, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_axpy_multi(hipStream_t s, int n, VT alpha, const VT *x, VT *y, long ld, int nc,
+                       const int *act)
+{
+    if (n <= 0 || nc <= 0) return;
+    k_axpy_multi<VT><<<grid2(n, nc), BS, 0, s>>>(n, alpha, x, y, ld, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_scale_multi(hipStream_t s, int n, VT alpha, VT *x, long ld, int nc, const int *act)
+{
+    if (n <= 0 || nc <= 0) return;
+    k_scale_multi<VT><<<grid2(n, nc), BS, 0, s>>>(n, alpha, x, ld, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+#define BCMG_INSTANTIATE(VT)                                                                 \
+    template void launch_rows_multi<VT>(hipStream_t, const DevCsr &, MatType, RowOp,         \
+                                        const VT *, long, const VT *, const VT *, VT *,      \
+                                        long, int, const int *);                             \
+    template void launch_jacobi_zero_multi<VT>(hipStream_t, int, const VT *, const VT *,     \
+                                               VT *, long, int, const int *);                \
+    template void launch_prolong_multi<VT>(hipStream_t, int, const int *, const VT *,        \
+                                           const VT *, long, VT *, long, int, const int *);  \
+    template void launch_fill_multi<VT>(hipStream_t, int, VT, VT *, long, int, const int *); \
+    template void launch_axpy_multi<VT>(hipStream_t, int, VT, const VT *, VT *, long, int,   \
+                                        const int *);                                        \
+    template void launch_scale_multi<VT>(hipStream_t, int, VT, VT *, long, int, const int *);
+BCMG_INSTANTIATE(double)
+BCMG_INSTANTIATE(float)
+#undef BCMG_INSTANTIATE
+template void launch_copy_multi<double, double>(hipStream_t, int, const double *, long, double *,
+                                                long, int, const int *);
+template void launch_copy_multi<float, float>(hipStream_t, int, const float *, long, float *,
+                                              long, int, const int *);
+template void launch_copy_multi<double, float>(hipStream_t, int, const double *, long, float *,
+                                               long, int, const int *);
+template void launch_copy_multi<float, double>(hipStream_t, int, const float *, long, double *,
+                                               long, int, const int *);
+
+size_t multi_part_size(int n, int nc)
+{
+    const size_t nbs = (size_t)((n + BS - 1) / BS);
+    const size_t nbr = (size_t)red_blocks(n);
+    return std::max(nbs, nbr) * 3 * (size_t)nc + 64;
+}
+
+void launch_norm2_multi(Ctx &c, int n, MultiFcg &M, const int *mask, double *part)
+{
+    /* ||r||^2 of every masked column into scal[8 j + 7] (launch_dots, k = 1) */
+    const int nb = red_blocks(n);
+    const FcgCols F = fcg_cols(M);
+    k_dots_multi<1, 0><<<dim3(nb, M.nc), BS, 0, c.stream>>>(n, F, mask, part);
+    k_reduce_final_multi<1><<<M.nc, BS, 0, c.stream>>>(nb, part, M.scal, 7, mask);
+    BCMG_CHECK_LAUNCH();
+}
+
+void launch_spmv_dots_multi(Ctx &c, const DevCsr &A, int k, MultiFcg &M, const int *mask,
+                            double *part)
+{
+    const int n = A.n;
+    const FcgCols F = fcg_cols(M);
+    if (!stream_ok<double>(A)) {
+        /* long rows: the single-column path's separate SpMV (k_rows<G>
+           arithmetic into Y), products (k_dots) and recurrence */
+        launch_rows_multi<double>(c.stream, A, MT_F64, OP_SPMV, M.Z, M.ld, nullptr, nullptr,
+                                  M.Y, M.ld, M.nc, mask);
+        const int nbr = red_blocks(n);
+        if (k == 2) {
+            k_dots_multi<2, 1><<<dim3(nbr, M.nc), BS, 0, c.stream>>>(n, F, mask, part);
+            k_reduce_final_multi<2><<<M.nc, BS, 0, c.stream>>>(nbr, part, M.scal, 0, mask);
+        } else {
+            k_dots_multi<3, 2><<<dim3(nbr, M.nc), BS, 0, c.stream>>>(n, F, mask, part);
+            k_reduce_final_multi<3><<<M.nc, BS, 0, c.stream>>>(nbr, part, M.scal, 0, mask);
+        }
+        k_fcg_scalars_multi<<<1, 64, 0, c.stream>>>(M.scal, M.nc, k == 2 ? 1 : 0, mask);
+        BCMG_CHECK_LAUNCH();
+        return;
+    }
+    const int nb = (n + BS - 1) / BS;
+    if (k == 2) {
+        k_spmv_dots_multi<2><<<nb, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(),
+                                                       F, M.nc, mask, part);
+        k_fcg_final_multi<2, false><<<M.nc, RED_FINAL_BS, 0, c.stream>>>(nb, part, F, mask);
+    } else {
+        k_spmv_dots_multi<3><<<nb, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(),
+                                                       F, M.nc, mask, part);
+        k_fcg_final_multi<3, false><<<M.nc, RED_FINAL_BS, 0, c.stream>>>(nb, part, F, mask);
+    }
+    BCMG_CHECK_LAUNCH();
+}
+
+void launch_fcg_update_multi(Ctx &c, int n, MultiFcg &M, double *part)
+{
+    const int nb = red_blocks(n);
+    const FcgCols F = fcg_cols(M);
+    k_fcg_update_multi<<<dim3(nb, M.nc), BS, 0, c.stream>>>(n, F, part);
+    k_fcg_final_multi<1, true><<<M.nc, RED_FINAL_BS, 0, c.stream>>>(nb, part, F, M.act);
+    BCMG_CHECK_LAUNCH();
+}
+
+void launch_fcg_first_multi(Ctx &c, int n, MultiFcg &M, const int *mask, double *part)
+{
+    const int nb = red_blocks(n);
+    const FcgCols F = fcg_cols(M);
+    k_fcg_first_multi<<<dim3(nb, M.nc), BS, 0, c.stream>>>(n, F, mask, part);
+    k_fcg_final_multi<1, false><<<M.nc, RED_FINAL_BS, 0, c.stream>>>(nb, part, F, mask);
+    BCMG_CHECK_LAUNCH();
+}
+
+void launch_true_residual_multi(Ctx &c, const DevCsr &A, MultiFcg &M, const int *mask,
+                                double *part)
+{
+    const int n = A.n;
+    launch_rows_multi<double>(c.stream, A, MT_F64, OP_SPMV, M.X, M.ld, nullptr, nullptr, M.Y,
+                              M.ld, M.nc, mask);
+    const int nb = red_blocks(n);
+    const FcgCols F = fcg_cols(M);
+    k_true_residual_multi<<<dim3(nb, M.nc), BS, 0, c.stream>>>(n, F, mask, part);
+    k_fcg_final_multi<1, false><<<M.nc, RED_FINAL_BS, 0, c.stream>>>(nb, part, F, mask);
+    BCMG_CHECK_LAUNCH();
+}
+
+} // namespace bcmg

@@ -12,44 +12,15 @@
  * floating-point atomics (the reference adds block results with atomicAdd,
  * src/SOLVER/FCG.cu:58-60, so its sums vary run to run).
  */
-#include "bcmg_internal.h"
+#include "kernels_common.h"
 
 namespace bcmg {
 
 namespace {
 
-constexpr int BS = 256;          /* threads per block */
-constexpr int RED_MAX_BLOCKS = 1024;
 /* layout of the context's reduction scratch (doubles) */
 constexpr size_t RED_UPD = 3 * RED_MAX_BLOCKS + 16;   /* update/norm partials */
 constexpr size_t RED_SPMV = 4 * RED_MAX_BLOCKS + 32;  /* SpMV+dots partials */
-constexpr int STREAM_TILE_BYTES = 32768;              /* LDS per block */
-/* matrices whose value+column stream exceeds this are read non-temporally */
-constexpr size_t NT_BYTES = (size_t)192 << 20;
-
-/* Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
-   Return the logical block index so that XCD x owns a contiguous range. */
-__device__ __forceinline__ int xcd_block(int bid, int nb)
-{
-    const int q = nb >> 3, rm = nb & 7;
-    const int x = bid & 7, l = bid >> 3;
-    return x * q + (x < rm ? x : rm) + l;
-}
-
-/* stored matrix value -> compute type (bfloat16: upper half of a float) */
-template <typename VT>
-__device__ __forceinline__ VT mval(double v) { return (VT)v; }
-template <typename VT>
-__device__ __forceinline__ VT mval(float v) { return (VT)v; }
-template <typename VT>
-__device__ __forceinline__ VT mval(unsigned short h) { return (VT)__uint_as_float((unsigned)h << 16); }
-
-template <bool NT, typename T>
-__device__ __forceinline__ T ldm(const T *p)
-{
-    if (NT) return __builtin_nontemporal_load(p);
-    return *p;
-}
 
 /* Row-block kernel: block b owns rows [256 b, 256 b + 256). Phase 1 streams
    the block's slice of val/col with consecutive lanes on consecutive entries
@@ -239,32 +210,6 @@ __global__ __launch_bounds__(BS) void k_to_bf16(size_t n, const double *__restri
     b[i] = (unsigned short)(u >> 16);
 }
 
-/* fixed-order block reduction of K partial sums; thread 0 writes out[0..K) */
-template <int K>
-__device__ __forceinline__ void block_reduce_store(double (&s)[K], double *out)
-{
-    __shared__ double sh[K][BS / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) s[k] += __shfl_down(s[k], o, 64);
-    }
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) sh[k][wid] = s[k];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            double t = 0.0;
-            for (int w = 0; w < BS / 64; w++) t += sh[k][w];
-            out[k] = t;
-        }
-    }
-}
-
 template <int K>
 __global__ __launch_bounds__(BS) void k_dots(int n, const double *__restrict__ a0,
                                              const double *__restrict__ a1,
@@ -289,105 +234,27 @@ template <int K, bool SQRT>
 __global__ __launch_bounds__(BS) void k_reduce_final(int nb, const double *__restrict__ part,
                                                      double *__restrict__ out)
 {
-    double s[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) s[k] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += BS) {
-#pragma unroll
-        for (int k = 0; k < K; k++) s[k] += part[(size_t)b * K + k];
-    }
-    __shared__ double res[K];
-    block_reduce_store<K>(s, res);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) out[k] = SQRT ? sqrt(res[k]) : res[k];
-    }
+    reduce_final_body<K, SQRT>(nb, part, out);
 }
 
-/* FCG v3 scalar recurrence (src/SOLVER/FCG.cu:531-534):
-   theta = gamma / delta; delta = beta - gamma^2 / delta; theta2 = alpha / delta */
+/* FCG v3 scalar recurrence (src/SOLVER/FCG.cu:531-534) for the unfused path */
 __global__ void k_fcg_scalars(double *scal)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const double alpha = scal[0], beta = scal[1], gamma = scal[2];
-        double delta = scal[3];
-        const double theta = gamma / delta;
-        delta = beta - (gamma * gamma) / delta;
-        scal[3] = delta;
-        scal[4] = theta;
-        scal[5] = alpha / delta;
-    }
+    if (threadIdx.x == 0 && blockIdx.x == 0) fcg_recurrence(scal);
 }
 
 /* first step (src/SOLVER/FCG.cu:475-478): delta = beta, theta = alpha / delta */
 __global__ void k_fcg_first_scalars(double *scal)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        scal[3] = scal[1];
-        scal[4] = scal[0] / scal[1];
-    }
+    if (threadIdx.x == 0 && blockIdx.x == 0) fcg_first_recurrence(scal);
 }
-
-/* Final pass of the FCG reductions (one block of RED_FINAL_BS threads): sums
-   the per-block partials part[b*K + k] in a fixed order and applies the
-   scalar recurrence, so the scalars never leave the device.
-   K = 2 (first step, src/SOLVER/FCG.cu:475-478): delta = beta, theta = alpha / delta.
-   K = 3 (src/SOLVER/FCG.cu:531-534): theta = gamma / delta,
-   delta = beta - gamma^2 / delta, theta2 = alpha / delta.
-   K = 1: scal[6] = sqrt(sum) (the residual norm). */
-constexpr int RED_FINAL_BS = 1024;
 
 template <int K>
 __global__ __launch_bounds__(RED_FINAL_BS) void k_fcg_final(int nb, const double *__restrict__ part,
                                                            double *__restrict__ scal)
 {
-    double s[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) s[k] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += RED_FINAL_BS) {
-#pragma unroll
-        for (int k = 0; k < K; k++) s[k] += part[(size_t)b * K + k];
-    }
-    __shared__ double sh[K][RED_FINAL_BS / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) s[k] += __shfl_down(s[k], o, 64);
-    }
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) sh[k][wid] = s[k];
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
     double res[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        double t = 0.0;
-        for (int w = 0; w < RED_FINAL_BS / 64; w++) t += sh[k][w];
-        res[k] = t;
-    }
-    if (K == 1) {
-        scal[6] = sqrt(res[0]);
-    } else if (K == 2) {
-        scal[0] = res[0];
-        scal[1] = res[1];
-        scal[3] = res[1];
-        scal[4] = res[0] / res[1];
-    } else {
-        const double alpha = res[0], beta = res[1], gamma = res[K - 1];
-        double delta = scal[3];
-        scal[0] = alpha;
-        scal[1] = beta;
-        scal[2] = gamma;
-        const double theta = gamma / delta;
-        delta = beta - (gamma * gamma) / delta;
-        scal[3] = delta;
-        scal[4] = theta;
-        scal[5] = alpha / delta;
-    }
+    if (fcg_final_sum<K>(nb, part, res)) fcg_final_apply<K>(res, scal);
 }
 
 /* FCG SpMV fused with its inner products (src/SOLVER/FCG.cu:508-528 and the
@@ -515,95 +382,6 @@ __global__ __launch_bounds__(BS) void k_true_residual(int n, const double *__res
     block_reduce_store<1>(s, part + blockIdx.x);
 }
 
-/* All `sweeps` Jacobi sweeps of a small level in one workgroup of 1024
-   threads: thread t owns rows t + 1024 q (q < RPT) and keeps their entries
-   (at most MAXNZ per row) in registers for all sweeps; both iterates live in
-   LDS and a barrier separates the sweeps. Row sums are sequential in column
-   order (the order of k_rows_stream), so the result equals `sweeps` separate
-   launches bit for bit. */
-template <typename MT, typename VT, int RPT, int MAXNZ>
-__global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restrict__ rp,
-                                                      const int *__restrict__ ci,
-                                                      const MT *__restrict__ av,
-                                                      const VT *__restrict__ d,
-                                                      const VT *__restrict__ f,
-                                                      const VT *__restrict__ xin,
-                                                      VT *__restrict__ xout, int sweeps,
-                                                      int zero)
-{
-    extern __shared__ double smem[];
-    VT *xs = reinterpret_cast<VT *>(smem);
-    VT *cur = xs, *nxt = xs + n;
-    VT a[RPT][MAXNZ];
-    int cj[RPT][MAXNZ];
-    int len[RPT];
-    VT fi[RPT], di[RPT];
-#pragma unroll
-    for (int q = 0; q < RPT; q++) {
-        const int i = threadIdx.x + 1024 * q;
-        len[q] = 0;
-        fi[q] = 0;
-        di[q] = 1;
-        if (i < n) {
-            const int j0 = rp[i];
-            len[q] = rp[i + 1] - j0;
-            fi[q] = f[i];
-            di[q] = d[i];
-#pragma unroll
-            for (int k = 0; k < MAXNZ; k++) {
-                a[q][k] = k < len[q] ? mval<VT>(av[j0 + k]) : VT(0);
-                cj[q][k] = k < len[q] ? ci[j0 + k] : 0;
-            }
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < RPT; q++) {
-        const int i = threadIdx.x + 1024 * q;
-        if (i < n) cur[i] = zero ? fi[q] / di[q] : xin[i];
-    }
-    __syncthreads();
-    for (int s = zero ? 1 : 0; s < sweeps; s++) {
-#pragma unroll
-        for (int q = 0; q < RPT; q++) {
-            const int i = threadIdx.x + 1024 * q;
-            if (i < n) {
-                VT t = 0;
-#pragma unroll
-                for (int k = 0; k < MAXNZ; k++)
-                    if (k < len[q]) t = t + a[q][k] * cur[cj[q][k]];
-                nxt[i] = (fi[q] - t) / di[q] + cur[i];
-            }
-        }
-        __syncthreads();
-        VT *tmp = cur;
-        cur = nxt;
-        nxt = tmp;
-    }
-#pragma unroll
-    for (int q = 0; q < RPT; q++) {
-        const int i = threadIdx.x + 1024 * q;
-        if (i < n) xout[i] = cur[i];
-    }
-}
-
-inline int blocks_for(long long work) { return (int)((work + BS - 1) / BS); }
-
-inline int red_blocks(int n)
-{
-    int nb = (int)(((long long)n + BS * 4 - 1) / (BS * 4));
-    if (nb < 1) nb = 1;
-    if (nb > RED_MAX_BLOCKS) nb = RED_MAX_BLOCKS;
-    return nb;
-}
-
-template <typename VT>
-inline bool stream_ok(const DevCsr &A)
-{
-    return A.tile > 0 && (size_t)A.tile * sizeof(VT) <= (size_t)STREAM_TILE_BYTES;
-}
-
-inline size_t mt_bytes(MatType mt) { return mt == MT_F64 ? 8 : (mt == MT_F32 ? 4 : 2); }
-
 template <typename MT, typename VT, int OP>
 void rows_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x, const VT *f,
                 const VT *d, VT *y)
@@ -646,18 +424,24 @@ void rows_op(hipStream_t s, const DevCsr &A, const MT *av, RowOp op, const VT *x
 
 template <typename MT, typename VT>
 void relax_block_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *d, const VT *f,
-                       const VT *xin, VT *xout, int sweeps, int z)
+                       const VT *xin, VT *xout, int sweeps, int z, int ncols, long ld,
+                       const int *act)
 {
     const size_t lds = (size_t)2 * A.n * sizeof(VT);
     const int *rp = A.row.get(), *ci = A.col.get();
-    if (A.n <= 1024 && A.maxrow <= 8)
-        k_relax_block<MT, VT, 1, 8><<<1, 1024, lds, s>>>(A.n, rp, ci, av, d, f, xin, xout, sweeps, z);
-    else if (A.n <= 4096 && A.maxrow <= 8)
-        k_relax_block<MT, VT, 4, 8><<<1, 1024, lds, s>>>(A.n, rp, ci, av, d, f, xin, xout, sweeps, z);
-    else if (A.n <= 2048 && A.maxrow <= 16)
-        k_relax_block<MT, VT, 2, 16><<<1, 1024, lds, s>>>(A.n, rp, ci, av, d, f, xin, xout, sweeps, z);
-    else if (A.n <= 1024 && A.maxrow <= 32)
-        k_relax_block<MT, VT, 1, 32><<<1, 1024, lds, s>>>(A.n, rp, ci, av, d, f, xin, xout, sweeps, z);
+    const int n = A.n;
+    if (n <= 1024 && A.maxrow <= 8)
+        k_relax_block<MT, VT, 1, 8><<<ncols, 1024, lds, s>>>(n, rp, ci, av, d, f, xin, xout,
+                                                              sweeps, z, ld, act);
+    else if (n <= 4096 && A.maxrow <= 8)
+        k_relax_block<MT, VT, 4, 8><<<ncols, 1024, lds, s>>>(n, rp, ci, av, d, f, xin, xout,
+                                                              sweeps, z, ld, act);
+    else if (n <= 2048 && A.maxrow <= 16)
+        k_relax_block<MT, VT, 2, 16><<<ncols, 1024, lds, s>>>(n, rp, ci, av, d, f, xin, xout,
+                                                               sweeps, z, ld, act);
+    else if (n <= 1024 && A.maxrow <= 32)
+        k_relax_block<MT, VT, 1, 32><<<ncols, 1024, lds, s>>>(n, rp, ci, av, d, f, xin, xout,
+                                                               sweeps, z, ld, act);
     else
         throw Error(BCMG_EINVAL, "relax_block: level too large");
 }
@@ -757,18 +541,24 @@ bool relax_block_ok(const DevCsr &A)
 
 template <typename VT>
 void launch_relax_block(hipStream_t s, const DevCsr &A, MatType mt, const VT *d, const VT *f,
-                        const VT *xin, VT *xout, int sweeps, bool zero)
+                        const VT *xin, VT *xout, int sweeps, bool zero, int ncols, long ld,
+                        const int *act)
 {
     const int z = zero ? 1 : 0;
+    if (ncols <= 0) return;
     if constexpr (std::is_same<VT, double>::value) {
-        relax_block_typed<double, double>(s, A, A.val.get(), d, f, xin, xout, sweeps, z);
+        relax_block_typed<double, double>(s, A, A.val.get(), d, f, xin, xout, sweeps, z, ncols,
+                                          ld, act);
     } else {
         if (mt == MT_F32)
-            relax_block_typed<float, float>(s, A, A.val32.get(), d, f, xin, xout, sweeps, z);
+            relax_block_typed<float, float>(s, A, A.val32.get(), d, f, xin, xout, sweeps, z,
+                                            ncols, ld, act);
         else if (mt == MT_BF16)
-            relax_block_typed<unsigned short, float>(s, A, A.val16.get(), d, f, xin, xout, sweeps, z);
+            relax_block_typed<unsigned short, float>(s, A, A.val16.get(), d, f, xin, xout,
+                                                     sweeps, z, ncols, ld, act);
         else
-            relax_block_typed<double, float>(s, A, A.val.get(), d, f, xin, xout, sweeps, z);
+            relax_block_typed<double, float>(s, A, A.val.get(), d, f, xin, xout, sweeps, z,
+                                             ncols, ld, act);
     }
     BCMG_CHECK_LAUNCH();
 }
@@ -840,7 +630,8 @@ void launch_f2d(hipStream_t s, size_t n, const float *a, double *b)
                                   const VT *, const VT *, VT *);                             \
     template bool relax_block_ok<VT>(const DevCsr &);                                        \
     template void launch_relax_block<VT>(hipStream_t, const DevCsr &, MatType, const VT *,   \
-                                         const VT *, const VT *, VT *, int, bool);           \
+                                         const VT *, const VT *, VT *, int, bool, int, long, \
+                                         const int *);                                       \
     template void launch_jacobi_zero<VT>(hipStream_t, int, const VT *, const VT *, VT *);    \
     template void launch_prolong<VT>(hipStream_t, int, const int *, const VT *, const VT *,  \
                                      VT *);                                                  \

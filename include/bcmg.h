@@ -123,6 +123,15 @@ int bcmg_fcg_restarts(bcmg_boot B, int *restarts);
 int bcmg_fcg(bcmg_ctx ctx, bcmg_boot B, const double *b_dev, double *x_dev, int *iters,
              double *resid);
 
+/* k right-hand sides on one composite at once (an extension: the
+   reference solves one). b_dev[j], x_dev[j] are device pointers of length
+   n; per column j: iters[j], resid[j] as bcmg_fcg reports them and
+   status[j] (BCMG_OK, BCMG_EZERORHS: x = 0, BCMG_EBREAKDOWN). Every launch
+   serves all columns (up to 32 per block; more are done 32 at a time), and
+   column j's results equal bcmg_fcg on b_dev[j] alone bit for bit. */
+int bcmg_fcg_multi(bcmg_ctx ctx, bcmg_boot B, int k, const double *const *b_dev,
+                   double *const *x_dev, int *iters, double *resid, int *status);
+
 /* setup + FCG + free on host arrays: the whole of solve()
    (src/SOLVER/solver.cu:20-129), with x returned in host memory.
    setup_ms / solve_ms (may be NULL) are the two phases' wall times. */
