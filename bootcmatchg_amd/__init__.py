@@ -316,6 +316,66 @@ class Boot:
         return r.value
 
 
+class Server:
+    """Serving loop with continuous batching over cached composites
+    (bcmg_server_*): requests are admitted into column slots of their
+    operator between iterations; each result equals Boot.fcg on its b."""
+
+    def __init__(self, ctx, hbm_budget_bytes=64 << 30, slots_per_operator=8):
+        self.ctx = ctx
+        self.h = _vp()
+        self._io = {}
+        self._boots = []
+        _check(lib().bcmg_server_create(ctx.h, ctypes.c_size_t(int(hbm_budget_bytes)),
+                                        int(slots_per_operator), ctypes.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bcmg_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def add(self, boot):
+        op = ctypes.c_int()
+        _check(lib().bcmg_server_add_operator(self.h, boot.h, ctypes.byref(op)))
+        self._boots.append(boot)
+        return op.value
+
+    def submit(self, op, b_dev, x_dev):
+        rid = ctypes.c_longlong()
+        _check(lib().bcmg_server_submit(self.h, int(op), b_dev.ptr, x_dev.ptr,
+                                        ctypes.byref(rid)))
+        self._io[rid.value] = (b_dev, x_dev)
+        return rid.value
+
+    def step(self):
+        c = ctypes.c_int()
+        _check(lib().bcmg_server_step(self.h, ctypes.byref(c)))
+        return c.value
+
+    def result(self, rid):
+        """None while pending, else dict(iterations, residual, status, latency_ms)."""
+        done, it, st = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        res, lat = ctypes.c_double(), ctypes.c_double()
+        _check(lib().bcmg_server_result(self.h, ctypes.c_longlong(rid), ctypes.byref(done),
+                                        ctypes.byref(it), ctypes.byref(res), ctypes.byref(st),
+                                        ctypes.byref(lat)))
+        if not done.value:
+            return None
+        self._io.pop(rid, None)
+        return dict(iterations=it.value, residual=res.value, status=st.value,
+                    latency_ms=lat.value)
+
+    def info(self):
+        q, r, nl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        by = ctypes.c_size_t()
+        _check(lib().bcmg_server_info(self.h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(by),
+                                      ctypes.byref(nl)))
+        return dict(queued=q.value, running=r.value, bytes=by.value, lanes=nl.value)
+
+
 def solve(A, p, rhs):
     """solve() of the reference (src/SOLVER/solver.cu:20-129) on host arrays:
     setup + FCG from x0 = 0. Returns (x, info) with info = dict(iterations,

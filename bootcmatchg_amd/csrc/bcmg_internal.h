@@ -352,6 +352,84 @@ struct MultiFcg {
     double *tol = nullptr;
     int maxit = 0;
 };
+/* work space of the multi-column solve on one composite: kmax column slots
+   of every cycle and FCG vector, per-slot FCG state, pinned staging */
+struct MultiWork {
+    int kmax = 0;
+    int prec = -1;             /* precision the float buffers were sized for */
+    std::vector<long> ld;      /* per level (largest hierarchy) */
+    CycleBufs<double> b64;
+    CycleBufs<float> b32;
+    DBuf<float> r32, z32;      /* level-0 input/output of an FP32 cycle */
+    DBuf<double> vec;          /* V0 V1 W0 W1 Z R X B Y */
+    DBuf<double> scal, tol, part;
+    DBuf<int> flags;           /* act, par, it, mask: kmax each */
+    MultiFcg F;
+    int *mask = nullptr;       /* device mask of admission / check steps */
+    /* pinned staging */
+    double *h_scal = nullptr, *h_tol = nullptr;
+    int *h_flags = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int gnc = 0, gprec = -1;
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+    }
+    ~MultiWork()
+    {
+        drop_graph();
+        if (h_scal) (void)hipHostFree(h_scal);
+        if (h_tol) (void)hipHostFree(h_tol);
+        if (h_flags) (void)hipHostFree(h_flags);
+    }
+};
+
+MultiWork &ensure_multi_into(Ctx &c, Boot &B, MultiWork &M, int kmax);
+MultiWork &multi_work(Ctx &c, Boot &B, MultiWork *&holder, int kmax);
+/* device bytes a MultiWork of kmax slots takes on composite B */
+size_t multi_work_bytes(const Boot &B, int kmax);
+
+enum SlotState { SLOT_FREE = 0, SLOT_RUN = 1, SLOT_DONE = 2 };
+struct SlotInfo {
+    int state = SLOT_FREE;
+    int status = BCMG_OK;
+    bool zero_rhs = false;
+    int iters = 0, restarts = 0;
+    double l2 = 0.0, delta0 = 0.0;
+};
+
+/* Per-slot FCG over a MultiWork: admit right-hand sides into free slots,
+   launch one iteration of all active slots (queued), wait, collect the slots
+   that stopped (true-residual restarts in a reduced precision), retire done
+   slots. A slot's results are those of fcg_solve on its right-hand side. */
+struct SlotEngine {
+    Ctx &c;
+    Boot &B;
+    MultiWork &M;
+    std::vector<SlotInfo> slots;
+    bool launched = false;
+    SlotEngine(Ctx &c, Boot &B, MultiWork &M);
+    bool any_active() const;
+    int free_slots() const;
+    void admit(const std::vector<int> &js, const std::vector<const double *> &bs);
+    void launch();
+    void wait();
+    int collect();
+    void retire(int j, double *x);
+};
+
+/* the serving loop (server.hip) */
+struct Server;
+Server *server_create(Ctx &c, size_t budget, int slots);
+void server_destroy(Server *S);
+int server_add(Server &S, Boot &B);
+long long server_submit(Server &S, int op, const double *b, double *x);
+int server_step(Server &S);
+bool server_result(Server &S, long long id, int *done, int *iters, double *resid, int *status,
+                   double *latency_ms);
+void server_info(const Server &S, int *queued, int *running, size_t *bytes, int *lanes);
+
 /* doubles of partials scratch the multi-column FCG kernels need */
 size_t multi_part_size(int n, int nc);
 /* scal[8 j + 7] = ||R_j||^2 for masked columns */

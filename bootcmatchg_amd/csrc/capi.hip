@@ -26,6 +26,11 @@ struct bcmg_boot_s {
     bcmg::Boot B;
 };
 
+struct bcmg_server_s {
+    bcmg_ctx ctx = nullptr;
+    bcmg::Server *S = nullptr;
+};
+
 namespace bcmg {
 
 static thread_local std::string g_last_error;
@@ -420,6 +425,86 @@ int bcmg_fcg_multi(bcmg_ctx ctx, bcmg_boot B, int k, const double *const *b_dev,
     CtxScope scope_(ctx);
     fcg_multi(ctx->c, B->B, k, b_dev, x_dev, iters, resid, status);
     API_END
+}
+
+int bcmg_server_create(bcmg_ctx ctx, size_t hbm_budget_bytes, int slots_per_operator,
+                       bcmg_server *out)
+{
+    if (!ctx || !out) return BCMG_EINVAL;
+    *out = nullptr;
+    API_BEGIN
+    CtxScope scope_(ctx);
+    auto *h = new bcmg_server_s();
+    try {
+        h->ctx = ctx;
+        h->S = server_create(ctx->c, hbm_budget_bytes, slots_per_operator);
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    *out = h;
+    API_END
+}
+
+int bcmg_server_destroy(bcmg_server S)
+{
+    if (!S) return BCMG_OK;
+    API_BEGIN
+    CtxScope scope_(S->ctx);
+    server_destroy(S->S);
+    delete S;
+    API_END
+}
+
+int bcmg_server_add_operator(bcmg_server S, bcmg_boot B, int *op_id)
+{
+    if (!S || !B || !op_id) return BCMG_EINVAL;
+    if (B->ctx != S->ctx) {
+        set_last_error("the composite belongs to another context");
+        return BCMG_EINVAL;
+    }
+    API_BEGIN
+    CtxScope scope_(S->ctx);
+    *op_id = server_add(*S->S, B->B);
+    API_END
+}
+
+int bcmg_server_submit(bcmg_server S, int op_id, const double *b_dev, double *x_dev,
+                       long long *req_id)
+{
+    if (!S || !b_dev || !x_dev || !req_id) return BCMG_EINVAL;
+    API_BEGIN
+    CtxScope scope_(S->ctx);
+    *req_id = server_submit(*S->S, op_id, b_dev, x_dev);
+    API_END
+}
+
+int bcmg_server_step(bcmg_server S, int *completed)
+{
+    if (!S) return BCMG_EINVAL;
+    API_BEGIN
+    CtxScope scope_(S->ctx);
+    const int c = server_step(*S->S);
+    if (completed) *completed = c;
+    API_END
+}
+
+int bcmg_server_result(bcmg_server S, long long req_id, int *done, int *iters, double *resid,
+                       int *status, double *latency_ms)
+{
+    if (!S) return BCMG_EINVAL;
+    if (!server_result(*S->S, req_id, done, iters, resid, status, latency_ms)) {
+        set_last_error("unknown (or already returned) request id");
+        return BCMG_EINVAL;
+    }
+    return BCMG_OK;
+}
+
+int bcmg_server_info(bcmg_server S, int *queued, int *running, size_t *bytes_in_use, int *lanes)
+{
+    if (!S) return BCMG_EINVAL;
+    server_info(*S->S, queued, running, bytes_in_use, lanes);
+    return BCMG_OK;
 }
 
 int bcmg_solve(int n, int nnz, const int *row, const int *col, const double *val,

@@ -22,37 +22,6 @@
 
 namespace bcmg {
 
-struct MultiWork {
-    int kmax = 0;
-    int prec = -1;             /* precision the float buffers were sized for */
-    std::vector<long> ld;      /* per level (largest hierarchy) */
-    CycleBufs<double> b64;
-    CycleBufs<float> b32;
-    DBuf<float> r32, z32;      /* level-0 input/output of an FP32 cycle */
-    DBuf<double> vec;          /* V0 V1 W0 W1 Z R X B Y */
-    DBuf<double> scal, tol, part;
-    DBuf<int> flags;           /* act, par, it, mask: kmax each */
-    MultiFcg F;
-    int *mask = nullptr;       /* device mask of admission / check steps */
-    /* pinned staging */
-    double *h_scal = nullptr, *h_tol = nullptr;
-    int *h_flags = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int gnc = 0;
-    void drop_graph()
-    {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        gexec = nullptr;
-    }
-    ~MultiWork()
-    {
-        drop_graph();
-        if (h_scal) (void)hipHostFree(h_scal);
-        if (h_tol) (void)hipHostFree(h_tol);
-        if (h_flags) (void)hipHostFree(h_flags);
-    }
-};
-
 void destroy_multi(MultiWork *m) { delete m; }
 void drop_multi_graph(MultiWork *m)
 {
@@ -86,13 +55,13 @@ void ensure_mbufs(const Boot &B, const MultiWork &M, CycleBufs<VT> &W)
     }
 }
 
+} // namespace
+
 /* work space for kmax slots (kept and grown across calls) */
-MultiWork &ensure_multi(Ctx &c, Boot &B, int kmax)
+MultiWork &ensure_multi_into(Ctx &c, Boot &B, MultiWork &M, int kmax)
 {
     if (kmax < 1 || kmax > MULTI_MAX_COLS)
         throw Error(BCMG_EINVAL, "multi-column solve: 1..32 columns per block");
-    if (!B.multi) B.multi = new MultiWork();
-    MultiWork &M = *B.multi;
     const int n = B.A0->n;
     const bool grow = kmax > M.kmax;
     if (grow) {
@@ -155,6 +124,8 @@ MultiWork &ensure_multi(Ctx &c, Boot &B, int kmax)
     }
     return M;
 }
+
+namespace {
 
 template <typename VT>
 CycleBufs<VT> &mbufs(MultiWork &M);
@@ -374,10 +345,89 @@ void enqueue_iteration_m(Ctx &c, Boot &B, MultiWork &M)
                             hipMemcpyDeviceToHost, c.stream));
 }
 
-void iterate(Ctx &c, Boot &B, MultiWork &M)
+} // namespace
+
+/* ------------------------------------------------------------ slot engine */
+
+SlotEngine::SlotEngine(Ctx &c_, Boot &B_, MultiWork &M_) : c(c_), B(B_), M(M_)
 {
+    slots.assign(M.kmax, SlotInfo());
+}
+
+bool SlotEngine::any_active() const
+{
+    for (int j = 0; j < M.kmax; j++)
+        if (slots[j].state == SLOT_RUN && M.h_flags[j]) return true;
+    return false;
+}
+
+int SlotEngine::free_slots() const
+{
+    int f = 0;
+    for (auto &s : slots) f += s.state == SLOT_FREE;
+    return f;
+}
+
+/* Admission of right-hand sides into free slots (fcg_solve's start,
+   src/SOLVER/FCG.cu:445-486): B = b, R = b, X = 0, delta0 = ||b||, the
+   stopping tolerance rtol delta0, then the first step; a slot with b = 0 is
+   done at once with BCMG_EZERORHS (x = 0). Every admitted slot then runs at
+   least one iteration (FCG.cu:488-497). Synchronises once (for ||b||). */
+void SlotEngine::admit(const std::vector<int> &js, const std::vector<const double *> &bs)
+{
+    if (js.empty()) return;
+    MultiFcg &F = M.F;
+    const int n = B.A0->n, km = M.kmax;
+    const long ld = F.ld;
+    int *h_act = M.h_flags, *h_par = M.h_flags + km, *h_it = M.h_flags + 2 * km,
+        *h_mask = M.h_flags + 3 * km;
+    for (int j = 0; j < km; j++) h_mask[j] = 0;
+    for (size_t q = 0; q < js.size(); q++) {
+        const int j = js[q];
+        if (j < 0 || j >= km || slots[j].state != SLOT_FREE)
+            throw Error(BCMG_EINVAL, "admission into a slot that is not free");
+        h_mask[j] = 1;
+        h_act[j] = 0;
+        h_par[j] = 0;
+        h_it[j] = 0;
+        launch_copy<double>(c.stream, n, bs[q], F.B + (size_t)j * ld);
+    }
+    upload_flags(c, M);
+    launch_copy_multi<double, double>(c.stream, n, F.B, ld, F.R, ld, km, M.mask);
+    launch_fill_multi<double>(c.stream, n, 0.0, F.X, ld, km, M.mask);
+    launch_norm2_multi(c, n, F, M.mask, M.part.get());
+    read_back(c, M, false, true);
+    bool any = false;
+    for (int j : js) {
+        SlotInfo &S = slots[j];
+        S = SlotInfo();
+        S.delta0 = std::sqrt(M.h_scal[(size_t)j * 8 + 7]);
+        if (S.delta0 <= DBL_EPSILON * S.delta0) {
+            /* the reference exits on a zero initial residual (FCG.cu:459-462) */
+            S.state = SLOT_DONE;
+            S.status = BCMG_EZERORHS;
+            S.zero_rhs = true;
+            h_mask[j] = 0;
+            continue;
+        }
+        S.state = SLOT_RUN;
+        M.h_tol[j] = B.prm.rtol * S.delta0;
+        h_act[j] = 1;
+        any = true;
+    }
+    if (!any) return;
+    BCMG_HIP(hipMemcpyAsync(M.tol.get(), M.h_tol, (size_t)km * sizeof(double),
+                            hipMemcpyHostToDevice, c.stream));
+    first_step(c, B, M);
+    /* h_flags stays untouched until the next synchronisation */
+}
+
+/* One iteration of every active slot, queued (graph replay); wait() ends it. */
+void SlotEngine::launch()
+{
+    if (!any_active()) return;
+    if (M.gexec && (M.gnc != M.kmax || M.gprec != B.prec)) M.drop_graph();
     if (c.use_graphs) {
-        if (M.gexec && M.gnc != M.kmax) M.drop_graph();
         if (!M.gexec) {
             hipGraph_t g = nullptr;
             BCMG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
@@ -393,134 +443,167 @@ void iterate(Ctx &c, Boot &B, MultiWork &M)
             (void)hipGraphDestroy(g);
             BCMG_HIP(e);
             M.gnc = M.kmax;
+            M.gprec = B.prec;
         }
         BCMG_HIP(hipGraphLaunch(M.gexec, c.stream));
     } else {
         enqueue_iteration_m(c, B, M);
     }
-    BCMG_HIP(hipStreamSynchronize(c.stream));
+    launched = true;
 }
 
-} // namespace
-
-/* Per-slot state machine of fcg_solve: admit (||b||, first step), iterate
-   while active, and in a reduced precision check the true residual and
-   restart; every decision is the single-column one. */
-namespace {
-
-void fcg_block(Ctx &c, Boot &B, int k, const double *const *b, double *const *x, int *iters,
-               double *resid, int *status)
+void SlotEngine::wait()
 {
-    const DevCsr &A = *B.A0;
-    const int n = A.n;
-    MultiWork &M = ensure_multi(c, B, k);
-    MultiFcg &F = M.F;
-    const long ld = F.ld;
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+    launched = false;
+}
+
+/* Slots whose stopping test ended their iterations: FP64 (the reference's
+   recursive test) makes them done; in a reduced precision the true residual
+   decides between done and a restart from it (fcg_solve's rule, per slot).
+   Returns the number of slots that became done. */
+int SlotEngine::collect()
+{
     const int km = M.kmax;
-    const double rtol = B.prm.rtol;
-    const int max_iter = B.prm.itnlim;
     int *h_act = M.h_flags, *h_par = M.h_flags + km, *h_it = M.h_flags + 2 * km,
         *h_mask = M.h_flags + 3 * km;
-    /* admission: B = b, R = b, X = 0, ||b|| */
-    for (int j = 0; j < km; j++) {
-        h_act[j] = 0;
-        h_par[j] = 0;
-        h_it[j] = 0;
-        h_mask[j] = j < k ? 1 : 0;
-    }
-    upload_flags(c, M);
-    for (int j = 0; j < k; j++) launch_copy<double>(c.stream, n, b[j], F.B + (size_t)j * ld);
-    launch_copy_multi<double, double>(c.stream, n, F.B, ld, F.R, ld, km, M.mask);
-    launch_fill_multi<double>(c.stream, n, 0.0, F.X, ld, km, M.mask);
-    launch_norm2_multi(c, n, F, M.mask, M.part.get());
-    read_back(c, M, false, true);
-    std::vector<double> delta0(k), l2(k, 0.0);
-    std::vector<int> state(k, 0); /* 0 running, 1 done */
-    for (int j = 0; j < k; j++) {
-        delta0[j] = std::sqrt(M.h_scal[(size_t)j * 8 + 7]);
-        status[j] = BCMG_OK;
-        iters[j] = 0;
-        resid[j] = 0.0;
-        if (delta0[j] <= DBL_EPSILON * delta0[j]) {
-            /* the reference exits on a zero initial residual (FCG.cu:459-462) */
-            status[j] = BCMG_EZERORHS;
-            state[j] = 1;
-            h_mask[j] = 0;
-        }
-        M.h_tol[j] = rtol * delta0[j];
-    }
-    for (int j = k; j < km; j++) M.h_tol[j] = 0.0;
-    BCMG_HIP(hipMemcpyAsync(M.tol.get(), M.h_tol, (size_t)km * sizeof(double),
-                            hipMemcpyHostToDevice, c.stream));
-    /* every admitted slot runs at least one iteration (FCG.cu:488-497); the
-       first step reads the mask, the iterations act (the pinned flags are
-       not touched again before the next synchronisation) */
-    for (int j = 0; j < k; j++) h_act[j] = h_mask[j];
-    first_step(c, B, M);
     const bool lowp = B.prec != BCMG_PREC_FP64;
+    const int max_iter = B.prm.itnlim;
+    int done = 0;
     for (;;) {
-        bool any = false;
-        for (int j = 0; j < k; j++) any |= h_act[j] != 0;
-        while (any) {
-            iterate(c, B, M);
-            any = false;
-            for (int j = 0; j < k; j++) any |= h_act[j] != 0;
-        }
-        read_back(c, M, true, true);
-        /* every running slot has stopped: the recursive test (FP64: done) */
-        bool check = false;
+        std::vector<int> stopped;
+        for (int j = 0; j < km; j++)
+            if (slots[j].state == SLOT_RUN && !h_act[j]) stopped.push_back(j);
+        if (stopped.empty()) break;
+        read_back(c, M, false, true);
         for (int j = 0; j < km; j++) h_mask[j] = 0;
-        for (int j = 0; j < k; j++) {
-            if (state[j]) continue;
-            l2[j] = M.h_scal[(size_t)j * 8 + 6];
-            if (!lowp || !std::isfinite(l2[j])) {
-                state[j] = 1;
+        bool check = false;
+        for (int j : stopped) {
+            SlotInfo &S = slots[j];
+            S.l2 = M.h_scal[(size_t)j * 8 + 6];
+            S.iters = h_it[j] + 1;
+            if (!lowp || !std::isfinite(S.l2)) {
+                S.state = SLOT_DONE;
+                S.status = std::isfinite(S.l2) ? BCMG_OK : BCMG_EBREAKDOWN;
+                done++;
                 continue;
             }
             h_mask[j] = 1;
             check = true;
         }
         if (!check) break;
-        /* true residual of the stopped slots */
         upload_flags(c, M);
-        launch_true_residual_multi(c, A, F, M.mask, M.part.get());
+        launch_true_residual_multi(c, *B.A0, M.F, M.mask, M.part.get());
         read_back(c, M, false, true);
         bool restart = false;
-        for (int j = 0; j < k; j++) {
+        for (int j : stopped) {
             if (!h_mask[j]) continue;
-            l2[j] = M.h_scal[(size_t)j * 8 + 6];
-            if (l2[j] <= M.h_tol[j] || h_it[j] >= max_iter || !std::isfinite(l2[j])) {
-                state[j] = 1;
+            SlotInfo &S = slots[j];
+            S.l2 = M.h_scal[(size_t)j * 8 + 6];
+            if (S.l2 <= M.h_tol[j] || h_it[j] >= max_iter || !std::isfinite(S.l2)) {
+                S.state = SLOT_DONE;
+                S.status = std::isfinite(S.l2) ? BCMG_OK : BCMG_EBREAKDOWN;
+                done++;
                 h_mask[j] = 0;
                 continue;
             }
+            h_par[j] = 0;
             restart = true;
         }
         if (!restart) break;
         /* restart from the true residual: the first step again */
-        for (int j = 0; j < k; j++)
-            if (h_mask[j]) h_par[j] = 0;
         first_step(c, B, M);
         read_back(c, M, false, true);
-        for (int j = 0; j < k; j++) {
+        for (int j : stopped) {
             if (!h_mask[j]) continue;
+            SlotInfo &S = slots[j];
+            S.restarts++;
             B.restarts++;
             h_it[j]++;
-            l2[j] = M.h_scal[(size_t)j * 8 + 6];
-            h_act[j] = (l2[j] > M.h_tol[j] && h_it[j] < max_iter) ? 1 : 0;
+            S.l2 = M.h_scal[(size_t)j * 8 + 6];
+            S.iters = h_it[j] + 1;
+            h_act[j] = (S.l2 > M.h_tol[j] && h_it[j] < max_iter) ? 1 : 0;
             h_mask[j] = 0;
         }
         upload_flags(c, M);
+        /* slots whose restart already meets the test go round again */
+    }
+    return done;
+}
+
+/* copy a done slot's x out (queued on the slot engine's stream) and free it */
+void SlotEngine::retire(int j, double *x)
+{
+    SlotInfo &S = slots[j];
+    if (S.state != SLOT_DONE) throw Error(BCMG_EINVAL, "retiring a slot that is not done");
+    const int n = B.A0->n;
+    if (x) {
+        if (S.zero_rhs)
+            launch_fill<double>(c.stream, n, 0.0, x);
+        else
+            launch_copy<double>(c.stream, n, M.F.X + (size_t)j * M.F.ld, x);
+    }
+    S.state = SLOT_FREE;
+}
+
+MultiWork &multi_work(Ctx &c, Boot &B, MultiWork *&holder, int kmax)
+{
+    if (!holder) holder = new MultiWork();
+    return ensure_multi_into(c, B, *holder, kmax);
+}
+
+size_t multi_work_bytes(const Boot &B, int kmax)
+{
+    size_t maxL = 0;
+    for (auto &h : B.H) maxL = std::max(maxL, h.L.size());
+    size_t per_col64 = 0, per_col32 = 0, ld0 = 0;
+    for (size_t l = 0; l < maxL; l++) {
+        size_t nl = 0;
+        for (auto &h : B.H)
+            if (l < h.L.size()) nl = std::max(nl, (size_t)h.L[l].mat().n);
+        const size_t ldl = (size_t)round_ld(nl);
+        if (l == 0) ld0 = ldl;
+        per_col64 += 3 * ldl * 8;
+        per_col32 += 3 * ldl * 4;
+    }
+    per_col64 += ld0 * 8 * (1 + 9 + (B.prm.solver_type == 2 ? 2 : 0));
+    per_col32 += ld0 * 4 * (1 + 2 + (B.prm.solver_type == 2 ? 2 : 0));
+    const size_t per_col = per_col64 + (B.prec != BCMG_PREC_FP64 ? per_col32 : 0);
+    return (size_t)kmax * per_col + multi_part_size(B.A0->n, kmax) * 8 + (size_t)kmax * 80;
+}
+
+/* k right-hand sides through one slot engine: admit all, iterate until
+   every slot is done, copy the results out */
+namespace {
+
+void fcg_block(Ctx &c, Boot &B, int k, const double *const *b, double *const *x, int *iters,
+               double *resid, int *status)
+{
+    MultiWork &M = multi_work(c, B, B.multi, k);
+    SlotEngine E(c, B, M);
+    std::vector<int> js(k);
+    std::vector<const double *> bs(k);
+    for (int j = 0; j < k; j++) {
+        js[j] = j;
+        bs[j] = b[j];
+    }
+    E.admit(js, bs);
+    for (;;) {
+        E.collect();
+        if (!E.any_active()) {
+            bool pending = false;
+            for (int j = 0; j < k; j++) pending |= E.slots[j].state == SLOT_RUN;
+            if (!pending) break;
+        }
+        E.launch();
+        E.wait();
     }
     for (int j = 0; j < k; j++) {
-        if (status[j] == BCMG_EZERORHS) {
-            launch_fill<double>(c.stream, n, 0.0, x[j]);
-            continue;
-        }
-        iters[j] = h_it[j] + 1;
-        resid[j] = l2[j];
-        if (!std::isfinite(l2[j])) status[j] = BCMG_EBREAKDOWN;
-        launch_copy<double>(c.stream, n, F.X + (size_t)j * ld, x[j]);
+        const SlotInfo &S = E.slots[j];
+        iters[j] = S.zero_rhs ? 0 : S.iters;
+        resid[j] = S.zero_rhs ? 0.0 : S.l2;
+        status[j] = S.status;
+        E.retire(j, x[j]);
     }
     BCMG_HIP(hipStreamSynchronize(c.stream));
 }

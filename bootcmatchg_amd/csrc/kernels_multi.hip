@@ -25,10 +25,12 @@ __device__ __forceinline__ bool col_on(int j, int nc, const int *act)
 }
 
 /* Stream-order row kernel (the arithmetic of k_rows_stream): one row per
-   thread, p = a_ij * x_j rounded, then added left to right from 0.
+   thread and CH columns per block row (chunk blockIdx.y); p = a_ij * x_j
+   rounded, then added left to right from 0.
    OP_SPMV: y = A x; OP_RESID: y = f - A x; OP_JACOBI: y = (f - A x) / d + x.
    x has leading dimension ldx, f and y ldy (restriction maps a level of ldx
-   rows onto one of ldy rows). */
+   rows onto one of ldy rows). When every column of the chunk is active the
+   gathers are unconditional, so a row's loads are all in flight at once. */
 template <typename MT, typename VT, int OP>
 __global__ __launch_bounds__(BS) void k_rows_multi(int n, const int *__restrict__ rp,
                                                    const int *__restrict__ ci,
@@ -39,25 +41,35 @@ __global__ __launch_bounds__(BS) void k_rows_multi(int n, const int *__restrict_
                                                    VT *__restrict__ y, long ldy, int nc,
                                                    const int *__restrict__ act)
 {
+    const int j0 = blockIdx.y * CH;
+    bool on[CH];
+    bool any = false, all = true;
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+        on[u] = col_on(j0 + u, nc, act);
+        any |= on[u];
+        all &= on[u];
+    }
+    if (!any) return;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int r = b * BS + threadIdx.x;
     if (r >= n) return;
     const int ra = rp[r], re = rp[r + 1];
-    VT dr = 1;
-    if (OP == OP_JACOBI) dr = d[r];
-    for (int j0 = 0; j0 < nc; j0 += CH) {
-        bool on[CH];
-        bool any = false;
+    VT t[CH];
 #pragma unroll
-        for (int u = 0; u < CH; u++) {
-            on[u] = col_on(j0 + u, nc, act);
-            any |= on[u];
+    for (int u = 0; u < CH; u++) t[u] = 0;
+    const VT *xj = x + (size_t)j0 * ldx;
+    if (all) {
+        for (int e = ra; e < re; e++) {
+            const int c = ci[e];
+            const VT v = mval<VT>(av[e]);
+            VT g[CH];
+#pragma unroll
+            for (int u = 0; u < CH; u++) g[u] = xj[(size_t)u * ldx + c];
+#pragma unroll
+            for (int u = 0; u < CH; u++) t[u] = t[u] + v * g[u];
         }
-        if (!any) continue;
-        VT t[CH];
-#pragma unroll
-        for (int u = 0; u < CH; u++) t[u] = 0;
-        const VT *xj = x + (size_t)j0 * ldx;
+    } else {
         for (int e = ra; e < re; e++) {
             const int c = ci[e];
             const VT v = mval<VT>(av[e]);
@@ -65,17 +77,19 @@ __global__ __launch_bounds__(BS) void k_rows_multi(int n, const int *__restrict_
             for (int u = 0; u < CH; u++)
                 if (on[u]) t[u] = t[u] + v * xj[(size_t)u * ldx + c];
         }
+    }
+    VT dr = 1;
+    if (OP == OP_JACOBI) dr = d[r];
 #pragma unroll
-        for (int u = 0; u < CH; u++) {
-            if (!on[u]) continue;
-            const size_t o = (size_t)(j0 + u) * ldy + r;
-            if (OP == OP_SPMV)
-                y[o] = t[u];
-            else if (OP == OP_RESID)
-                y[o] = f[o] - t[u];
-            else
-                y[o] = (f[o] - t[u]) / dr + x[(size_t)(j0 + u) * ldx + r];
-        }
+    for (int u = 0; u < CH; u++) {
+        if (!on[u]) continue;
+        const size_t o = (size_t)(j0 + u) * ldy + r;
+        if (OP == OP_SPMV)
+            y[o] = t[u];
+        else if (OP == OP_RESID)
+            y[o] = f[o] - t[u];
+        else
+            y[o] = (f[o] - t[u]) / dr + x[(size_t)(j0 + u) * ldx + r];
     }
 }
 
@@ -268,7 +282,7 @@ __global__ void k_fcg_scalars_multi(double *scal, int nc, int first, const int *
 /* SpMV fused with the FCG products (the arithmetic of k_spmv_dots): t = A z,
    per column the block partials of (r.z, t.z[, to.z]). K = 2: first step, t
    into pair 0; K = 3: t into pair par ^ 1, to from pair par. One row per
-   thread; columns in register chunks of CH. */
+   thread, CH columns per block row (chunk blockIdx.y). */
 template <int K>
 __global__ __launch_bounds__(BS) void k_spmv_dots_multi(int n, const int *__restrict__ rp,
                                                         const int *__restrict__ ci,
@@ -276,7 +290,17 @@ __global__ __launch_bounds__(BS) void k_spmv_dots_multi(int n, const int *__rest
                                                         int nc, const int *__restrict__ mask,
                                                         double *__restrict__ part)
 {
-    __shared__ double sh[32][K][BS / 64];
+    __shared__ double sh[CH][K][BS / 64];
+    const int j0 = blockIdx.y * CH;
+    bool on[CH];
+    bool any = false, all = true;
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+        on[u] = col_on(j0 + u, nc, mask);
+        any |= on[u];
+        all &= on[u];
+    }
+    if (!any) return;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int r = b * BS + threadIdx.x;
     const bool own = r < n;
@@ -287,19 +311,21 @@ __global__ __launch_bounds__(BS) void k_spmv_dots_multi(int n, const int *__rest
         ra = rp[r];
         re = rp[r + 1];
     }
-    for (int j0 = 0; j0 < nc; j0 += CH) {
-        bool on[CH];
-        bool any = false;
+    double t[CH];
 #pragma unroll
-        for (int u = 0; u < CH; u++) {
-            on[u] = col_on(j0 + u, nc, mask);
-            any |= on[u];
+    for (int u = 0; u < CH; u++) t[u] = 0.0;
+    const double *zj = F.Z + (size_t)j0 * ld;
+    if (all) {
+        for (int e = ra; e < re; e++) {
+            const int c = ci[e];
+            const double v = av[e];
+            double g[CH];
+#pragma unroll
+            for (int u = 0; u < CH; u++) g[u] = zj[(size_t)u * ld + c];
+#pragma unroll
+            for (int u = 0; u < CH; u++) t[u] = t[u] + v * g[u];
         }
-        if (!any) continue;
-        double t[CH];
-#pragma unroll
-        for (int u = 0; u < CH; u++) t[u] = 0.0;
-        const double *zj = F.Z + (size_t)j0 * ld;
+    } else {
         for (int e = ra; e < re; e++) {
             const int c = ci[e];
             const double v = av[e];
@@ -307,46 +333,46 @@ __global__ __launch_bounds__(BS) void k_spmv_dots_multi(int n, const int *__rest
             for (int u = 0; u < CH; u++)
                 if (on[u]) t[u] = t[u] + v * zj[(size_t)u * ld + c];
         }
+    }
 #pragma unroll
-        for (int u = 0; u < CH; u++) {
-            if (!on[u]) continue;
-            const int j = j0 + u;
-            double sd[K];
+    for (int u = 0; u < CH; u++) {
+        if (!on[u]) continue;
+        const int j = j0 + u;
+        double sd[K];
 #pragma unroll
-            for (int k = 0; k < K; k++) sd[k] = 0.0;
-            if (own) {
-                const size_t o = (size_t)j * ld + r;
-                const int p = F.par[j];
-                const double zr = F.Z[o];
-                const double rr = F.R[o];
-                double *tv = slot(F.W0, F.W1, K == 2 ? 0 : (p ^ 1), j, ld);
-                tv[r] = t[u];
-                sd[0] = rr * zr;
-                sd[1] = t[u] * zr;
-                if constexpr (K > 2) sd[K - 1] = slot(F.W0, F.W1, p, j, ld)[r] * zr;
-            }
-            /* the in-wave part of block_reduce_store<K> */
+        for (int k = 0; k < K; k++) sd[k] = 0.0;
+        if (own) {
+            const size_t o = (size_t)j * ld + r;
+            const int p = F.par[j];
+            const double zr = F.Z[o];
+            const double rr = F.R[o];
+            double *tv = slot(F.W0, F.W1, K == 2 ? 0 : (p ^ 1), j, ld);
+            tv[r] = t[u];
+            sd[0] = rr * zr;
+            sd[1] = t[u] * zr;
+            if constexpr (K > 2) sd[K - 1] = slot(F.W0, F.W1, p, j, ld)[r] * zr;
+        }
+        /* the in-wave part of block_reduce_store<K> */
 #pragma unroll
-            for (int k = 0; k < K; k++) {
+        for (int k = 0; k < K; k++) {
 #pragma unroll
-                for (int o = 32; o > 0; o >>= 1) sd[k] += __shfl_down(sd[k], o, 64);
-            }
-            if (lane == 0) {
+            for (int o = 32; o > 0; o >>= 1) sd[k] += __shfl_down(sd[k], o, 64);
+        }
+        if (lane == 0) {
 #pragma unroll
-                for (int k = 0; k < K; k++) sh[j][k][wid] = sd[k];
-            }
+            for (int k = 0; k < K; k++) sh[u][k][wid] = sd[k];
         }
     }
     __syncthreads();
-    /* the cross-wave part: thread j sums column j's wave results */
-    const int j = threadIdx.x;
-    if (j < nc && col_on(j, nc, mask)) {
-        double *out = part + ((size_t)j * gridDim.x + b) * K;
+    /* the cross-wave part: thread u sums column j0 + u's wave results */
+    const int u = threadIdx.x;
+    if (u < CH && col_on(j0 + u, nc, mask)) {
+        double *out = part + ((size_t)(j0 + u) * gridDim.x + b) * K;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            double t = 0.0;
-            for (int w = 0; w < BS / 64; w++) t += sh[j][k][w];
-            out[k] = t;
+            double tt = 0.0;
+            for (int w = 0; w < BS / 64; w++) tt += sh[u][k][w];
+            out[k] = tt;
         }
     }
 }
@@ -464,7 +490,8 @@ void rows_multi_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x,
     const int n = A.n;
     const int *rp = A.row.get(), *ci = A.col.get();
     if (stream_ok<VT>(A)) {
-        k_rows_multi<MT, VT, OP><<<blocks_for(n), BS, 0, s>>>(n, rp, ci, av, x, ldx, f, d, y,
+        const dim3 g((unsigned)blocks_for(n), (unsigned)((nc + CH - 1) / CH));
+        k_rows_multi<MT, VT, OP><<<g, BS, 0, s>>>(n, rp, ci, av, x, ldx, f, d, y,
                                                               ldy, nc, act);
     } else {
         const int G = A.group;
@@ -651,12 +678,13 @@ void launch_spmv_dots_multi(Ctx &c, const DevCsr &A, int k, MultiFcg &M, const i
         return;
     }
     const int nb = (n + BS - 1) / BS;
+    const int nch = (M.nc + CH - 1) / CH;
     if (k == 2) {
-        k_spmv_dots_multi<2><<<nb, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(),
+        k_spmv_dots_multi<2><<<dim3(nb, nch), BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(),
                                                        F, M.nc, mask, part);
         k_fcg_final_multi<2, false><<<M.nc, RED_FINAL_BS, 0, c.stream>>>(nb, part, F, mask);
     } else {
-        k_spmv_dots_multi<3><<<nb, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(),
+        k_spmv_dots_multi<3><<<dim3(nb, nch), BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(),
                                                        F, M.nc, mask, part);
         k_fcg_final_multi<3, false><<<M.nc, RED_FINAL_BS, 0, c.stream>>>(nb, part, F, mask);
     }

@@ -1,0 +1,269 @@
+/*
+ * server.hip - the serving loop: solves on cached composites with
+ * continuous batching (an extension: the reference solves one right-hand
+ * side per call, src/SOLVER/solver.cu:20-129).
+ *
+ * Every registered composite (operator) gets a lane when it has work: a
+ * stream of its own and a slot engine (host_multi.hip) of kmax column slots.
+ * A step of the loop
+ *   1. waits for the iteration each lane queued in the previous step,
+ *   2. collects the slots whose stopping test ended them and retires the
+ *      done ones (x copied to the request's output, the result recorded),
+ *   3. admits queued requests into free slots - requests that arrived while
+ *      others were on the device join those lanes at an iteration boundary,
+ *      and a lane is opened for an operator only while the device memory it
+ *      needs fits the budget (idle lanes are closed to make room),
+ *   4. queues one iteration (a captured graph) on every lane with active
+ *      slots; the lanes' streams run concurrently on the device.
+ * A slot computes exactly what a single solve of its right-hand side
+ * computes, so a result does not depend on what else is being served.
+ */
+#include "bcmg_internal.h"
+
+#include <chrono>
+#include <deque>
+#include <memory>
+
+namespace bcmg {
+
+namespace {
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    long long id = 0;
+    int op = 0;
+    const double *b = nullptr;
+    double *x = nullptr;
+    double t_submit = 0.0;
+};
+
+struct Lane {
+    Ctx c;
+    Boot *B = nullptr;
+    MultiWork *M = nullptr;
+    std::unique_ptr<SlotEngine> E;
+    std::vector<Request> slot_req;
+    size_t bytes = 0;
+    double last_used = 0.0;
+    ~Lane()
+    {
+        if (c.stream) (void)hipStreamSynchronize(c.stream);
+        E.reset();
+        destroy_multi(M);
+    }
+};
+
+} // namespace
+
+struct ServerResult {
+    bool done = false;
+    int iters = 0, status = BCMG_OK;
+    double resid = 0.0, latency_ms = 0.0;
+};
+
+struct Server {
+    Ctx &main;
+    size_t budget;
+    int max_slots;
+    std::vector<Boot *> ops;
+    std::vector<std::deque<Request>> queue;
+    std::vector<std::unique_ptr<Lane>> lanes;
+    std::unordered_map<long long, ServerResult> results;
+    size_t in_use = 0;
+    long long next_id = 1;
+    Server(Ctx &c, size_t b, int k) : main(c), budget(b), max_slots(k) {}
+
+    int add(Boot *B)
+    {
+        ops.push_back(B);
+        queue.emplace_back();
+        lanes.emplace_back();
+        return (int)ops.size() - 1;
+    }
+
+    long long submit(int op, const double *b, double *x)
+    {
+        if (op < 0 || op >= (int)ops.size()) throw Error(BCMG_EINVAL, "unknown operator id");
+        Request r;
+        r.id = next_id++;
+        r.op = op;
+        r.b = b;
+        r.x = x;
+        r.t_submit = now_ms();
+        queue[op].push_back(r);
+        results[r.id] = ServerResult();
+        return r.id;
+    }
+
+    bool lane_idle(const Lane &L) const
+    {
+        for (auto &s : L.E->slots)
+            if (s.state != SLOT_FREE) return false;
+        return true;
+    }
+
+    void close_lane(int op)
+    {
+        Lane &L = *lanes[op];
+        in_use -= L.bytes;
+        lanes[op].reset();
+    }
+
+    /* open a lane for op with as many slots as the budget allows (closing
+       idle lanes, least recently used first, when it does not fit) */
+    bool open_lane(int op)
+    {
+        Boot &B = *ops[op];
+        const int want = max_slots;
+        auto fits = [&](int k) { return in_use + multi_work_bytes(B, k) <= budget; };
+        while (!fits(1)) {
+            int victim = -1;
+            double oldest = 0.0;
+            for (size_t o = 0; o < lanes.size(); o++) {
+                if ((int)o == op || !lanes[o] || !lane_idle(*lanes[o])) continue;
+                if (victim < 0 || lanes[o]->last_used < oldest) {
+                    victim = (int)o;
+                    oldest = lanes[o]->last_used;
+                }
+            }
+            if (victim < 0) return false;
+            close_lane(victim);
+        }
+        int k = want;
+        while (k > 1 && !fits(k)) k--;
+        auto L = std::make_unique<Lane>();
+        L->c.device = main.device;
+        L->c.use_graphs = main.use_graphs;
+        BCMG_HIP(hipStreamCreateWithFlags(&L->c.stream, hipStreamNonBlocking));
+        L->B = &B;
+        multi_work(L->c, B, L->M, k);
+        L->E = std::make_unique<SlotEngine>(L->c, B, *L->M);
+        L->slot_req.assign(k, Request());
+        L->bytes = multi_work_bytes(B, k);
+        in_use += L->bytes;
+        lanes[op] = std::move(L);
+        return true;
+    }
+
+    int step()
+    {
+        int completed = 0;
+        /* 1. the iterations queued by the previous step */
+        for (auto &L : lanes)
+            if (L && L->E->launched) L->E->wait();
+        /* 2. stopped slots: done or restarted; retire the done ones */
+        for (auto &L : lanes) {
+            if (!L) continue;
+            SlotEngine &E = *L->E;
+            E.collect();
+            bool retired = false;
+            for (size_t j = 0; j < E.slots.size(); j++) {
+                if (E.slots[j].state != SLOT_DONE) continue;
+                const SlotInfo S = E.slots[j];
+                const Request &r = L->slot_req[j];
+                E.retire((int)j, r.x);
+                ServerResult &res = results[r.id];
+                res.iters = S.zero_rhs ? 0 : S.iters;
+                res.resid = S.zero_rhs ? 0.0 : S.l2;
+                res.status = S.status;
+                retired = true;
+            }
+            if (retired) {
+                BCMG_HIP(hipStreamSynchronize(L->c.stream));
+                const double t = now_ms();
+                for (size_t j = 0; j < E.slots.size(); j++) {
+                    Request &r = L->slot_req[j];
+                    if (r.id && E.slots[j].state == SLOT_FREE) {
+                        ServerResult &res = results[r.id];
+                        if (!res.done) {
+                            res.done = true;
+                            res.latency_ms = t - r.t_submit;
+                            completed++;
+                        }
+                        r = Request();
+                    }
+                }
+                L->last_used = t;
+            }
+        }
+        /* 3. admission into free slots */
+        for (size_t op = 0; op < ops.size(); op++) {
+            auto &q = queue[op];
+            if (q.empty()) continue;
+            if (!lanes[op] && !open_lane((int)op)) continue;
+            Lane &L = *lanes[op];
+            ensure_multi_into(L.c, *L.B, *L.M, L.M->kmax);
+            std::vector<int> js;
+            std::vector<const double *> bs;
+            for (size_t j = 0; j < L.E->slots.size() && !q.empty(); j++) {
+                if (L.E->slots[j].state != SLOT_FREE) continue;
+                L.slot_req[j] = q.front();
+                q.pop_front();
+                js.push_back((int)j);
+                bs.push_back(L.slot_req[j].b);
+            }
+            if (js.empty()) continue;
+            L.E->admit(js, bs);
+            L.last_used = now_ms();
+        }
+        /* zero right-hand sides are done at admission: retired next step */
+        /* 4. one iteration of every lane with active slots */
+        for (auto &L : lanes)
+            if (L) L->E->launch();
+        return completed;
+    }
+
+    void info(int *queued, int *running, size_t *bytes, int *nlanes) const
+    {
+        int qn = 0, rn = 0, ln = 0;
+        for (auto &q : queue) qn += (int)q.size();
+        for (auto &L : lanes) {
+            if (!L) continue;
+            ln++;
+            for (auto &s : L->E->slots) rn += s.state != SLOT_FREE;
+        }
+        if (queued) *queued = qn;
+        if (running) *running = rn;
+        if (bytes) *bytes = in_use;
+        if (nlanes) *nlanes = ln;
+    }
+};
+
+Server *server_create(Ctx &c, size_t budget, int slots)
+{
+    if (slots < 1 || slots > MULTI_MAX_COLS)
+        throw Error(BCMG_EINVAL, "slots per operator must be 1..32");
+    return new Server(c, budget, slots);
+}
+void server_destroy(Server *S) { delete S; }
+int server_add(Server &S, Boot &B) { return S.add(&B); }
+long long server_submit(Server &S, int op, const double *b, double *x)
+{
+    return S.submit(op, b, x);
+}
+int server_step(Server &S) { return S.step(); }
+bool server_result(Server &S, long long id, int *done, int *iters, double *resid, int *status,
+                   double *latency_ms)
+{
+    auto it = S.results.find(id);
+    if (it == S.results.end()) return false;
+    const ServerResult &r = it->second;
+    if (done) *done = r.done ? 1 : 0;
+    if (iters) *iters = r.iters;
+    if (resid) *resid = r.resid;
+    if (status) *status = r.status;
+    if (latency_ms) *latency_ms = r.latency_ms;
+    if (r.done) S.results.erase(it);
+    return true;
+}
+void server_info(const Server &S, int *queued, int *running, size_t *bytes, int *lanes)
+{
+    S.info(queued, running, bytes, lanes);
+}
+
+} // namespace bcmg

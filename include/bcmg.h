@@ -132,6 +132,36 @@ int bcmg_fcg(bcmg_ctx ctx, bcmg_boot B, const double *b_dev, double *x_dev, int 
 int bcmg_fcg_multi(bcmg_ctx ctx, bcmg_boot B, int k, const double *const *b_dev,
                    double *const *x_dev, int *iters, double *resid, int *status);
 
+/* ---- serving loop (an extension: continuous batching of solves) ---------
+   A server serves solves on composites built beforehand (cached on the
+   device). Requests name an operator and give device pointers b (read when
+   the request is admitted) and x (written when it completes). Each call of
+   bcmg_server_step waits for the iteration the previous step queued,
+   retires finished requests, admits queued requests into free column slots
+   (requests arriving while others run join at the next iteration), and
+   queues one iteration on every operator with active slots, each operator
+   on a stream of its own. An operator's slots (up to slots_per_operator,
+   at most 32) take device memory only while it has work, within
+   hbm_budget_bytes; idle operators give theirs back when another needs it.
+   A request's result equals bcmg_fcg on its b alone, bit for bit. The
+   composites must stay alive and keep their precision while served; destroy
+   the server before them and before its context. */
+typedef struct bcmg_server_s *bcmg_server;
+int bcmg_server_create(bcmg_ctx ctx, size_t hbm_budget_bytes, int slots_per_operator,
+                       bcmg_server *out);
+int bcmg_server_destroy(bcmg_server S);
+int bcmg_server_add_operator(bcmg_server S, bcmg_boot B, int *op_id);
+int bcmg_server_submit(bcmg_server S, int op_id, const double *b_dev, double *x_dev,
+                       long long *req_id);
+/* one round of the loop; *completed = requests finished in it */
+int bcmg_server_step(bcmg_server S, int *completed);
+/* done = 0 while pending; a finished result is returned once (then
+   forgotten); latency_ms runs from submit to completion */
+int bcmg_server_result(bcmg_server S, long long req_id, int *done, int *iters, double *resid,
+                       int *status, double *latency_ms);
+int bcmg_server_info(bcmg_server S, int *queued, int *running, size_t *bytes_in_use,
+                     int *lanes);
+
 /* setup + FCG + free on host arrays: the whole of solve()
    (src/SOLVER/solver.cu:20-129), with x returned in host memory.
    setup_ms / solve_ms (may be NULL) are the two phases' wall times. */
