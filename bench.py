@@ -14,7 +14,10 @@ timed region), the serial CPU oracle on the same workload, and the precision
 form: the same step with the hierarchy converted to FP32 (or bfloat16 values
 on levels >= 1) inside FP64 FCG with a true-residual check, on this workload
 and on the bootstrap composite workload (cell-coefficient diffusion on 128^3,
-k = exp(g), g ~ N(0, 4), max_hrc 3, rho 0.5).
+k = exp(g), g ~ N(0, 4), max_hrc 3, rho 0.5); the multi-right-hand-side
+form (32 columns through every launch, anisotropic 2048^2, W-cycle); and the
+serving loop (8 cached operators, Poisson arrivals, continuous batching)
+against one FP64 solve at a time on the same arrival trace.
 """
 import argparse
 import json
@@ -118,6 +121,99 @@ def precision_forms(bc, ctx, A, b, settings, steps, warmup, x_ref=None, ref_name
     return out
 
 
+C3_GRID = 2048
+C3_RHS = 32
+C3_SETTINGS = dict(SETTINGS, aggrsweeps=3, cycle_type=2)
+
+
+def multi_rhs(bc, ctx, steps):
+    """32 right-hand sides on one hierarchy (anisotropic 2D, W-cycle): FP32
+    hierarchy inside FP64 FCG, all columns in every launch, against the
+    same right-hand sides solved one at a time in FP64 and in FP32."""
+    from bootcmatchg_amd import operators as ops
+    A = ops.aniso2d(C3_GRID, 0.01)
+    n = len(A[0]) - 1
+    S = ops.to_scipy(A)
+    dA = bc.Csr(ctx, A)
+    ctx.sync()
+    t0 = time.perf_counter()
+    B = bc.Boot(ctx, dA, bc.make_params(**C3_SETTINGS))
+    ctx.sync()
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    bh = [ops.uniform_pm1(j, n) for j in range(C3_RHS)]
+    bs = [bc.DeviceArray.from_numpy(ctx, b) for b in bh]
+    xs = [bc.DeviceArray(ctx, n) for _ in range(C3_RHS)]
+    out = {"workload": "aniso2d 5-pt %d^2 eps 0.01, %d rhs uniform[-1,1] seeds 0..%d, "
+                       "settings 0,1,0.8,3,0,39,2,4,4,16,4,4,2000,1e-8"
+                       % (C3_GRID, C3_RHS, C3_RHS - 1),
+           "rows": n, "nnz": len(A[1]), "setup_ms": setup_ms}
+    single = {}
+    for prec in (0, 1):
+        B.set_precision(prec)
+        B.fcg(bs[0], xs[0])
+        ctx.sync()
+        t0 = time.perf_counter()
+        its = []
+        for j in range(4):
+            its.append(B.fcg(bs[j], xs[j])[0])
+        ctx.sync()
+        single[prec] = (time.perf_counter() - t0) / 4
+        out["single_%s_ms_per_rhs" % PREC_NAMES[prec]] = single[prec] * 1e3
+    B.set_precision(1)
+    B.fcg_multi(bs, xs)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        it, res, st = B.fcg_multi(bs, xs)
+    ctx.sync()
+    per = (time.perf_counter() - t0) / steps
+    rel = [float(np.linalg.norm(bh[j] - S @ xs[j].numpy()) / np.linalg.norm(bh[j]))
+           for j in range(C3_RHS)]
+    out.update({"multi_fp32_ms_per_batch": per * 1e3,
+                "multi_fp32_ms_per_rhs": per * 1e3 / C3_RHS,
+                "rhs_solves_per_s": C3_RHS / per,
+                "speedup_vs_single_fp64": single[0] * C3_RHS / per,
+                "speedup_vs_single_fp32": single[1] * C3_RHS / per,
+                "iterations_min_max": [int(it.min()), int(it.max())],
+                "max_true_rel_residual": max(rel)})
+    B.free()
+    return out
+
+
+def serving(bc, ctx):
+    """The serving workload: 8 cached operators, Poisson arrivals, uniform
+    operator choice; the served loop (continuous batching) against one solve
+    at a time in FP64 on the same trace."""
+    from bootcmatchg_amd import serving as sv
+    pool = sv.Pool(bc, ctx, SETTINGS, 0)
+    out = {"operators": [{"name": o["name"], "rows": o["n"], "nnz": o["nnz"]} for o in pool.ops],
+           "settings": "0,1,0.8,2,0,39,0,4,4,16,4,4,2000,1e-8",
+           "slots_per_operator": 16}
+    arr, opsel, rhsel = sv.trace(4000, len(pool.ops), 400, 3.0, seed=1)
+    out["saturation_trace"] = "%d requests offered at 4000/s" % len(arr)
+    out["plain_fp64_one_at_a_time"] = sv.run_plain(pool, arr, opsel, rhsel)
+    for prec in (0, 1):
+        pool.set_precision(prec)
+        S = bc.Server(ctx, 96 << 30, 16)
+        for o in pool.ops:
+            S.add(o["B"])
+        a2, o2, r2 = sv.trace(4000, len(pool.ops), 64, 1.0, seed=9)
+        sv.run_served(pool, S, a2, o2, r2)          # lanes opened, graphs captured
+        out["served_%s" % PREC_NAMES[prec]] = sv.run_served(pool, S, arr, opsel, rhsel)
+        if prec == 1:
+            for rate in (50, 200):
+                a, o_, r_ = sv.trace(rate, len(pool.ops), 300, 3.0, seed=2)
+                out["served_fp32_at_%d_per_s" % rate] = sv.run_served(pool, S, a, o_, r_)
+        out["device_bytes_%s" % PREC_NAMES[prec]] = S.info()["bytes"]
+        S.close()
+    plain = out["plain_fp64_one_at_a_time"]["solves_per_s"]
+    out["served_fp32_vs_plain_fp64"] = out["served_fp32"]["solves_per_s"] / plain
+    out["served_fp64_vs_plain_fp64"] = out["served_fp64"]["solves_per_s"] / plain
+    for o in pool.ops:
+        o["B"].free()
+    return out
+
+
 def sweep_bytes(n, nnz):
     """Algorithmic HBM bytes of one Jacobi/L1 sweep u' = u + (f - A u) / d:
     val + col (12 B/nnz), row pointer, u (gathered, counted once), f, d, u'."""
@@ -138,6 +234,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP32/BF16 precision-form measurements")
+    ap.add_argument("--no-multi", action="store_true",
+                    help="skip the 32-right-hand-side measurement")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving-loop measurement")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every FCG kernel from the host instead of replaying "
                          "captured HIP graphs (used for kernel-trace profiling)")
@@ -246,6 +346,9 @@ def main():
             "cell-coefficient diffusion 128^3, k=exp(g), g~N(0,4) seed 0, harmonic faces; "
             "b uniform[-1,1] seed 0; settings 0,3,0.5,2,0,39,0,4,4,16,4,4,2000,1e-8")
 
+    multi = None if args.no_multi else multi_rhs(bc, ctx, max(1, args.steps // 2))
+    served = None if args.no_serving else serving(bc, ctx)
+
     out = {
         "metric": METRIC,
         "value": args.steps / elapsed,
@@ -280,6 +383,8 @@ def main():
                      "launches": sm_launches},
         "cpu_baseline": cpu,
         "precision_form": precision,
+        "multi_rhs": multi,
+        "serving": served,
     }
     print(json.dumps(out))
 

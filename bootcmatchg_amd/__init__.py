@@ -11,6 +11,7 @@ There is no CPU fallback: if libbcmg.so is missing or no gfx950 device is
 present, the calls that need the GPU raise.
 """
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -123,10 +124,24 @@ class Context:
 
     def __init__(self, device=0):
         self.h = _vp()
+        self._kids = []
         _check(lib().bcmg_ctx_create(device, ctypes.byref(self.h)))
+
+    def _adopt(self, obj):
+        """Track an object that lives in this context: close() frees the
+        live ones first, newest first (a server before its composites, a
+        composite before its matrix), so nothing outlives its context."""
+        self._kids.append(weakref.ref(obj))
+        if len(self._kids) > 4096:
+            self._kids = [k for k in self._kids if k() is not None]
 
     def close(self):
         if getattr(self, "h", None):
+            for ref in reversed(self._kids):
+                obj = ref()
+                if obj is not None:
+                    obj._release()
+            self._kids = []
             lib().bcmg_ctx_destroy(self.h)
             self.h = None
 
@@ -147,6 +162,7 @@ class DeviceArray:
         self.ptr = _vp()
         _check(lib().bcmg_dmalloc(ctx.h, max(self.n, 1) * self.dtype.itemsize,
                                   ctypes.byref(self.ptr)))
+        ctx._adopt(self)
 
     @classmethod
     def from_numpy(cls, ctx, a):
@@ -166,9 +182,11 @@ class DeviceArray:
         return out
 
     def free(self):
-        if getattr(self, "ptr", None) and self.ptr.value:
+        if getattr(self, "ptr", None) and self.ptr.value and self.ctx.h:
             lib().bcmg_dfree(self.ctx.h, self.ptr)
-            self.ptr = _vp()
+        self.ptr = _vp()
+
+    _release = free
 
     def __del__(self):
         try:
@@ -192,6 +210,7 @@ class Csr:
             _check(lib().bcmg_csr_upload(ctx.h, n, len(col), row.ctypes.data_as(_i32p),
                                          col.ctypes.data_as(_i32p), val.ctypes.data_as(_f64p),
                                          ctypes.byref(self.h)))
+        ctx._adopt(self)
         n = ctypes.c_int()
         nnz = ctypes.c_int()
         _check(lib().bcmg_csr_info(self.h, ctypes.byref(n), ctypes.byref(nnz)))
@@ -206,9 +225,11 @@ class Csr:
         return row, col, val
 
     def free(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and self.ctx.h:
             lib().bcmg_csr_free(self.h)
-            self.h = None
+        self.h = None
+
+    _release = free
 
     def __del__(self):
         self.free()
@@ -223,11 +244,14 @@ class Boot:
         self.A = A
         self.h = _vp()
         _check(lib().bcmg_setup(ctx.h, A.h, ctypes.byref(prm), ctypes.byref(self.h)))
+        ctx._adopt(self)
 
     def free(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and self.ctx.h:
             lib().bcmg_boot_free(self.h)
-            self.h = None
+        self.h = None
+
+    _release = free
 
     def __del__(self):
         self.free()
@@ -328,11 +352,14 @@ class Server:
         self._boots = []
         _check(lib().bcmg_server_create(ctx.h, ctypes.c_size_t(int(hbm_budget_bytes)),
                                         int(slots_per_operator), ctypes.byref(self.h)))
+        ctx._adopt(self)
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and self.ctx.h:
             lib().bcmg_server_destroy(self.h)
-            self.h = None
+        self.h = None
+
+    _release = close
 
     def __del__(self):
         self.close()
@@ -367,6 +394,14 @@ class Server:
         self._io.pop(rid, None)
         return dict(iterations=it.value, residual=res.value, status=st.value,
                     latency_ms=lat.value)
+
+    def completed(self, cap=4096):
+        """ids of requests finished since the last call"""
+        ids = np.zeros(cap, np.int64)
+        cnt = ctypes.c_int()
+        _check(lib().bcmg_server_completed(self.h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                                           cap, ctypes.byref(cnt)))
+        return ids[:cnt.value].tolist()
 
     def info(self):
         q, r, nl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

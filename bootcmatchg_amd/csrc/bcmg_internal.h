@@ -34,10 +34,12 @@ void set_last_error(const std::string &msg);
 #define BCMG_HIP(call)                                                              \
     do {                                                                            \
         hipError_t e_ = (call);                                                     \
-        if (e_ != hipSuccess)                                                       \
+        if (e_ != hipSuccess) {                                                     \
+            (void)hipGetLastError(); /* do not leave it for the next check */       \
             throw ::bcmg::Error(BCMG_EHIP, std::string(#call) + ": " +              \
                                                hipGetErrorString(e_) + " @" +       \
                                                __FILE__ + ":" + std::to_string(__LINE__)); \
+        }                                                                           \
     } while (0)
 
 #define BCMG_CHECK_LAUNCH() BCMG_HIP(hipGetLastError())
@@ -429,6 +431,7 @@ int server_step(Server &S);
 bool server_result(Server &S, long long id, int *done, int *iters, double *resid, int *status,
                    double *latency_ms);
 void server_info(const Server &S, int *queued, int *running, size_t *bytes, int *lanes);
+int server_completed(Server &S, long long *ids, int cap);
 
 /* doubles of partials scratch the multi-column FCG kernels need */
 size_t multi_part_size(int n, int nc);

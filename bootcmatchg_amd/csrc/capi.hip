@@ -500,6 +500,13 @@ int bcmg_server_result(bcmg_server S, long long req_id, int *done, int *iters, d
     return BCMG_OK;
 }
 
+int bcmg_server_completed(bcmg_server S, long long *ids, int cap, int *count)
+{
+    if (!S || !count || cap < 0 || (cap > 0 && !ids)) return BCMG_EINVAL;
+    *count = server_completed(*S->S, ids, cap);
+    return BCMG_OK;
+}
+
 int bcmg_server_info(bcmg_server S, int *queued, int *running, size_t *bytes_in_use, int *lanes)
 {
     if (!S) return BCMG_EINVAL;

@@ -74,6 +74,7 @@ struct Server {
     std::vector<std::deque<Request>> queue;
     std::vector<std::unique_ptr<Lane>> lanes;
     std::unordered_map<long long, ServerResult> results;
+    std::vector<long long> completed_ids; /* finished since the last drain */
     size_t in_use = 0;
     long long next_id = 1;
     Server(Ctx &c, size_t b, int k) : main(c), budget(b), max_slots(k) {}
@@ -136,6 +137,9 @@ struct Server {
         }
         int k = want;
         while (k > 1 && !fits(k)) k--;
+        /* the pool hands blocks out in the order of one stream: the lane's
+           stream may only take blocks the main stream is done with */
+        BCMG_HIP(hipStreamSynchronize(main.stream));
         auto L = std::make_unique<Lane>();
         L->c.device = main.device;
         L->c.use_graphs = main.use_graphs;
@@ -183,6 +187,7 @@ struct Server {
                         if (!res.done) {
                             res.done = true;
                             res.latency_ms = t - r.t_submit;
+                            completed_ids.push_back(r.id);
                             completed++;
                         }
                         r = Request();
@@ -197,7 +202,11 @@ struct Server {
             if (q.empty()) continue;
             if (!lanes[op] && !open_lane((int)op)) continue;
             Lane &L = *lanes[op];
-            ensure_multi_into(L.c, *L.B, *L.M, L.M->kmax);
+            if (L.B->prec != BCMG_PREC_FP64 && L.M->prec != L.B->prec) {
+                /* reduced-precision buffers are allocated now (see open_lane) */
+                BCMG_HIP(hipStreamSynchronize(main.stream));
+                ensure_multi_into(L.c, *L.B, *L.M, L.M->kmax);
+            }
             std::vector<int> js;
             std::vector<const double *> bs;
             for (size_t j = 0; j < L.E->slots.size() && !q.empty(); j++) {
@@ -260,6 +269,13 @@ bool server_result(Server &S, long long id, int *done, int *iters, double *resid
     if (latency_ms) *latency_ms = r.latency_ms;
     if (r.done) S.results.erase(it);
     return true;
+}
+int server_completed(Server &S, long long *ids, int cap)
+{
+    const int k = std::min<int>(cap, (int)S.completed_ids.size());
+    for (int i = 0; i < k; i++) ids[i] = S.completed_ids[i];
+    S.completed_ids.erase(S.completed_ids.begin(), S.completed_ids.begin() + k);
+    return k;
 }
 void server_info(const Server &S, int *queued, int *running, size_t *bytes, int *lanes)
 {

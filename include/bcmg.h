@@ -161,6 +161,8 @@ int bcmg_server_result(bcmg_server S, long long req_id, int *done, int *iters, d
                        int *status, double *latency_ms);
 int bcmg_server_info(bcmg_server S, int *queued, int *running, size_t *bytes_in_use,
                      int *lanes);
+/* ids of requests finished since the last call (up to cap, oldest first) */
+int bcmg_server_completed(bcmg_server S, long long *ids, int cap, int *count);
 
 /* setup + FCG + free on host arrays: the whole of solve()
    (src/SOLVER/solver.cu:20-129), with x returned in host memory.
