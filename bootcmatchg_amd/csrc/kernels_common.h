@@ -39,6 +39,37 @@ __device__ __forceinline__ T ldm(const T *p)
     return *p;
 }
 
+/* Phase 1 of the row-block kernels: the block's entries [e0, e0 + cnt) of
+   col/val are streamed with consecutive lanes on consecutive entries (U
+   independent loads in flight per lane), x is gathered and the products
+   a_ij x_j are stored in LDS (prod[e - e0]). (Paired 2-element loads were
+   measured slower on gfx950 for both FP64 and FP32.) */
+template <typename MT, typename VT, bool NT>
+__device__ __forceinline__ void stage_products(const int *__restrict__ ci,
+                                               const MT *__restrict__ av,
+                                               const VT *__restrict__ x, int e0, int cnt,
+                                               VT *__restrict__ prod)
+{
+    constexpr int U = 4;
+    for (int k0 = threadIdx.x; k0 < cnt; k0 += BS * U) {
+        int c[U];
+        MT v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k0 + u * BS;
+            if (k < cnt) {
+                c[u] = ldm<NT>(ci + e0 + k);
+                v[u] = ldm<NT>(av + e0 + k);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k0 + u * BS;
+            if (k < cnt) prod[k] = mval<VT>(v[u]) * x[c[u]];
+        }
+    }
+}
+
 constexpr int RED_FINAL_BS = 1024; /* threads of the one-block final reductions */
 
 /* fixed-order block reduction of K partial sums; thread 0 writes out[0..K) */

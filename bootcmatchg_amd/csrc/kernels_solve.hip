@@ -61,24 +61,7 @@ __global__ __launch_bounds__(BS) void k_rows_stream(int n, const int *__restrict
     }
     const int e0 = rp[r0];
     const int cnt = rp[r1] - e0;
-    constexpr int U = 4;
-    for (int k0 = threadIdx.x; k0 < cnt; k0 += BS * U) {
-        int c[U];
-        MT v[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k = k0 + u * BS;
-            if (k < cnt) {
-                c[u] = ldm<NT>(ci + e0 + k);
-                v[u] = ldm<NT>(av + e0 + k);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k = k0 + u * BS;
-            if (k < cnt) prod[k] = mval<VT>(v[u]) * x[c[u]];
-        }
-    }
+    stage_products<MT, VT, NT>(ci, av, x, e0, cnt, prod);
     __syncthreads();
     if (!own) return;
     VT t = 0;
@@ -289,24 +272,7 @@ __global__ __launch_bounds__(BS) void k_spmv_dots(int n, const int *__restrict__
     }
     const int e0 = rp[r0];
     const int cnt = rp[r1] - e0;
-    constexpr int U = 4;
-    for (int k0 = threadIdx.x; k0 < cnt; k0 += BS * U) {
-        int c[U];
-        double v[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k = k0 + u * BS;
-            if (k < cnt) {
-                c[u] = ldm<NT>(ci + e0 + k);
-                v[u] = ldm<NT>(av + e0 + k);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k = k0 + u * BS;
-            if (k < cnt) prod[k] = v[u] * z[c[u]];
-        }
-    }
+    stage_products<double, double, NT>(ci, av, z, e0, cnt, prod);
     __syncthreads();
     double sdot[K];
 #pragma unroll
