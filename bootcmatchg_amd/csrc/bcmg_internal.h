@@ -303,14 +303,14 @@ void launch_spmv_dots(Ctx &c, const DevCsr &A, int k, const double *z, double *t
 /* doubles of reduction scratch needed for an operator of n rows */
 size_t red_size_for(int n);
 /* all sweeps of a small level in one workgroup (when relax_block_ok(A));
-   with ncols > 1, workgroup j does column j of f/xin/xout (columns ld
-   elements apart), skipped when act[j] == 0 */
+   with ncols > 1, workgroup j does column j of f/xin/xout (element i of
+   column j at j * cs + i * rs), skipped when act[j] == 0 */
 template <typename VT>
 bool relax_block_ok(const DevCsr &A);
 template <typename VT>
 void launch_relax_block(hipStream_t s, const DevCsr &A, MatType mt, const VT *d, const VT *f,
                         const VT *xin, VT *xout, int sweeps, bool zero, int ncols = 1,
-                        long ld = 0, const int *act = nullptr);
+                        long cs = 0, int rs = 1, const int *act = nullptr);
 
 /* ---------------------------------------- multi-column kernels (kernels_multi.hip)
    Column blocks: column j of a level's block at base + j * ld. Each kernel
@@ -339,6 +339,33 @@ void launch_axpy_multi(hipStream_t s, int n, VT alpha, const VT *x, VT *y, long 
 template <typename VT>
 void launch_scale_multi(hipStream_t s, int n, VT alpha, VT *x, long ld, int nc, const int *act);
 
+/* interleaved blocks (element (i, j) at i * 2^lkp + j): the cycle's layout */
+template <typename VT>
+void launch_rows_il(hipStream_t s, const DevCsr &A, MatType mt, RowOp op, const VT *x,
+                    const VT *f, const VT *d, VT *y, int lkp, int nc, const int *act);
+template <typename VT>
+void launch_jacobi_zero_il(hipStream_t s, int n, const VT *f, const VT *d, VT *y, int lkp,
+                           int nc, const int *act);
+template <typename VT>
+void launch_prolong_il(hipStream_t s, int n, const int *agg, const VT *p, const VT *xc, VT *x,
+                       int lkp, int nc, const int *act);
+template <typename VT>
+void launch_fill_il(hipStream_t s, int n, VT v, VT *x, int lkp, int nc, const int *act);
+template <typename VT>
+void launch_copy_il(hipStream_t s, int n, const VT *a, VT *b, int lkp, int nc, const int *act);
+template <typename VT>
+void launch_axpy_il(hipStream_t s, int n, VT alpha, const VT *x, VT *y, int lkp, int nc,
+                    const int *act);
+template <typename VT>
+void launch_scale_il(hipStream_t s, int n, VT alpha, VT *x, int lkp, int nc, const int *act);
+/* column-major (leading dimension ld) <-> interleaved, converting S -> D */
+template <typename S, typename D>
+void launch_to_il(hipStream_t s, int n, const S *a, long lda, D *b, int lkp, int nc,
+                  const int *act);
+template <typename S, typename D>
+void launch_from_il(hipStream_t s, int n, const S *a, D *b, long ldb, int lkp, int nc,
+                    const int *act);
+
 /* per-column FCG state on the device (FP64 blocks of leading dimension ld):
    direction pairs (V[s], W[s]), the newest in pair par[j]; Z the
    preconditioner output; R residual, X iterate, B right-hand side, Y
@@ -363,6 +390,10 @@ struct MultiWork {
     CycleBufs<double> b64;
     CycleBufs<float> b32;
     DBuf<float> r32, z32;      /* level-0 input/output of an FP32 cycle */
+    DBuf<double> r64, z64;     /* the same for an FP64 cycle */
+    std::vector<long> nl;      /* rows per level (largest hierarchy) */
+    int lkp = 0;               /* cycle blocks interleave 2^lkp columns */
+    bool prec64 = false;       /* FP64 cycle buffers allocated */
     DBuf<double> vec;          /* V0 V1 W0 W1 Z R X B Y */
     DBuf<double> scal, tol, part;
     DBuf<int> flags;           /* act, par, it, mask: kmax each */

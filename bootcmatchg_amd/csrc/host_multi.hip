@@ -32,27 +32,35 @@ namespace {
 
 inline long round_ld(size_t n) { return (long)((n + 63) / 64 * 64); }
 
+/* cycle blocks: level l holds n_l rows of 2^lkp interleaved columns */
 template <typename VT>
 void ensure_mbufs(const Boot &B, const MultiWork &M, CycleBufs<VT> &W)
 {
-    const size_t L = M.ld.size();
+    const size_t L = M.nl.size();
     if (W.xa.size() < L) {
         W.xa.resize(L);
         W.xb.resize(L);
         W.f.resize(L);
     }
     for (size_t l = 0; l < L; l++) {
-        const size_t sz = (size_t)M.kmax * M.ld[l];
+        const size_t sz = (size_t)M.nl[l] << M.lkp;
         W.xa[l].ensure(sz);
         W.xb[l].ensure(sz);
         W.f[l].ensure(sz);
     }
-    const size_t s0 = (size_t)M.kmax * M.ld[0];
+    const size_t s0 = (size_t)M.nl[0] << M.lkp;
     W.res.ensure(s0);
     if (B.prm.solver_type == 2) {
         W.xadd.ensure(s0);
         W.xsave.ensure(s0);
     }
+}
+
+inline int log2_cols(int k)
+{
+    int l = 0;
+    while ((1 << l) < k) l++;
+    return l;
 }
 
 } // namespace
@@ -70,12 +78,15 @@ MultiWork &ensure_multi_into(Ctx &c, Boot &B, MultiWork &M, int kmax)
         size_t maxL = 0;
         for (auto &h : B.H) maxL = std::max(maxL, h.L.size());
         M.ld.assign(maxL, 0);
+        M.nl.assign(maxL, 0);
         for (size_t l = 0; l < maxL; l++) {
             size_t nl = 0;
             for (auto &h : B.H)
                 if (l < h.L.size()) nl = std::max(nl, (size_t)h.L[l].mat().n);
             M.ld[l] = round_ld(nl);
+            M.nl[l] = (long)nl;
         }
+        M.lkp = log2_cols(kmax);
         const size_t blk = (size_t)kmax * M.ld[0];
         M.vec.alloc(9 * blk);
         M.scal.alloc((size_t)kmax * 8);
@@ -111,15 +122,21 @@ MultiWork &ensure_multi_into(Ctx &c, Boot &B, MultiWork &M, int kmax)
         /* no slot active, no stale flags */
         BCMG_HIP(hipMemsetAsync(M.flags.get(), 0, 4 * (size_t)kmax * sizeof(int), c.stream));
         std::memset(M.h_flags, 0, 4 * (size_t)kmax * sizeof(int));
-        ensure_mbufs<double>(B, M, M.b64);
         M.prec = -1;
+        M.prec64 = false;
+    }
+    if (B.prec == BCMG_PREC_FP64 && !M.prec64) {
+        ensure_mbufs<double>(B, M, M.b64);
+        M.r64.ensure((size_t)M.nl[0] << M.lkp);
+        M.z64.ensure((size_t)M.nl[0] << M.lkp);
+        M.prec64 = true;
     }
     M.F.nc = M.kmax;
     M.F.maxit = B.prm.itnlim;
     if (B.prec != BCMG_PREC_FP64 && M.prec != B.prec) {
         ensure_mbufs<float>(B, M, M.b32);
-        M.r32.ensure((size_t)M.kmax * M.ld[0]);
-        M.z32.ensure((size_t)M.kmax * M.ld[0]);
+        M.r32.ensure((size_t)M.nl[0] << M.lkp);
+        M.z32.ensure((size_t)M.nl[0] << M.lkp);
         M.prec = B.prec;
     }
     return M;
@@ -153,18 +170,20 @@ struct MRun {
     const int *act;
 };
 
-/* relax_level of host_amg.hip on a column block */
+/* relax_level of host_amg.hip on an interleaved block */
 template <typename VT>
 void relax_level_m(MRun<VT> &R, const Level &lv, MatType mt, int type, int sweeps, const VT *f,
-                   MState<VT> &s, VT *final_out, long ld)
+                   MState<VT> &s, VT *final_out)
 {
     Ctx &c = R.c;
+    const int lkp = R.M.lkp;
     const DevCsr &A = lv.mat();
     const VT *d = divisor<VT>(lv, type);
     if (sweeps == 0) return;
     if (relax_block_ok<VT>(A)) {
         VT *out = (final_out && s.x != final_out) ? final_out : s.x;
-        launch_relax_block<VT>(c.stream, A, mt, d, f, s.x, out, sweeps, s.zero, R.nc, ld, R.act);
+        launch_relax_block<VT>(c.stream, A, mt, d, f, s.x, out, sweeps, s.zero, R.nc, 1,
+                               1 << lkp, R.act);
         if (out != s.x) {
             s.x2 = s.x;
             s.x = out;
@@ -176,47 +195,45 @@ void relax_level_m(MRun<VT> &R, const Level &lv, MatType mt, int type, int sweep
         VT *out = s.x2;
         if (final_out && k == sweeps - 1 && s.x != final_out) out = final_out;
         if (s.zero)
-            launch_jacobi_zero_multi<VT>(c.stream, A.n, f, d, out, ld, R.nc, R.act);
+            launch_jacobi_zero_il<VT>(c.stream, A.n, f, d, out, lkp, R.nc, R.act);
         else
-            launch_rows_multi<VT>(c.stream, A, mt, OP_JACOBI, s.x, ld, f, d, out, ld, R.nc,
-                                  R.act);
+            launch_rows_il<VT>(c.stream, A, mt, OP_JACOBI, s.x, f, d, out, lkp, R.nc, R.act);
         s.x2 = s.x;
         s.x = out;
         s.zero = false;
     }
 }
 
-/* cycle of host_amg.hip on a column block (src/SOLVER/GAMG_cycle.cu:25-140) */
+/* cycle of host_amg.hip on interleaved blocks (src/SOLVER/GAMG_cycle.cu:25-140) */
 template <typename VT>
 void cycle_m(MRun<VT> &R, int l)
 {
     Ctx &c = R.c;
     Boot &B = R.B;
     CycleBufs<VT> &W = mbufs<VT>(R.M);
+    const int lkp = R.M.lkp;
     const Hierarchy &H = R.H;
     const int L = H.num_levels();
     const Level &lv = H.L[l - 1];
     const MatType mt = level_mt(B, l - 1);
     MState<VT> &s = R.st[l - 1];
-    const long ld = R.M.ld[l - 1];
     const VT *f = (l == 1) ? R.f0 : W.f[l - 1].get();
     const bool top = (l == 1);
     if (l == L) {
         relax_level_m<VT>(R, lv, mt, B.coarse_solver, B.prm.relaxnumber_coarse, f, s,
-                          top ? R.final_out : nullptr, ld);
+                          top ? R.final_out : nullptr);
         return;
     }
-    relax_level_m<VT>(R, lv, mt, B.prm.relax_type, B.prm.prerelax_sweeps, f, s, nullptr, ld);
+    relax_level_m<VT>(R, lv, mt, B.prm.relax_type, B.prm.prerelax_sweeps, f, s, nullptr);
     const DevCsr &A = lv.mat();
-    const long ldc = R.M.ld[l];
     const VT *rsrc = f;
     if (!s.zero) {
-        launch_rows_multi<VT>(c.stream, A, mt, OP_RESID, s.x, ld, f, nullptr, W.res.get(), ld,
-                              R.nc, R.act);
+        launch_rows_il<VT>(c.stream, A, mt, OP_RESID, s.x, f, nullptr, W.res.get(), lkp, R.nc,
+                           R.act);
         rsrc = W.res.get();
     }
-    launch_rows_multi<VT>(c.stream, lv.R, std::is_same<VT, double>::value ? MT_F64 : MT_F32,
-                          OP_SPMV, rsrc, ld, nullptr, nullptr, W.f[l].get(), ldc, R.nc, R.act);
+    launch_rows_il<VT>(c.stream, lv.R, std::is_same<VT, double>::value ? MT_F64 : MT_F32,
+                       OP_SPMV, rsrc, nullptr, nullptr, W.f[l].get(), lkp, R.nc, R.act);
     MState<VT> &sc = R.st[l];
     sc.zero = true;
     for (int i = 1; i <= B.num_grid_sweeps[l - 1]; i++) {
@@ -225,14 +242,14 @@ void cycle_m(MRun<VT> &R, int l)
     }
     if (!sc.zero) {
         if (s.zero) {
-            launch_fill_multi<VT>(c.stream, A.n, VT(0), s.x, ld, R.nc, R.act);
+            launch_fill_il<VT>(c.stream, A.n, VT(0), s.x, lkp, R.nc, R.act);
             s.zero = false;
         }
-        launch_prolong_multi<VT>(c.stream, A.n, lv.agg.get(), pvals<VT>(lv), sc.x, ldc, s.x, ld,
-                                 R.nc, R.act);
+        launch_prolong_il<VT>(c.stream, A.n, lv.agg.get(), pvals<VT>(lv), sc.x, s.x, lkp, R.nc,
+                              R.act);
     }
     relax_level_m<VT>(R, lv, mt, B.prm.relax_type, B.prm.postrelax_sweeps, f, s,
-                      top ? R.final_out : nullptr, ld);
+                      top ? R.final_out : nullptr);
 }
 
 template <typename VT>
@@ -249,16 +266,16 @@ void run_hierarchy_m(MRun<VT> &R, int k)
     cycle_m<VT>(Rk, 1);
 }
 
-/* precond_apply_t of host_amg.hip from x = 0 on a column block */
+/* precond_apply_t of host_amg.hip from x = 0 on interleaved blocks */
 template <typename VT>
 void precond_m(Ctx &c, Boot &B, MultiWork &M, const VT *r, VT *x, const int *act)
 {
     CycleBufs<VT> &W = mbufs<VT>(M);
     const int n = B.A0->n;
-    const long ld = M.ld[0];
+    const int lkp = M.lkp;
     const int nc = M.kmax;
     const int nh = (int)B.H.size();
-    std::vector<MState<VT>> st(M.ld.size());
+    std::vector<MState<VT>> st(M.nl.size());
     if (B.prm.solver_type == 0 || B.prm.solver_type == 1) {
         st[0].x = x;
         st[0].x2 = W.xa[0].get();
@@ -273,37 +290,42 @@ void precond_m(Ctx &c, Boot &B, MultiWork &M, const VT *r, VT *x, const int *act
             run_hierarchy_m<VT>(R, order[t]);
         }
         if (st[0].zero)
-            launch_fill_multi<VT>(c.stream, n, VT(0), x, ld, nc, act);
+            launch_fill_il<VT>(c.stream, n, VT(0), x, lkp, nc, act);
         else if (st[0].x != x)
-            launch_copy_multi<VT, VT>(c.stream, n, st[0].x, ld, x, ld, nc, act);
+            launch_copy_il<VT>(c.stream, n, st[0].x, x, lkp, nc, act);
     } else {
-        launch_fill_multi<VT>(c.stream, n, VT(0), W.xadd.get(), ld, nc, act);
+        launch_fill_il<VT>(c.stream, n, VT(0), W.xadd.get(), lkp, nc, act);
         for (int k = 0; k < nh; k++) {
             st[0].x = W.xb[0].get();
             st[0].x2 = W.xa[0].get();
             st[0].zero = true;
             MRun<VT> R{c, B, M, B.H[k], st, r, nullptr, nc, act};
             run_hierarchy_m<VT>(R, k);
-            if (st[0].zero) launch_fill_multi<VT>(c.stream, n, VT(0), st[0].x, ld, nc, act);
-            launch_axpy_multi<VT>(c.stream, n, VT(1), st[0].x, W.xadd.get(), ld, nc, act);
+            if (st[0].zero) launch_fill_il<VT>(c.stream, n, VT(0), st[0].x, lkp, nc, act);
+            launch_axpy_il<VT>(c.stream, n, VT(1), st[0].x, W.xadd.get(), lkp, nc, act);
         }
-        launch_scale_multi<VT>(c.stream, n, VT(1) / (VT)nh, W.xadd.get(), ld, nc, act);
-        launch_copy_multi<VT, VT>(c.stream, n, W.xadd.get(), ld, x, ld, nc, act);
+        launch_scale_il<VT>(c.stream, n, VT(1) / (VT)nh, W.xadd.get(), lkp, nc, act);
+        launch_copy_il<VT>(c.stream, n, W.xadd.get(), x, lkp, nc, act);
     }
 }
 
-/* Z = B(R) for the masked slots, FP64 or through the FP32 cycle */
+/* Z = B(R) for the masked slots: R is turned into the cycle's interleaved
+   layout (rounded to FP32 for a reduced-precision hierarchy), the cycles run,
+   and the result comes back column-major in FP64 */
 void precond_block(Ctx &c, Boot &B, MultiWork &M, const int *act)
 {
     const int n = B.A0->n;
-    const long ld = M.ld[0];
+    const long ld = M.F.ld;
+    const int lkp = M.lkp, nc = M.kmax;
     if (B.prec == BCMG_PREC_FP64) {
-        precond_m<double>(c, B, M, M.F.R, M.F.Z, act);
+        launch_to_il<double, double>(c.stream, n, M.F.R, ld, M.r64.get(), lkp, nc, act);
+        precond_m<double>(c, B, M, M.r64.get(), M.z64.get(), act);
+        launch_from_il<double, double>(c.stream, n, M.z64.get(), M.F.Z, ld, lkp, nc, act);
         return;
     }
-    launch_copy_multi<double, float>(c.stream, n, M.F.R, ld, M.r32.get(), ld, M.kmax, act);
+    launch_to_il<double, float>(c.stream, n, M.F.R, ld, M.r32.get(), lkp, nc, act);
     precond_m<float>(c, B, M, M.r32.get(), M.z32.get(), act);
-    launch_copy_multi<float, double>(c.stream, n, M.z32.get(), ld, M.F.Z, ld, M.kmax, act);
+    launch_from_il<float, double>(c.stream, n, M.z32.get(), M.F.Z, ld, lkp, nc, act);
 }
 
 void upload_flags(Ctx &c, MultiWork &M)
@@ -556,20 +578,19 @@ size_t multi_work_bytes(const Boot &B, int kmax)
 {
     size_t maxL = 0;
     for (auto &h : B.H) maxL = std::max(maxL, h.L.size());
-    size_t per_col64 = 0, per_col32 = 0, ld0 = 0;
+    const size_t kp = (size_t)1 << log2_cols(kmax);
+    size_t cyc = 0, n0 = 0;
     for (size_t l = 0; l < maxL; l++) {
         size_t nl = 0;
         for (auto &h : B.H)
             if (l < h.L.size()) nl = std::max(nl, (size_t)h.L[l].mat().n);
-        const size_t ldl = (size_t)round_ld(nl);
-        if (l == 0) ld0 = ldl;
-        per_col64 += 3 * ldl * 8;
-        per_col32 += 3 * ldl * 4;
+        if (l == 0) n0 = nl;
+        cyc += 3 * nl * kp;
     }
-    per_col64 += ld0 * 8 * (1 + 9 + (B.prm.solver_type == 2 ? 2 : 0));
-    per_col32 += ld0 * 4 * (1 + 2 + (B.prm.solver_type == 2 ? 2 : 0));
-    const size_t per_col = per_col64 + (B.prec != BCMG_PREC_FP64 ? per_col32 : 0);
-    return (size_t)kmax * per_col + multi_part_size(B.A0->n, kmax) * 8 + (size_t)kmax * 80;
+    cyc += n0 * kp * (3 + (B.prm.solver_type == 2 ? 2 : 0)); /* res, r, z (+ additive) */
+    const size_t vt = B.prec == BCMG_PREC_FP64 ? 8 : 4;
+    const size_t fcg = 9 * (size_t)round_ld(n0) * kmax * 8;
+    return cyc * vt + fcg + multi_part_size(B.A0->n, kmax) * 8 + (size_t)kmax * 80;
 }
 
 /* k right-hand sides through one slot engine: admit all, iterate until

@@ -230,15 +230,16 @@ __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restri
                                                       const VT *__restrict__ f,
                                                       const VT *__restrict__ xin,
                                                       VT *__restrict__ xout, int sweeps,
-                                                      int zero, long ld,
+                                                      int zero, long cs, int rs,
                                                       const int *__restrict__ act)
 {
-    /* block j works on column j of (f, xin, xout), columns ld apart; columns
-       whose act flag is 0 are skipped (act == nullptr: one column) */
+    /* block j works on column j of (f, xin, xout): element i of column j at
+       j * cs + i * rs; columns whose act flag is 0 are skipped (act ==
+       nullptr: one column) */
     if (act && !act[blockIdx.x]) return;
-    f += (size_t)blockIdx.x * ld;
-    xin += (size_t)blockIdx.x * ld;
-    xout += (size_t)blockIdx.x * ld;
+    f += (size_t)blockIdx.x * cs;
+    xin += (size_t)blockIdx.x * cs;
+    xout += (size_t)blockIdx.x * cs;
     extern __shared__ double smem[];
     VT *xs = reinterpret_cast<VT *>(smem);
     VT *cur = xs, *nxt = xs + n;
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restri
         if (i < n) {
             const int j0 = rp[i];
             len[q] = rp[i + 1] - j0;
-            fi[q] = f[i];
+            fi[q] = f[(size_t)i * rs];
             di[q] = d[i];
 #pragma unroll
             for (int k = 0; k < MAXNZ; k++) {
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restri
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
         const int i = threadIdx.x + 1024 * q;
-        if (i < n) cur[i] = zero ? fi[q] / di[q] : xin[i];
+        if (i < n) cur[i] = zero ? fi[q] / di[q] : xin[(size_t)i * rs];
     }
     __syncthreads();
     for (int s = zero ? 1 : 0; s < sweeps; s++) {
@@ -290,7 +291,7 @@ __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restri
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
         const int i = threadIdx.x + 1024 * q;
-        if (i < n) xout[i] = cur[i];
+        if (i < n) xout[(size_t)i * rs] = cur[i];
     }
 }
 

@@ -195,6 +195,296 @@ __global__ __launch_bounds__(BS) void k_scale_multi(int n, VT alpha, VT *__restr
     x[o] = alpha * x[o];
 }
 
+/* --------------------------------------------- interleaved column blocks
+   The cycle's blocks keep a row's columns together: element (i, j) at
+   i * kp + j, kp = 2^lkp >= the number of columns. A gathered row is then
+   one contiguous run for all columns, whatever the gather pattern (stencil
+   neighbours, an aggregate's scattered fine rows, a coarse level's long
+   rows). Threads are (row, column) pairs, column fastest. */
+
+__device__ __forceinline__ bool il_index(long t, int lkp, int n, int nc, const int *act, int &i,
+                                         int &j)
+{
+    i = (int)(t >> lkp);
+    j = (int)(t & ((1 << lkp) - 1));
+    return i < n && col_on(j, nc, act);
+}
+
+/* W adjacent columns of one row as a native vector (one 16-byte access) */
+template <typename VT, int W>
+struct VecW {
+    typedef VT type __attribute__((ext_vector_type(W)));
+};
+template <typename VT>
+struct VecW<VT, 1> {
+    typedef VT type;
+};
+template <typename VT, int W>
+using vec_t = typename VecW<VT, W>::type;
+
+template <typename VT, int W>
+__device__ __forceinline__ VT vget(const vec_t<VT, W> &v, int u)
+{
+    if constexpr (W == 1)
+        return v;
+    else
+        return v[u];
+}
+
+/* the W columns [jg W, jg W + W) of a lane: any of them active? all? */
+__device__ __forceinline__ void group_on(int jg, int W, int nc, const int *act, bool &any,
+                                         bool &all)
+{
+    any = false;
+    all = true;
+    for (int u = 0; u < W; u++) {
+        const bool on = col_on(jg * W + u, nc, act);
+        any |= on;
+        all &= on;
+    }
+}
+
+/* stream-order row sums (k_rows_stream's arithmetic) on interleaved blocks:
+   a lane owns W adjacent columns of one row and gathers them with one
+   vector load per entry; per column p = a_ij x_j rounded, added left to
+   right from 0. Inactive columns of a partly active group are computed but
+   not stored. */
+template <typename MT, typename VT, int OP, int W>
+__global__ __launch_bounds__(BS) void k_rows_il(int n, const int *__restrict__ rp,
+                                                const int *__restrict__ ci,
+                                                const MT *__restrict__ av,
+                                                const VT *__restrict__ x,
+                                                const VT *__restrict__ f,
+                                                const VT *__restrict__ d, VT *__restrict__ y,
+                                                int lkp, int nc, const int *__restrict__ act)
+{
+    using V = vec_t<VT, W>;
+    const int kp = 1 << lkp;
+    const int gpr = kp / W; /* lanes per row */
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const long t = (long)b * BS + threadIdx.x;
+    const int r = (int)(t / gpr);
+    const int jg = (int)(t % gpr);
+    if (r >= n) return;
+    bool any, all;
+    group_on(jg, W, nc, act, any, all);
+    if (!any) return;
+    const int ra = rp[r], re = rp[r + 1];
+    V acc = V(0);
+    const VT *xg = x + jg * W;
+    /* PF entries at a time: their indices and values first, then all their
+       gathers in flight, then the sums in entry order */
+    constexpr int PF = 8;
+    for (int e0 = ra; e0 < re; e0 += PF) {
+        int cc[PF];
+        VT vv[PF];
+        V gg[PF];
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            if (e0 + u < re) {
+                cc[u] = ci[e0 + u];
+                vv[u] = mval<VT>(av[e0 + u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PF; u++)
+            if (e0 + u < re) gg[u] = *reinterpret_cast<const V *>(xg + ((size_t)cc[u] << lkp));
+#pragma unroll
+        for (int u = 0; u < PF; u++)
+            if (e0 + u < re) acc = acc + vv[u] * gg[u];
+    }
+    const size_t o = ((size_t)r << lkp) + jg * W;
+    V res;
+    if (OP == OP_SPMV) {
+        res = acc;
+    } else if (OP == OP_RESID) {
+        res = *reinterpret_cast<const V *>(f + o) - acc;
+    } else {
+        const VT dr = d[r];
+        res = (*reinterpret_cast<const V *>(f + o) - acc) / dr + *reinterpret_cast<const V *>(x + o);
+    }
+    if (all) {
+        *reinterpret_cast<V *>(y + o) = res;
+    } else {
+        for (int u = 0; u < W; u++)
+            if (col_on(jg * W + u, nc, act)) y[o + u] = vget<VT, W>(res, u);
+    }
+}
+
+/* group-order row sums (k_rows<G>'s arithmetic) on interleaved blocks: G
+   lanes per row, one column at a time */
+template <typename MT, typename VT, int OP>
+__global__ __launch_bounds__(BS) void k_rows_il_g(int n, int G, const int *__restrict__ rp,
+                                                  const int *__restrict__ ci,
+                                                  const MT *__restrict__ av,
+                                                  const VT *__restrict__ x,
+                                                  const VT *__restrict__ f,
+                                                  const VT *__restrict__ d, VT *__restrict__ y,
+                                                  int lkp, int nc, const int *__restrict__ act)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int tid = b * BS + threadIdx.x;
+    const int row = tid / G;
+    const int lane = threadIdx.x & (G - 1);
+    if (row >= n) return;
+    const int j0 = rp[row], j1 = rp[row + 1];
+    for (int j = 0; j < nc; j++) {
+        if (!col_on(j, nc, act)) continue;
+        VT t = 0;
+        for (int e = j0 + lane; e < j1; e += G)
+            t = fma(mval<VT>(av[e]), x[((size_t)ci[e] << lkp) + j], t);
+        for (int o = G >> 1; o > 0; o >>= 1) t += __shfl_xor(t, o, G);
+        if (lane == 0) {
+            const size_t o = ((size_t)row << lkp) + j;
+            if (OP == OP_SPMV)
+                y[o] = t;
+            else if (OP == OP_RESID)
+                y[o] = f[o] - t;
+            else
+                y[o] = (f[o] - t) / d[row] + x[o];
+        }
+    }
+}
+
+#define IL_BEGIN                                                                    \
+    int i, j;                                                                       \
+    if (!il_index((long)blockIdx.x * BS + threadIdx.x, lkp, n, nc, act, i, j)) return; \
+    const size_t o = ((size_t)i << lkp) + j;
+
+/* lane = (row, W adjacent columns) for the vectorised element-wise kernels */
+#define ILW_BEGIN                                                                   \
+    using V = vec_t<VT, W>;                                                         \
+    const int gpr = (1 << lkp) / W;                                                 \
+    const long t_ = (long)blockIdx.x * BS + threadIdx.x;                            \
+    const int i = (int)(t_ / gpr), jg = (int)(t_ % gpr);                            \
+    if (i >= n) return;                                                             \
+    bool any, all;                                                                  \
+    group_on(jg, W, nc, act, any, all);                                             \
+    if (!any) return;                                                               \
+    const size_t o = ((size_t)i << lkp) + jg * W;
+
+#define ILW_STORE(y, res)                                                           \
+    if (all) {                                                                      \
+        *reinterpret_cast<V *>((y) + o) = (res);                                    \
+    } else {                                                                        \
+        for (int u = 0; u < W; u++)                                                 \
+            if (col_on(jg * W + u, nc, act)) (y)[o + u] = vget<VT, W>((res), u);    \
+    }
+
+template <typename VT, int W>
+__global__ __launch_bounds__(BS) void k_jz_il(int n, const VT *__restrict__ f,
+                                              const VT *__restrict__ d, VT *__restrict__ y,
+                                              int lkp, int nc, const int *__restrict__ act)
+{
+    ILW_BEGIN
+    const V res = *reinterpret_cast<const V *>(f + o) / d[i];
+    ILW_STORE(y, res)
+}
+
+template <typename VT, int W>
+__global__ __launch_bounds__(BS) void k_prolong_il(int n, const int *__restrict__ agg,
+                                                   const VT *__restrict__ p,
+                                                   const VT *__restrict__ xc,
+                                                   VT *__restrict__ x, int lkp, int nc,
+                                                   const int *__restrict__ act)
+{
+    ILW_BEGIN
+    const VT pi = p[i];
+    const V xcv = *reinterpret_cast<const V *>(xc + ((size_t)agg[i] << lkp) + jg * W);
+    const V xo = *reinterpret_cast<const V *>(x + o);
+    V res;
+    for (int u = 0; u < W; u++) {
+        if constexpr (W == 1)
+            res = fma(pi, xcv, xo);
+        else
+            res[u] = fma(pi, xcv[u], xo[u]);
+    }
+    ILW_STORE(x, res)
+}
+
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_fill_il(int n, VT v, VT *__restrict__ x, int lkp, int nc,
+                                                const int *__restrict__ act)
+{
+    IL_BEGIN
+    x[o] = v;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_copy_il(int n, const VT *__restrict__ a,
+                                                VT *__restrict__ b, int lkp, int nc,
+                                                const int *__restrict__ act)
+{
+    IL_BEGIN
+    b[o] = a[o];
+}
+
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_axpy_il(int n, VT alpha, const VT *__restrict__ x,
+                                                VT *__restrict__ y, int lkp, int nc,
+                                                const int *__restrict__ act)
+{
+    IL_BEGIN
+    y[o] = fma(alpha, x[o], y[o]);
+}
+
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_scale_il(int n, VT alpha, VT *__restrict__ x, int lkp,
+                                                 int nc, const int *__restrict__ act)
+{
+    IL_BEGIN
+    x[o] = alpha * x[o];
+}
+
+/* column-major (column j at a + j * lda) <-> interleaved, with the type
+   conversion of k_convert; 64 rows per block through an LDS tile so that
+   both sides are read and written along their contiguous direction */
+constexpr int TR_ROWS = 64;
+template <typename S, typename D, bool TO_IL>
+__global__ __launch_bounds__(BS) void k_transpose_il(int n, const S *__restrict__ a,
+                                                     D *__restrict__ b, long ldc, int lkp, int nc,
+                                                     const int *__restrict__ act)
+{
+    __shared__ D tile[TR_ROWS * (MULTI_MAX_COLS + 1)];
+    const int kp = 1 << lkp;
+    const int i0 = blockIdx.x * TR_ROWS;
+    const int w = kp + 1;
+    for (int q = threadIdx.x; q < TR_ROWS * kp; q += BS) {
+        int ii, j;
+        if (TO_IL) {
+            /* read column-major along rows */
+            j = q / TR_ROWS;
+            ii = q % TR_ROWS;
+        } else {
+            ii = q >> lkp;
+            j = q & (kp - 1);
+        }
+        const int i = i0 + ii;
+        if (i >= n || !col_on(j, nc, act)) continue;
+        if (TO_IL)
+            tile[ii * w + j] = (D)a[(size_t)j * ldc + i];
+        else
+            tile[ii * w + j] = (D)a[((size_t)i << lkp) + j];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < TR_ROWS * kp; q += BS) {
+        int ii, j;
+        if (TO_IL) {
+            ii = q >> lkp;
+            j = q & (kp - 1);
+        } else {
+            j = q / TR_ROWS;
+            ii = q % TR_ROWS;
+        }
+        const int i = i0 + ii;
+        if (i >= n || !col_on(j, nc, act)) continue;
+        if (TO_IL)
+            b[((size_t)i << lkp) + j] = tile[ii * w + j];
+        else
+            b[(size_t)j * ldc + i] = tile[ii * w + j];
+    }
+}
+
 /* ------------------------------------------------------------------ FCG */
 
 /* Per-column FCG state. Directions live in two slot pairs (V[s], W[s]); the
@@ -538,6 +828,45 @@ inline FcgCols fcg_cols(const MultiFcg &M)
 
 } // namespace
 
+template <typename MT, typename VT, int OP>
+void rows_il_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x, const VT *f,
+                   const VT *d, VT *y, int lkp, int nc, const int *act)
+{
+    const int n = A.n;
+    const int *rp = A.row.get(), *ci = A.col.get();
+    if (stream_ok<VT>(A)) {
+        constexpr int WM = 16 / sizeof(VT);
+        const int kp = 1 << lkp;
+        if (kp >= WM)
+            k_rows_il<MT, VT, OP, WM><<<blocks_for(((long long)n << lkp) / WM), BS, 0, s>>>(
+                n, rp, ci, av, x, f, d, y, lkp, nc, act);
+        else if (kp >= 2)
+            k_rows_il<MT, VT, OP, 2><<<blocks_for(((long long)n << lkp) / 2), BS, 0, s>>>(
+                n, rp, ci, av, x, f, d, y, lkp, nc, act);
+        else
+            k_rows_il<MT, VT, OP, 1><<<blocks_for((long long)n << lkp), BS, 0, s>>>(
+                n, rp, ci, av, x, f, d, y, lkp, nc, act);
+    } else {
+        const int G = A.group;
+        k_rows_il_g<MT, VT, OP><<<blocks_for((long long)n * G), BS, 0, s>>>(
+            n, G, rp, ci, av, x, f, d, y, lkp, nc, act);
+    }
+}
+
+template <typename MT, typename VT>
+void rows_il_op(hipStream_t s, const DevCsr &A, const MT *av, RowOp op, const VT *x, const VT *f,
+                const VT *d, VT *y, int lkp, int nc, const int *act)
+{
+    if (op == OP_SPMV)
+        rows_il_typed<MT, VT, OP_SPMV>(s, A, av, x, f, d, y, lkp, nc, act);
+    else if (op == OP_RESID)
+        rows_il_typed<MT, VT, OP_RESID>(s, A, av, x, f, d, y, lkp, nc, act);
+    else
+        rows_il_typed<MT, VT, OP_JACOBI>(s, A, av, x, f, d, y, lkp, nc, act);
+}
+
+inline int il_blocks(int n, int lkp) { return blocks_for((long long)n << lkp); }
+
 /* ============================================================== host */
 
 template <typename VT>
@@ -637,6 +966,138 @@ template void launch_copy_multi<double, float>(hipStream_t, int, const double *,
                                                long, int, const int *);
 template void launch_copy_multi<float, double>(hipStream_t, int, const float *, long, double *,
                                                long, int, const int *);
+
+template <typename VT>
+void launch_rows_il(hipStream_t s, const DevCsr &A, MatType mt, RowOp op, const VT *x,
+                    const VT *f, const VT *d, VT *y, int lkp, int nc, const int *act)
+{
+    if (A.n <= 0 || nc <= 0) return;
+    if (nc > MULTI_MAX_COLS || nc > (1 << lkp))
+        throw Error(BCMG_EINVAL, "interleaved block narrower than its columns");
+    if constexpr (std::is_same<VT, double>::value) {
+        if (mt != MT_F64) throw Error(BCMG_EINVAL, "FP64 vectors need FP64 matrix values");
+        rows_il_op<double, double>(s, A, A.val.get(), op, x, f, d, y, lkp, nc, act);
+    } else {
+        if (mt == MT_F32)
+            rows_il_op<float, float>(s, A, A.val32.get(), op, x, f, d, y, lkp, nc, act);
+        else if (mt == MT_BF16)
+            rows_il_op<unsigned short, float>(s, A, A.val16.get(), op, x, f, d, y, lkp, nc, act);
+        else
+            rows_il_op<double, float>(s, A, A.val.get(), op, x, f, d, y, lkp, nc, act);
+    }
+    BCMG_CHECK_LAUNCH();
+}
+
+/* lanes of W = 16 / sizeof(VT) columns when the block is that wide */
+#define IL_VEC_LAUNCH(KERNEL, VT, ...)                                                   \
+    do {                                                                                 \
+        constexpr int WM = 16 / sizeof(VT);                                              \
+        const int kp = 1 << lkp;                                                         \
+        if (kp >= WM)                                                                    \
+            KERNEL<VT, WM><<<blocks_for(((long long)n << lkp) / WM), BS, 0, s>>>(__VA_ARGS__); \
+        else if (kp >= 2)                                                                \
+            KERNEL<VT, 2><<<blocks_for(((long long)n << lkp) / 2), BS, 0, s>>>(__VA_ARGS__); \
+        else                                                                             \
+            KERNEL<VT, 1><<<blocks_for((long long)n << lkp), BS, 0, s>>>(__VA_ARGS__);   \
+    } while (0)
+
+template <typename VT>
+void launch_jacobi_zero_il(hipStream_t s, int n, const VT *f, const VT *d, VT *y, int lkp,
+                           int nc, const int *act)
+{
+    if (n <= 0 || nc <= 0) return;
+    IL_VEC_LAUNCH(k_jz_il, VT, n, f, d, y, lkp, nc, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_prolong_il(hipStream_t s, int n, const int *agg, const VT *p, const VT *xc, VT *x,
+                       int lkp, int nc, const int *act)
+{
+    if (n <= 0 || nc <= 0) return;
+    IL_VEC_LAUNCH(k_prolong_il, VT, n, agg, p, xc, x, lkp, nc, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_fill_il(hipStream_t s, int n, VT v, VT *x, int lkp, int nc, const int *act)
+{
+    if (n <= 0 || nc <= 0) return;
+    k_fill_il<VT><<<il_blocks(n, lkp), BS, 0, s>>>(n, v, x, lkp, nc, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_copy_il(hipStream_t s, int n, const VT *a, VT *b, int lkp, int nc, const int *act)
+{
+    if (n <= 0 || nc <= 0 || a == b) return;
+    k_copy_il<VT><<<il_blocks(n, lkp), BS, 0, s>>>(n, a, b, lkp, nc, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_axpy_il(hipStream_t s, int n, VT alpha, const VT *x, VT *y, int lkp, int nc,
+                    const int *act)
+{
+    if (n <= 0 || nc <= 0) return;
+    k_axpy_il<VT><<<il_blocks(n, lkp), BS, 0, s>>>(n, alpha, x, y, lkp, nc, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_scale_il(hipStream_t s, int n, VT alpha, VT *x, int lkp, int nc, const int *act)
+{
+    if (n <= 0 || nc <= 0) return;
+    k_scale_il<VT><<<il_blocks(n, lkp), BS, 0, s>>>(n, alpha, x, lkp, nc, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename S, typename D>
+void launch_to_il(hipStream_t s, int n, const S *a, long lda, D *b, int lkp, int nc,
+                  const int *act)
+{
+    if (n <= 0 || nc <= 0) return;
+    if (lkp > 5) throw Error(BCMG_EINVAL, "interleaved block wider than 32 columns");
+    k_transpose_il<S, D, true><<<(n + TR_ROWS - 1) / TR_ROWS, BS, 0, s>>>(n, a, b, lda, lkp, nc,
+                                                                          act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename S, typename D>
+void launch_from_il(hipStream_t s, int n, const S *a, D *b, long ldb, int lkp, int nc,
+                    const int *act)
+{
+    if (n <= 0 || nc <= 0) return;
+    if (lkp > 5) throw Error(BCMG_EINVAL, "interleaved block wider than 32 columns");
+    k_transpose_il<S, D, false><<<(n + TR_ROWS - 1) / TR_ROWS, BS, 0, s>>>(n, a, b, ldb, lkp,
+                                                                           nc, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+#define BCMG_INSTANTIATE(VT)                                                                 \
+    template void launch_rows_il<VT>(hipStream_t, const DevCsr &, MatType, RowOp, const VT *, \
+                                     const VT *, const VT *, VT *, int, int, const int *);    \
+    template void launch_jacobi_zero_il<VT>(hipStream_t, int, const VT *, const VT *, VT *,  \
+                                            int, int, const int *);                          \
+    template void launch_prolong_il<VT>(hipStream_t, int, const int *, const VT *, const VT *, \
+                                        VT *, int, int, const int *);                        \
+    template void launch_fill_il<VT>(hipStream_t, int, VT, VT *, int, int, const int *);     \
+    template void launch_copy_il<VT>(hipStream_t, int, const VT *, VT *, int, int,           \
+                                     const int *);                                           \
+    template void launch_axpy_il<VT>(hipStream_t, int, VT, const VT *, VT *, int, int,       \
+                                     const int *);                                           \
+    template void launch_scale_il<VT>(hipStream_t, int, VT, VT *, int, int, const int *);
+BCMG_INSTANTIATE(double)
+BCMG_INSTANTIATE(float)
+#undef BCMG_INSTANTIATE
+template void launch_to_il<double, double>(hipStream_t, int, const double *, long, double *, int,
+                                           int, const int *);
+template void launch_to_il<double, float>(hipStream_t, int, const double *, long, float *, int,
+                                          int, const int *);
+template void launch_from_il<double, double>(hipStream_t, int, const double *, double *, long,
+                                             int, int, const int *);
+template void launch_from_il<float, double>(hipStream_t, int, const float *, double *, long,
+                                            int, int, const int *);
 
 size_t multi_part_size(int n, int nc)
 {

@@ -154,46 +154,101 @@ __global__ __launch_bounds__(BS) void k_weights_C(int n, const int *__restrict__
     C[i] = c;
 }
 
+/* The weight kernels run over entries, not rows: block b owns rows
+   [256 b, 256 b + 256) and its threads walk the block's entries with
+   consecutive lanes on consecutive entries (coalesced); an entry's row is
+   found by binary search in the block's row pointers, kept in LDS. */
+__device__ __forceinline__ int row_in_block(const int *__restrict__ srp, int nr, int e)
+{
+    int lo = 0, hi = nr - 1; /* largest q with srp[q] <= e (empty rows skipped) */
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (srp[mid] <= e)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int load_block_rows(const int *__restrict__ rp, int n, int *srp,
+                                               int &r0)
+{
+    r0 = blockIdx.x * BS;
+    const int nr = min(BS, n - r0);
+    for (int q = threadIdx.x; q <= nr; q += BS) srp[q] = rp[r0 + q];
+    __syncthreads();
+    return nr;
+}
+
 /* AH_ij = 1 - 2 a_ij w_i w_j / (C_i + C_j) or DBL_EPSILON
-   (src/AMG/matching.cu:200-238) into W; min |AH| by atomicMin on the bit
-   pattern (non-negative doubles order like their unsigned bit patterns). */
+   (src/AMG/matching.cu:200-238) into W (diagonal slots 0); per block the
+   minimum of |AH| as a bit pattern (non-negative doubles order like their
+   unsigned bit patterns) into bmin[b] */
 __global__ __launch_bounds__(BS) void k_weights_AH(int n, const int *__restrict__ rp,
                                                    const int *__restrict__ ci,
                                                    const double *__restrict__ av,
                                                    const double *__restrict__ w,
                                                    const double *__restrict__ C,
                                                    double *__restrict__ W,
-                                                   unsigned long long *__restrict__ minbits)
+                                                   unsigned long long *__restrict__ bmin)
 {
-    const int i = blockIdx.x * BS + threadIdx.x;
+    __shared__ int srp[BS + 1];
+    __shared__ unsigned long long sm[BS / 64];
+    int r0;
+    const int nr = load_block_rows(rp, n, srp, r0);
     unsigned long long mb = 0x7FEFFFFFFFFFFFFFull; /* DBL_MAX */
-    if (i < n) {
-        for (int j = rp[i]; j < rp[i + 1]; j++) {
-            const int c = ci[j];
-            if (c == i) {
-                W[j] = 0.0;
-                continue;
-            }
-            const int lo = c < i ? c : i, hi = c < i ? i : c;
-            const double norm = C[lo] + C[hi];
-            double ah;
-            if (norm > DBL_EPSILON) {
-                const double wt = w[lo] * w[hi];
-                ah = 1. - ((2. * av[j] * wt) / norm);
-            } else {
-                ah = DBL_EPSILON;
-            }
-            W[j] = ah;
-            const unsigned long long b = __double_as_longlong(fabs(ah));
-            mb = b < mb ? b : mb;
+    for (int e = srp[0] + threadIdx.x; e < srp[nr]; e += BS) {
+        const int i = r0 + row_in_block(srp, nr, e);
+        const int c = ci[e];
+        if (c == i) {
+            W[e] = 0.0;
+            continue;
         }
+        const int lo = c < i ? c : i, hi = c < i ? i : c;
+        const double norm = C[lo] + C[hi];
+        double ah;
+        if (norm > DBL_EPSILON) {
+            const double wt = w[lo] * w[hi];
+            ah = 1. - ((2. * av[e] * wt) / norm);
+        } else {
+            ah = DBL_EPSILON;
+        }
+        W[e] = ah;
+        const unsigned long long bts = __double_as_longlong(fabs(ah));
+        mb = bts < mb ? bts : mb;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         const unsigned long long t = __shfl_xor(mb, o, 64);
         mb = t < mb ? t : mb;
     }
-    if ((threadIdx.x & 63) == 0) atomicMin(minbits, mb);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mb;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < BS / 64; q++) mb = sm[q] < mb ? sm[q] : mb;
+        bmin[blockIdx.x] = mb < sm[0] ? mb : sm[0];
+    }
+}
+
+/* minimum of the per-block minima (one block; order-independent) */
+__global__ __launch_bounds__(BS) void k_min_final(int nb, const unsigned long long *__restrict__ bmin,
+                                                  unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long sm[BS / 64];
+    unsigned long long mb = 0x7FEFFFFFFFFFFFFFull;
+    for (int q = threadIdx.x; q < nb; q += BS) mb = bmin[q] < mb ? bmin[q] : mb;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long t = __shfl_xor(mb, o, 64);
+        mb = t < mb ? t : mb;
+    }
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mb;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < BS / 64; q++) mb = sm[q] < mb ? sm[q] : mb;
+        out[0] = mb;
+    }
 }
 
 /* W = log(|AH| / (0.999 m)) (src/AMG/matching.cu:386-396); diagonal slots 0 */
@@ -202,12 +257,15 @@ __global__ __launch_bounds__(BS) void k_weights_log(int n, const int *__restrict
                                                     const unsigned long long *__restrict__ minbits,
                                                     double *__restrict__ W)
 {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i >= n) return;
+    __shared__ int srp[BS + 1];
+    int r0;
+    const int nr = load_block_rows(rp, n, srp, r0);
     const double m = __longlong_as_double(*minbits);
     const double den = 0.999 * m;
-    for (int j = rp[i]; j < rp[i + 1]; j++)
-        if (ci[j] != i) W[j] = bcmg_log(fabs(W[j]) / den);
+    for (int e = srp[0] + threadIdx.x; e < srp[nr]; e += BS) {
+        const int i = r0 + row_in_block(srp, nr, e);
+        if (ci[e] != i) W[e] = bcmg_log(fabs(W[e]) / den);
+    }
 }
 
 /* ------------------------------------------------------------- suitor */
@@ -678,14 +736,15 @@ void launch_abs_rowsum(hipStream_t s, const DevCsr &A, double *M)
 void match_weights(Ctx &c, const DevCsr &A, const double *w, double *W)
 {
     const int n = A.n;
+    if (n <= 0) return;
     DBuf<double> C(n);
-    DBuf<unsigned long long> mb(1);
-    k_fill_u64<<<1, BS, 0, c.stream>>>(1, 0x7FEFFFFFFFFFFFFFull, mb.get()); /* DBL_MAX */
-    k_weights_C<<<blocks_for(n), BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(), w,
-                                                     C.get());
-    k_weights_AH<<<blocks_for(n), BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(), w,
-                                                      C.get(), W, mb.get());
-    k_weights_log<<<blocks_for(n), BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), mb.get(), W);
+    const int nb = blocks_for(n);
+    DBuf<unsigned long long> mb(nb + 1);
+    k_weights_C<<<nb, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(), w, C.get());
+    k_weights_AH<<<nb, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(), w, C.get(), W,
+                                          mb.get() + 1);
+    k_min_final<<<1, BS, 0, c.stream>>>(nb, mb.get() + 1, mb.get());
+    k_weights_log<<<nb, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), mb.get(), W);
     BCMG_CHECK_LAUNCH();
 }
 

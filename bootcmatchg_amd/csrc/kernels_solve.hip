@@ -390,24 +390,24 @@ void rows_op(hipStream_t s, const DevCsr &A, const MT *av, RowOp op, const VT *x
 
 template <typename MT, typename VT>
 void relax_block_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *d, const VT *f,
-                       const VT *xin, VT *xout, int sweeps, int z, int ncols, long ld,
-                       const int *act)
+                       const VT *xin, VT *xout, int sweeps, int z, int ncols, long cs,
+                       int rs, const int *act)
 {
     const size_t lds = (size_t)2 * A.n * sizeof(VT);
     const int *rp = A.row.get(), *ci = A.col.get();
     const int n = A.n;
     if (n <= 1024 && A.maxrow <= 8)
         k_relax_block<MT, VT, 1, 8><<<ncols, 1024, lds, s>>>(n, rp, ci, av, d, f, xin, xout,
-                                                              sweeps, z, ld, act);
+                                                              sweeps, z, cs, rs, act);
     else if (n <= 4096 && A.maxrow <= 8)
         k_relax_block<MT, VT, 4, 8><<<ncols, 1024, lds, s>>>(n, rp, ci, av, d, f, xin, xout,
-                                                              sweeps, z, ld, act);
+                                                              sweeps, z, cs, rs, act);
     else if (n <= 2048 && A.maxrow <= 16)
         k_relax_block<MT, VT, 2, 16><<<ncols, 1024, lds, s>>>(n, rp, ci, av, d, f, xin, xout,
-                                                               sweeps, z, ld, act);
+                                                               sweeps, z, cs, rs, act);
     else if (n <= 1024 && A.maxrow <= 32)
         k_relax_block<MT, VT, 1, 32><<<ncols, 1024, lds, s>>>(n, rp, ci, av, d, f, xin, xout,
-                                                               sweeps, z, ld, act);
+                                                               sweeps, z, cs, rs, act);
     else
         throw Error(BCMG_EINVAL, "relax_block: level too large");
 }
@@ -507,24 +507,24 @@ bool relax_block_ok(const DevCsr &A)
 
 template <typename VT>
 void launch_relax_block(hipStream_t s, const DevCsr &A, MatType mt, const VT *d, const VT *f,
-                        const VT *xin, VT *xout, int sweeps, bool zero, int ncols, long ld,
-                        const int *act)
+                        const VT *xin, VT *xout, int sweeps, bool zero, int ncols, long cs,
+                        int rs, const int *act)
 {
     const int z = zero ? 1 : 0;
     if (ncols <= 0) return;
     if constexpr (std::is_same<VT, double>::value) {
         relax_block_typed<double, double>(s, A, A.val.get(), d, f, xin, xout, sweeps, z, ncols,
-                                          ld, act);
+                                          cs, rs, act);
     } else {
         if (mt == MT_F32)
             relax_block_typed<float, float>(s, A, A.val32.get(), d, f, xin, xout, sweeps, z,
-                                            ncols, ld, act);
+                                            ncols, cs, rs, act);
         else if (mt == MT_BF16)
             relax_block_typed<unsigned short, float>(s, A, A.val16.get(), d, f, xin, xout,
-                                                     sweeps, z, ncols, ld, act);
+                                                     sweeps, z, ncols, cs, rs, act);
         else
             relax_block_typed<double, float>(s, A, A.val.get(), d, f, xin, xout, sweeps, z,
-                                             ncols, ld, act);
+                                             ncols, cs, rs, act);
     }
     BCMG_CHECK_LAUNCH();
 }
@@ -597,7 +597,7 @@ void launch_f2d(hipStream_t s, size_t n, const float *a, double *b)
     template bool relax_block_ok<VT>(const DevCsr &);                                        \
     template void launch_relax_block<VT>(hipStream_t, const DevCsr &, MatType, const VT *,   \
                                          const VT *, const VT *, VT *, int, bool, int, long, \
-                                         const int *);                                       \
+                                         int, const int *);                                  \
     template void launch_jacobi_zero<VT>(hipStream_t, int, const VT *, const VT *, VT *);    \
     template void launch_prolong<VT>(hipStream_t, int, const int *, const VT *, const VT *,  \
                                      VT *);                                                  \
