@@ -74,19 +74,54 @@ __global__ __launch_bounds__(BS) void k_rows_stream(int n, const int *__restrict
         y[r] = (fr - t) / dr + xr;
 }
 
-/* largest nnz of any 256-row block (the LDS tile of k_rows_stream) into
-   maxv[0], longest row into maxv[1] */
+/* per 256-row block b: its nnz (the LDS tile of k_rows_stream) and its
+   longest row, into part[2 b], part[2 b + 1] */
 __global__ __launch_bounds__(BS) void k_block_nnz(int n, const int *__restrict__ rp,
-                                                  int *__restrict__ maxv)
+                                                  int *__restrict__ part)
 {
-    const int b = blockIdx.x * BS + threadIdx.x;
-    const long long r0 = (long long)b * BS;
-    if (r0 >= n) return;
-    const int r1 = (int)min(r0 + BS, (long long)n);
-    atomicMax(maxv, rp[r1] - rp[r0]);
-    int m = 0;
-    for (int r = (int)r0; r < r1; r++) m = max(m, rp[r + 1] - rp[r]);
-    atomicMax(maxv + 1, m);
+    __shared__ int sm[BS / 64];
+    const int r0 = blockIdx.x * BS;
+    const int r = r0 + threadIdx.x;
+    int m = r < n ? rp[r + 1] - rp[r] : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < BS / 64; q++) m = max(m, sm[q]);
+        part[2 * blockIdx.x] = rp[min(r0 + BS, n)] - rp[r0];
+        part[2 * blockIdx.x + 1] = m;
+    }
+}
+
+/* maxima of the per-block pairs into out[0] (tile), out[1] (longest row) */
+__global__ __launch_bounds__(BS) void k_block_nnz_final(int nb, const int *__restrict__ part,
+                                                        int *__restrict__ out)
+{
+    __shared__ int sm[2][BS / 64];
+    int a = 0, m = 0;
+    for (int q = threadIdx.x; q < nb; q += BS) {
+        a = max(a, part[2 * q]);
+        m = max(m, part[2 * q + 1]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a = max(a, __shfl_xor(a, o, 64));
+        m = max(m, __shfl_xor(m, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sm[0][threadIdx.x >> 6] = a;
+        sm[1][threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < BS / 64; q++) {
+            a = max(a, sm[0][q]);
+            m = max(m, sm[1][q]);
+        }
+        out[0] = a;
+        out[1] = m;
+    }
 }
 
 /* Fallback for rows too long for the LDS tile: G lanes per row (a power of
@@ -427,11 +462,14 @@ int lanes_for(int n, long long nnz)
 void launch_block_nnz(Ctx &c, DevCsr &A)
 {
     A.group = lanes_for(A.n, A.nnz);
-    A.tile_dev.ensure(2);
-    BCMG_HIP(hipMemsetAsync(A.tile_dev.get(), 0, 2 * sizeof(int), c.stream));
     const int nblk = (A.n + BS - 1) / BS;
-    if (A.n > 0)
-        k_block_nnz<<<blocks_for(nblk), BS, 0, c.stream>>>(A.n, A.row.get(), A.tile_dev.get());
+    A.tile_dev.ensure(2 + 2 * (size_t)nblk);
+    if (A.n > 0) {
+        k_block_nnz<<<nblk, BS, 0, c.stream>>>(A.n, A.row.get(), A.tile_dev.get() + 2);
+        k_block_nnz_final<<<1, BS, 0, c.stream>>>(nblk, A.tile_dev.get() + 2, A.tile_dev.get());
+    } else {
+        BCMG_HIP(hipMemsetAsync(A.tile_dev.get(), 0, 2 * sizeof(int), c.stream));
+    }
     BCMG_CHECK_LAUNCH();
     A.tile = -1; /* pending until resolve_tiles */
 }
