@@ -598,24 +598,26 @@ __global__ __launch_bounds__(BS) void k_rap_fill(int nc, const int *__restrict__
     for (int e = s0 + lane; e < s1; e += G) {
         if (!first[e]) continue;
         const int J = Jst[e];
+        /* one pass over the staged entries: the rank of J among the row's
+           distinct columns, and the two members' sums in staging order */
         int rank = 0;
-        for (int e2 = s0; e2 < s1; e2++) rank += (first[e2] && Jst[e2] < J) ? 1 : 0;
+        double sa = 0.0, sb = 0.0;
+        bool ha = false, hb = false;
+        for (int e2 = s0; e2 < s1; e2++) {
+            const int J2 = Jst[e2];
+            rank += (J2 < J && first[e2]) ? 1 : 0;
+            if (J2 == J) {
+                if (e2 < split) {
+                    sa = sa + tst[e2];
+                    ha = true;
+                } else {
+                    sb = sb + tst[e2];
+                    hb = true;
+                }
+            }
+        }
         double acc = 0.0;
-        double sa = 0.0;
-        bool ha = false;
-        for (int e2 = s0; e2 < split; e2++)
-            if (Jst[e2] == J) {
-                sa = sa + tst[e2];
-                ha = true;
-            }
         if (ha) acc = acc + p0 * sa;
-        double sb = 0.0;
-        bool hb = false;
-        for (int e2 = split; e2 < s1; e2++)
-            if (Jst[e2] == J) {
-                sb = sb + tst[e2];
-                hb = true;
-            }
         if (hb) acc = acc + p1 * sb;
         colc[out0 + rank] = J;
         valc[out0 + rank] = acc;
