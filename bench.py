@@ -180,15 +180,25 @@ def multi_rhs(bc, ctx, steps):
     return out
 
 
-def serving(bc, ctx):
+def serving(bc, ctx, quick=False):
     """The serving workload: 8 cached operators, Poisson arrivals, uniform
     operator choice; the served loop (continuous batching) against one solve
-    at a time in FP64 on the same trace."""
+    at a time in FP64 on the same trace. quick: only the served FP32 loop on
+    a 200-request trace (for profiling)."""
     from bootcmatchg_amd import serving as sv
     pool = sv.Pool(bc, ctx, SETTINGS, 0)
     out = {"operators": [{"name": o["name"], "rows": o["n"], "nnz": o["nnz"]} for o in pool.ops],
            "settings": "0,1,0.8,2,0,39,0,4,4,16,4,4,2000,1e-8",
            "slots_per_operator": 16}
+    if quick:
+        pool.set_precision(1)
+        S = bc.Server(ctx, 96 << 30, 16)
+        for o in pool.ops:
+            S.add(o["B"])
+        arr, opsel, rhsel = sv.trace(4000, len(pool.ops), 200, 3.0, seed=1)
+        out["served_fp32"] = sv.run_served(pool, S, arr, opsel, rhsel)
+        S.close()
+        return out
     arr, opsel, rhsel = sv.trace(4000, len(pool.ops), 400, 3.0, seed=1)
     out["saturation_trace"] = "%d requests offered at 4000/s" % len(arr)
     out["plain_fp64_one_at_a_time"] = sv.run_plain(pool, arr, opsel, rhsel)
@@ -241,6 +251,8 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every FCG kernel from the host instead of replaying "
                          "captured HIP graphs (used for kernel-trace profiling)")
+    ap.add_argument("--only", choices=("multi", "serving"), default=None,
+                    help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path has no multi-GPU form here "
@@ -249,6 +261,18 @@ def main():
 
     import bootcmatchg_amd as bc
     from bootcmatchg_amd import operators as ops
+
+    if args.only:
+        ctx = bc.Context(0)
+        if args.no_graphs:
+            bc.use_graphs(ctx, False)
+        if args.only == "multi":
+            res = multi_rhs(bc, ctx, max(1, args.steps // 2))
+        else:
+            res = serving(bc, ctx, quick=args.steps <= 1)
+        print(json.dumps({args.only: res}))
+        ctx.close()
+        return
 
     N = args.grid
     A = ops.poisson3d(N)

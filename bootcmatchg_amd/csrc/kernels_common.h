@@ -50,7 +50,7 @@ __device__ __forceinline__ void stage_products(const int *__restrict__ ci,
                                                const VT *__restrict__ x, int e0, int cnt,
                                                VT *__restrict__ prod)
 {
-    constexpr int U = 4;
+    constexpr int U = 8;
     for (int k0 = threadIdx.x; k0 < cnt; k0 += BS * U) {
         int c[U];
         MT v[U];
