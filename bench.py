@@ -256,7 +256,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the path has no multi-GPU form here "
-              "(each GPU would serve its own solves; see DESIGN.md)", file=sys.stderr)
+              "(each GPU would serve its own operators and requests)", file=sys.stderr)
         sys.exit(2)
 
     import bootcmatchg_amd as bc

@@ -2,7 +2,7 @@
  * oracle.h - C interface of the serial CPU oracle for BootCMatchG's
  * AMG-preconditioned flexible CG (setup + solve).
  *
- * TEST INFRASTRUCTURE ONLY. The oracle is the checker that the HIP library is
+ * TEST INFRASTRUCTURE ONLY. The oracle is what the HIP library's results are
  * compared against: only tests/, __graft_entry__.smoke() and bench.py's CPU
  * baseline leg may load it. The product (bootcmatchg_amd, include/) never
  * links or calls it.

@@ -120,3 +120,25 @@ def test_operator_generators_are_deterministic():
     c = ops.lognormal3d(5, 4.0, seed=2)
     assert all(np.array_equal(x, y) for x, y in zip(a, b))
     assert not np.array_equal(a[2], c[2])
+
+
+def test_extension_entry_points_validate_arguments():
+    """The extensions (precision, multi-column FCG, serving loop) reject bad
+    handles and arguments with BCMG_EINVAL before touching a device."""
+    L = bc.lib()
+    vp = ctypes.c_void_p
+    i = ctypes.c_int()
+    ll = ctypes.c_longlong()
+    assert L.bcmg_set_precision(vp(), 1) == bc.EINVAL
+    assert L.bcmg_fcg_restarts(vp(), ctypes.byref(i)) == bc.EINVAL
+    assert L.bcmg_fcg_multi(vp(), vp(), 1, vp(), vp(), vp(), vp(), vp()) == bc.EINVAL
+    assert L.bcmg_server_create(vp(), ctypes.c_size_t(1 << 30), 8, ctypes.byref(vp())) == \
+        bc.EINVAL
+    assert L.bcmg_server_add_operator(vp(), vp(), ctypes.byref(i)) == bc.EINVAL
+    assert L.bcmg_server_submit(vp(), 0, vp(), vp(), ctypes.byref(ll)) == bc.EINVAL
+    assert L.bcmg_server_step(vp(), ctypes.byref(i)) == bc.EINVAL
+    assert L.bcmg_server_result(vp(), ctypes.c_longlong(1), None, None, None, None, None) == \
+        bc.EINVAL
+    assert L.bcmg_server_completed(vp(), None, 0, ctypes.byref(i)) == bc.EINVAL
+    assert L.bcmg_server_info(vp(), None, None, None, None) == bc.EINVAL
+    assert L.bcmg_server_destroy(vp()) == bc.OK
