@@ -402,7 +402,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": SWEEP_TRAFFIC_BYTES_160 if N == GRID else None,
-                     "kernel": "level-0 Jacobi/L1 sweep (k_rows_stream<OP_JACOBI, non-temporal>)",
+                     "kernel": "level-0 Jacobi/L1 sweep (k_rows_staged<OP_JACOBI, non-temporal>)",
                      "bytes_per_launch": sweep_b, "avg_launch_ms": sweep_avg_ms,
                      "launches": sm_launches},
         "cpu_baseline": cpu,

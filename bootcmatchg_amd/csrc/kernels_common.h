@@ -11,7 +11,6 @@ namespace {
 
 constexpr int BS = 256;          /* threads per block */
 constexpr int RED_MAX_BLOCKS = 1024;
-constexpr int STREAM_TILE_BYTES = 32768;              /* LDS per block */
 /* matrices whose value+column stream exceeds this are read non-temporally */
 constexpr size_t NT_BYTES = (size_t)192 << 20;
 
@@ -39,18 +38,16 @@ __device__ __forceinline__ T ldm(const T *p)
     return *p;
 }
 
-/* Phase 1 of the row-block kernels: the block's entries [e0, e0 + cnt) of
-   col/val are streamed with consecutive lanes on consecutive entries (U
-   independent loads in flight per lane), x is gathered and the products
-   a_ij x_j are stored in LDS (prod[e - e0]). (Paired 2-element loads were
-   measured slower on gfx950 for both FP64 and FP32.) */
-template <typename MT, typename VT, bool NT>
-__device__ __forceinline__ void stage_products(const int *__restrict__ ci,
-                                               const MT *__restrict__ av,
-                                               const VT *__restrict__ x, int e0, int cnt,
-                                               VT *__restrict__ prod)
+/* Phase 1 of the staged row-block kernels: the block's entries
+   [e0, e0 + cnt) of col/val are streamed into LDS (lcol, lval) with
+   consecutive lanes on consecutive entries (coalesced, U loads in flight per
+   lane; non-temporal when NT). */
+template <typename MT, bool NT>
+__device__ __forceinline__ void stage_entries(const int *__restrict__ ci,
+                                              const MT *__restrict__ av, int e0, int cnt,
+                                              int *__restrict__ lcol, MT *__restrict__ lval)
 {
-    constexpr int U = 8;
+    constexpr int U = 4;
     for (int k0 = threadIdx.x; k0 < cnt; k0 += BS * U) {
         int c[U];
         MT v[U];
@@ -65,9 +62,59 @@ __device__ __forceinline__ void stage_products(const int *__restrict__ ci,
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int k = k0 + u * BS;
-            if (k < cnt) prod[k] = mval<VT>(v[u]) * x[c[u]];
+            if (k < cnt) {
+                lcol[k] = c[u];
+                lval[k] = v[u];
+            }
         }
     }
+}
+
+/* Phase 2: one row per thread; the row's staged entries [ka, kb) gather x
+   (consecutive rows gather neighbouring addresses, so the gathers coalesce
+   across the wave), 8 entries in flight, and p = a_ij x_j (rounded) is added
+   left to right from 0 - the serial order. */
+template <typename MT, typename VT>
+__device__ __forceinline__ VT staged_row_sum(const int *__restrict__ lcol,
+                                             const MT *__restrict__ lval,
+                                             const VT *__restrict__ x, int ka, int kb)
+{
+    VT t = 0;
+    constexpr int PF = 8;
+    for (int k0 = ka; k0 < kb; k0 += PF) {
+        const int kn = kb - k0;
+        int c[PF];
+        VT v[PF], g[PF];
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            if (u < kn) {
+                c[u] = lcol[k0 + u];
+                v[u] = mval<VT>(lval[k0 + u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PF; u++)
+            if (u < kn) g[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < PF; u++)
+            if (u < kn) t = t + v[u] * g[u];
+    }
+    return t;
+}
+
+/* LDS of the staged row kernels: the tile's columns (8-byte aligned), then
+   its values */
+inline size_t staged_lds_bytes(int tile, size_t mt_size)
+{
+    return ((size_t)(tile + 1) / 2) * 8 + (size_t)tile * mt_size;
+}
+
+template <typename MT>
+__device__ __forceinline__ void staged_lds(int tile, int *&lcol, MT *&lval)
+{
+    extern __shared__ double smem[];
+    lcol = reinterpret_cast<int *>(smem);
+    lval = reinterpret_cast<MT *>(smem + ((tile + 1) >> 1));
 }
 
 constexpr int RED_FINAL_BS = 1024; /* threads of the one-block final reductions */
@@ -210,17 +257,20 @@ inline int red_blocks(int n)
     return nb;
 }
 
+/* rows that fit the LDS tile of the staged row kernel (48 KB for values of
+   the vector type): these get the stream-order sum, others the group-order
+   kernel; single- and multi-column paths decide alike */
 template <typename VT>
 inline bool stream_ok(const DevCsr &A)
 {
-    return A.tile > 0 && (size_t)A.tile * sizeof(VT) <= (size_t)STREAM_TILE_BYTES;
+    return A.tile > 0 && staged_lds_bytes(A.tile, sizeof(VT)) <= (size_t)48 * 1024;
 }
 
 /* All `sweeps` Jacobi sweeps of a small level in one workgroup of 1024
    threads: thread t owns rows t + 1024 q (q < RPT) and keeps their entries
    (at most MAXNZ per row) in registers for all sweeps; both iterates live in
    LDS and a barrier separates the sweeps. Row sums are sequential in column
-   order (the order of k_rows_stream), so the result equals `sweeps` separate
+   order (the order of k_rows_staged), so the result equals `sweeps` separate
    launches bit for bit. */
 template <typename MT, typename VT, int RPT, int MAXNZ>
 __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restrict__ rp,

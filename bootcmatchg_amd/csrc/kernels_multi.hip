@@ -24,7 +24,7 @@ __device__ __forceinline__ bool col_on(int j, int nc, const int *act)
     return j < nc && (act == nullptr || act[j] != 0);
 }
 
-/* Stream-order row kernel (the arithmetic of k_rows_stream): one row per
+/* Stream-order row kernel (the arithmetic of k_rows_staged): one row per
    thread and CH columns per block row (chunk blockIdx.y); p = a_ij * x_j
    rounded, then added left to right from 0.
    OP_SPMV: y = A x; OP_RESID: y = f - A x; OP_JACOBI: y = (f - A x) / d + x.
@@ -244,7 +244,7 @@ __device__ __forceinline__ void group_on(int jg, int W, int nc, const int *act, 
     }
 }
 
-/* stream-order row sums (k_rows_stream's arithmetic) on interleaved blocks:
+/* stream-order row sums (k_rows_staged's arithmetic) on interleaved blocks:
    a lane owns W adjacent columns of one row and gathers them with one
    vector load per entry; per column p = a_ij x_j rounded, added left to
    right from 0. Inactive columns of a partly active group are computed but

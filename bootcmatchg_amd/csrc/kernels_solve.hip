@@ -23,26 +23,28 @@ constexpr size_t RED_UPD = 3 * RED_MAX_BLOCKS + 16;   /* update/norm partials */
 constexpr size_t RED_SPMV = 4 * RED_MAX_BLOCKS + 32;  /* SpMV+dots partials */
 
 /* Row-block kernel: block b owns rows [256 b, 256 b + 256). Phase 1 streams
-   the block's slice of val/col with consecutive lanes on consecutive entries
-   (fully coalesced; non-temporal when the matrix is larger than the 256 MB
-   last-level cache, so that the once-read stream does not push the gathered
-   x out of the caches, while a smaller matrix stays cached for the next
-   sweep) and stores the products a_ij x_j in LDS; phase 2 gives one row to
-   each thread, which sums its products in column order and applies the
-   epilogue. The row's own operands (row pointers, f, d, x) are loaded before
-   phase 1 so that their latency overlaps the stream. The sum order (products
-   rounded, then added left to right) is the serial one. */
+   the block's slice of col/val into LDS with consecutive lanes on
+   consecutive entries (coalesced; non-temporal when the matrix is larger
+   than the 256 MB last-level cache, so that the once-read stream does not
+   push the gathered x out of the caches, while a smaller matrix stays cached
+   for the next sweep); phase 2 gives one row to each thread, which gathers x
+   at its columns (coalesced across the wave for banded and stencil
+   matrices), sums the products in column order and applies the epilogue. The
+   row's own operands (row pointers, f, d, x) are loaded before phase 1 so
+   that their latency overlaps the stream. The sum order (products rounded,
+   then added left to right) is the serial one. */
 template <typename MT, typename VT, int OP, bool NT>
-__global__ __launch_bounds__(BS) void k_rows_stream(int n, const int *__restrict__ rp,
+__global__ __launch_bounds__(BS) void k_rows_staged(int n, const int *__restrict__ rp,
                                                     const int *__restrict__ ci,
                                                     const MT *__restrict__ av,
                                                     const VT *__restrict__ x,
                                                     const VT *__restrict__ f,
                                                     const VT *__restrict__ d,
-                                                    VT *__restrict__ y)
+                                                    VT *__restrict__ y, int tile)
 {
-    extern __shared__ double smem[];
-    VT *prod = reinterpret_cast<VT *>(smem);
+    int *lcol;
+    MT *lval;
+    staged_lds<MT>(tile, lcol, lval);
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int r0 = b * BS;
     const int r1 = min(r0 + BS, n);
@@ -61,11 +63,10 @@ __global__ __launch_bounds__(BS) void k_rows_stream(int n, const int *__restrict
     }
     const int e0 = rp[r0];
     const int cnt = rp[r1] - e0;
-    stage_products<MT, VT, NT>(ci, av, x, e0, cnt, prod);
+    stage_entries<MT, NT>(ci, av, e0, cnt, lcol, lval);
     __syncthreads();
     if (!own) return;
-    VT t = 0;
-    for (int k = ra - e0; k < re - e0; k++) t = t + prod[k];
+    const VT t = staged_row_sum<MT, VT>(lcol, lval, x, ra - e0, re - e0);
     if (OP == OP_SPMV)
         y[r] = t;
     else if (OP == OP_RESID)
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(BS) void k_rows_stream(int n, const int *__restrict
         y[r] = (fr - t) / dr + xr;
 }
 
-/* per 256-row block b: its nnz (the LDS tile of k_rows_stream) and its
+/* per 256-row block b: its nnz (the LDS tile of k_rows_staged) and its
    longest row, into part[2 b], part[2 b + 1] */
 __global__ __launch_bounds__(BS) void k_block_nnz(int n, const int *__restrict__ rp,
                                                   int *__restrict__ part)
@@ -288,9 +289,11 @@ __global__ __launch_bounds__(BS) void k_spmv_dots(int n, const int *__restrict__
                                                   double *__restrict__ t,
                                                   const double *__restrict__ rv,
                                                   const double *__restrict__ to,
-                                                  double *__restrict__ part)
+                                                  double *__restrict__ part, int tile)
 {
-    extern __shared__ double prod[];
+    int *lcol;
+    double *lval;
+    staged_lds<double>(tile, lcol, lval);
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int r0 = b * BS;
     const int r1 = min(r0 + BS, n);
@@ -307,14 +310,13 @@ __global__ __launch_bounds__(BS) void k_spmv_dots(int n, const int *__restrict__
     }
     const int e0 = rp[r0];
     const int cnt = rp[r1] - e0;
-    stage_products<double, double, NT>(ci, av, z, e0, cnt, prod);
+    stage_entries<double, NT>(ci, av, e0, cnt, lcol, lval);
     __syncthreads();
     double sdot[K];
 #pragma unroll
     for (int k = 0; k < K; k++) sdot[k] = 0.0;
     if (own) {
-        double tr = 0.0;
-        for (int k = ra - e0; k < re - e0; k++) tr = tr + prod[k];
+        const double tr = staged_row_sum<double, double>(lcol, lval, z, ra - e0, re - e0);
         t[r] = tr;
         sdot[0] = rr * zr;
         sdot[1] = tr * zr;
@@ -391,11 +393,13 @@ void rows_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x, const
     const int *rp = A.row.get(), *ci = A.col.get();
     if (stream_ok<VT>(A)) {
         const int nb = (n + BS - 1) / BS;
-        const size_t lds = (size_t)A.tile * sizeof(VT);
+        const size_t lds = staged_lds_bytes(A.tile, sizeof(MT));
+        if (lds > 65536) throw Error(BCMG_EUNSUP, "row tile too large for this value type");
         if ((size_t)A.nnz * (4 + sizeof(MT)) > NT_BYTES)
-            k_rows_stream<MT, VT, OP, true><<<nb, BS, lds, s>>>(n, rp, ci, av, x, f, d, y);
+            k_rows_staged<MT, VT, OP, true><<<nb, BS, lds, s>>>(n, rp, ci, av, x, f, d, y, A.tile);
         else
-            k_rows_stream<MT, VT, OP, false><<<nb, BS, lds, s>>>(n, rp, ci, av, x, f, d, y);
+            k_rows_staged<MT, VT, OP, false><<<nb, BS, lds, s>>>(n, rp, ci, av, x, f, d, y,
+                                                                  A.tile);
         return;
     }
     const int G = A.group;
@@ -721,22 +725,26 @@ void launch_spmv_dots(Ctx &c, const DevCsr &A, int k, const double *z, double *t
     if (c.red.n < red_size_for(n))
         throw Error(BCMG_EINVAL, "reduction scratch not sized for this operator");
     const int nb = (n + BS - 1) / BS;
-    const size_t lds = (size_t)A.tile * sizeof(double);
+    const size_t lds = staged_lds_bytes(A.tile, sizeof(double));
     double *part = c.red.get() + RED_SPMV;
     const bool nt = (size_t)A.nnz * 12 > NT_BYTES;
     const int *rp = A.row.get(), *ci = A.col.get();
     const double *av = A.val.get();
     if (k == 2) {
         if (nt)
-            k_spmv_dots<2, true><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part);
+            k_spmv_dots<2, true><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part,
+                                                             A.tile);
         else
-            k_spmv_dots<2, false><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part);
+            k_spmv_dots<2, false><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part,
+                                                             A.tile);
         k_fcg_final<2><<<1, RED_FINAL_BS, 0, c.stream>>>(nb, part, scal);
     } else {
         if (nt)
-            k_spmv_dots<3, true><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part);
+            k_spmv_dots<3, true><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part,
+                                                             A.tile);
         else
-            k_spmv_dots<3, false><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part);
+            k_spmv_dots<3, false><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part,
+                                                             A.tile);
         k_fcg_final<3><<<1, RED_FINAL_BS, 0, c.stream>>>(nb, part, scal);
     }
     BCMG_CHECK_LAUNCH();
