@@ -402,12 +402,15 @@ struct MultiWork {
     /* pinned staging */
     double *h_scal = nullptr, *h_tol = nullptr;
     int *h_flags = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int gnc = 0, gprec = -1;
+    /* captured iterations, one per span of slots in use (2^0 .. 2^5) */
+    hipGraphExec_t gexec[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int gprec = -1;
     void drop_graph()
     {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        gexec = nullptr;
+        for (auto &g : gexec) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
     }
     ~MultiWork()
     {

@@ -167,6 +167,7 @@ struct MRun {
     const VT *f0;
     VT *final_out;
     int nc;
+    int lkp;    /* interleave stride 2^lkp of this application */
     const int *act;
 };
 
@@ -176,7 +177,7 @@ void relax_level_m(MRun<VT> &R, const Level &lv, MatType mt, int type, int sweep
                    MState<VT> &s, VT *final_out)
 {
     Ctx &c = R.c;
-    const int lkp = R.M.lkp;
+    const int lkp = R.lkp;
     const DevCsr &A = lv.mat();
     const VT *d = divisor<VT>(lv, type);
     if (sweeps == 0) return;
@@ -211,7 +212,7 @@ void cycle_m(MRun<VT> &R, int l)
     Ctx &c = R.c;
     Boot &B = R.B;
     CycleBufs<VT> &W = mbufs<VT>(R.M);
-    const int lkp = R.M.lkp;
+    const int lkp = R.lkp;
     const Hierarchy &H = R.H;
     const int L = H.num_levels();
     const Level &lv = H.L[l - 1];
@@ -262,7 +263,7 @@ void run_hierarchy_m(MRun<VT> &R, int k)
         R.st[l].x2 = W.xb[l].get();
         R.st[l].zero = true;
     }
-    MRun<VT> Rk{R.c, R.B, R.M, H, R.st, R.f0, R.final_out, R.nc, R.act};
+    MRun<VT> Rk{R.c, R.B, R.M, H, R.st, R.f0, R.final_out, R.nc, R.lkp, R.act};
     cycle_m<VT>(Rk, 1);
 }
 
@@ -272,8 +273,11 @@ void precond_m(Ctx &c, Boot &B, MultiWork &M, const VT *r, VT *x, const int *act
 {
     CycleBufs<VT> &W = mbufs<VT>(M);
     const int n = B.A0->n;
-    const int lkp = M.lkp;
-    const int nc = M.kmax;
+    const int nc = M.F.nc; /* columns in use */
+    /* the cycle's blocks are rebuilt from the FCG blocks at every
+       application, so their stride follows the columns in use: a lightly
+       used block stays dense */
+    const int lkp = log2_cols(nc);
     const int nh = (int)B.H.size();
     std::vector<MState<VT>> st(M.nl.size());
     if (B.prm.solver_type == 0 || B.prm.solver_type == 1) {
@@ -286,7 +290,7 @@ void precond_m(Ctx &c, Boot &B, MultiWork &M, const VT *r, VT *x, const int *act
             for (int k = nh - 1; k >= 0; k--) order.push_back(k);
         for (size_t t = 0; t < order.size(); t++) {
             const bool last = (t + 1 == order.size());
-            MRun<VT> R{c, B, M, B.H[order[t]], st, r, last ? x : nullptr, nc, act};
+            MRun<VT> R{c, B, M, B.H[order[t]], st, r, last ? x : nullptr, nc, lkp, act};
             run_hierarchy_m<VT>(R, order[t]);
         }
         if (st[0].zero)
@@ -299,7 +303,7 @@ void precond_m(Ctx &c, Boot &B, MultiWork &M, const VT *r, VT *x, const int *act
             st[0].x = W.xb[0].get();
             st[0].x2 = W.xa[0].get();
             st[0].zero = true;
-            MRun<VT> R{c, B, M, B.H[k], st, r, nullptr, nc, act};
+            MRun<VT> R{c, B, M, B.H[k], st, r, nullptr, nc, lkp, act};
             run_hierarchy_m<VT>(R, k);
             if (st[0].zero) launch_fill_il<VT>(c.stream, n, VT(0), st[0].x, lkp, nc, act);
             launch_axpy_il<VT>(c.stream, n, VT(1), st[0].x, W.xadd.get(), lkp, nc, act);
@@ -316,7 +320,7 @@ void precond_block(Ctx &c, Boot &B, MultiWork &M, const int *act)
 {
     const int n = B.A0->n;
     const long ld = M.F.ld;
-    const int lkp = M.lkp, nc = M.kmax;
+    const int nc = M.F.nc, lkp = log2_cols(nc);
     if (B.prec == BCMG_PREC_FP64) {
         launch_to_il<double, double>(c.stream, n, M.F.R, ld, M.r64.get(), lkp, nc, act);
         precond_m<double>(c, B, M, M.r64.get(), M.z64.get(), act);
@@ -403,6 +407,7 @@ void SlotEngine::admit(const std::vector<int> &js, const std::vector<const doubl
     const long ld = F.ld;
     int *h_act = M.h_flags, *h_par = M.h_flags + km, *h_it = M.h_flags + 2 * km,
         *h_mask = M.h_flags + 3 * km;
+    F.nc = km; /* admission launches span every slot (masked) */
     for (int j = 0; j < km; j++) h_mask[j] = 0;
     for (size_t q = 0; q < js.size(); q++) {
         const int j = js[q];
@@ -444,13 +449,24 @@ void SlotEngine::admit(const std::vector<int> &js, const std::vector<const doubl
     /* h_flags stays untouched until the next synchronisation */
 }
 
-/* One iteration of every active slot, queued (graph replay); wait() ends it. */
+/* One iteration of every active slot, queued (graph replay); wait() ends it.
+   The launches span the slots 0 .. 2^s - 1 that hold the highest active
+   slot (admission fills the lowest free slots), one captured graph per
+   span: a lane with few requests does not pay for its idle slots. */
 void SlotEngine::launch()
 {
-    if (!any_active()) return;
-    if (M.gexec && (M.gnc != M.kmax || M.gprec != B.prec)) M.drop_graph();
+    int hi = -1;
+    for (int j = 0; j < M.kmax; j++)
+        if (slots[j].state == SLOT_RUN && M.h_flags[j]) hi = j;
+    if (hi < 0) return;
+    int cls = 0;
+    while ((1 << cls) < hi + 1) cls++;
+    const int span = std::min(1 << cls, M.kmax);
+    M.F.nc = span;
+    if (M.gprec != B.prec) M.drop_graph();
     if (c.use_graphs) {
-        if (!M.gexec) {
+        hipGraphExec_t &ge = M.gexec[cls];
+        if (!ge) {
             hipGraph_t g = nullptr;
             BCMG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
             try {
@@ -461,13 +477,12 @@ void SlotEngine::launch()
                 throw;
             }
             BCMG_HIP(hipStreamEndCapture(c.stream, &g));
-            const hipError_t e = hipGraphInstantiate(&M.gexec, g, nullptr, nullptr, 0);
+            const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             BCMG_HIP(e);
-            M.gnc = M.kmax;
             M.gprec = B.prec;
         }
-        BCMG_HIP(hipGraphLaunch(M.gexec, c.stream));
+        BCMG_HIP(hipGraphLaunch(ge, c.stream));
     } else {
         enqueue_iteration_m(c, B, M);
     }
@@ -492,6 +507,7 @@ int SlotEngine::collect()
     const bool lowp = B.prec != BCMG_PREC_FP64;
     const int max_iter = B.prm.itnlim;
     int done = 0;
+    M.F.nc = km;
     for (;;) {
         std::vector<int> stopped;
         for (int j = 0; j < km; j++)

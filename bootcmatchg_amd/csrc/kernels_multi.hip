@@ -205,8 +205,11 @@ __global__ __launch_bounds__(BS) void k_scale_multi(int n, VT alpha, VT *__restr
 __device__ __forceinline__ bool il_index(long t, int lkp, int n, int nc, const int *act, int &i,
                                          int &j)
 {
-    i = (int)(t >> lkp);
-    j = (int)(t & ((1 << lkp) - 1));
+    /* threads cover the nc columns in use of each row (the row stride stays
+       2^lkp) */
+    (void)lkp;
+    i = (int)(t / nc);
+    j = (int)(t % nc);
     return i < n && col_on(j, nc, act);
 }
 
@@ -259,8 +262,7 @@ __global__ __launch_bounds__(BS) void k_rows_il(int n, const int *__restrict__ r
                                                 int lkp, int nc, const int *__restrict__ act)
 {
     using V = vec_t<VT, W>;
-    const int kp = 1 << lkp;
-    const int gpr = kp / W; /* lanes per row */
+    const int gpr = (nc + W - 1) / W; /* lanes per row: the columns in use */
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const long t = (long)b * BS + threadIdx.x;
     const int r = (int)(t / gpr);
@@ -272,9 +274,42 @@ __global__ __launch_bounds__(BS) void k_rows_il(int n, const int *__restrict__ r
     const int ra = rp[r], re = rp[r + 1];
     V acc = V(0);
     const VT *xg = x + jg * W;
+    constexpr int PF = 8;
+    if (!all) {
+        /* a partly active group gathers only its active columns (a lane's
+           vector would fetch the idle ones too) */
+        for (int u = 0; u < W; u++) {
+            if (!col_on(jg * W + u, nc, act)) continue;
+            VT t = 0;
+            for (int e0 = ra; e0 < re; e0 += PF) {
+                int cc[PF];
+                VT vv[PF], gg[PF];
+#pragma unroll
+                for (int q = 0; q < PF; q++) {
+                    if (e0 + q < re) {
+                        cc[q] = ci[e0 + q];
+                        vv[q] = mval<VT>(av[e0 + q]);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < PF; q++)
+                    if (e0 + q < re) gg[q] = xg[((size_t)cc[q] << lkp) + u];
+#pragma unroll
+                for (int q = 0; q < PF; q++)
+                    if (e0 + q < re) t = t + vv[q] * gg[q];
+            }
+            const size_t o = ((size_t)r << lkp) + jg * W + u;
+            if (OP == OP_SPMV)
+                y[o] = t;
+            else if (OP == OP_RESID)
+                y[o] = f[o] - t;
+            else
+                y[o] = (f[o] - t) / d[r] + x[o];
+        }
+        return;
+    }
     /* PF entries at a time: their indices and values first, then all their
        gathers in flight, then the sums in entry order */
-    constexpr int PF = 8;
     for (int e0 = ra; e0 < re; e0 += PF) {
         int cc[PF];
         VT vv[PF];
@@ -354,7 +389,7 @@ __global__ __launch_bounds__(BS) void k_rows_il_g(int n, int G, const int *__res
 /* lane = (row, W adjacent columns) for the vectorised element-wise kernels */
 #define ILW_BEGIN                                                                   \
     using V = vec_t<VT, W>;                                                         \
-    const int gpr = (1 << lkp) / W;                                                 \
+    const int gpr = (nc + W - 1) / W;                                               \
     const long t_ = (long)blockIdx.x * BS + threadIdx.x;                            \
     const int i = (int)(t_ / gpr), jg = (int)(t_ % gpr);                            \
     if (i >= n) return;                                                             \
@@ -446,18 +481,17 @@ __global__ __launch_bounds__(BS) void k_transpose_il(int n, const S *__restrict_
                                                      const int *__restrict__ act)
 {
     __shared__ D tile[TR_ROWS * (MULTI_MAX_COLS + 1)];
-    const int kp = 1 << lkp;
     const int i0 = blockIdx.x * TR_ROWS;
-    const int w = kp + 1;
-    for (int q = threadIdx.x; q < TR_ROWS * kp; q += BS) {
+    const int w = nc + 1;
+    for (int q = threadIdx.x; q < TR_ROWS * nc; q += BS) {
         int ii, j;
         if (TO_IL) {
             /* read column-major along rows */
             j = q / TR_ROWS;
             ii = q % TR_ROWS;
         } else {
-            ii = q >> lkp;
-            j = q & (kp - 1);
+            ii = q / nc;
+            j = q % nc;
         }
         const int i = i0 + ii;
         if (i >= n || !col_on(j, nc, act)) continue;
@@ -467,11 +501,11 @@ __global__ __launch_bounds__(BS) void k_transpose_il(int n, const S *__restrict_
             tile[ii * w + j] = (D)a[((size_t)i << lkp) + j];
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < TR_ROWS * kp; q += BS) {
+    for (int q = threadIdx.x; q < TR_ROWS * nc; q += BS) {
         int ii, j;
         if (TO_IL) {
-            ii = q >> lkp;
-            j = q & (kp - 1);
+            ii = q / nc;
+            j = q % nc;
         } else {
             j = q / TR_ROWS;
             ii = q % TR_ROWS;
@@ -836,15 +870,14 @@ void rows_il_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x, co
     const int *rp = A.row.get(), *ci = A.col.get();
     if (stream_ok<VT>(A)) {
         constexpr int WM = 16 / sizeof(VT);
-        const int kp = 1 << lkp;
-        if (kp >= WM)
-            k_rows_il<MT, VT, OP, WM><<<blocks_for(((long long)n << lkp) / WM), BS, 0, s>>>(
-                n, rp, ci, av, x, f, d, y, lkp, nc, act);
-        else if (kp >= 2)
-            k_rows_il<MT, VT, OP, 2><<<blocks_for(((long long)n << lkp) / 2), BS, 0, s>>>(
+        if (nc >= WM)
+            k_rows_il<MT, VT, OP, WM><<<blocks_for((long long)n * ((nc + WM - 1) / WM)), BS, 0,
+                                        s>>>(n, rp, ci, av, x, f, d, y, lkp, nc, act);
+        else if (nc >= 2)
+            k_rows_il<MT, VT, OP, 2><<<blocks_for((long long)n * ((nc + 1) / 2)), BS, 0, s>>>(
                 n, rp, ci, av, x, f, d, y, lkp, nc, act);
         else
-            k_rows_il<MT, VT, OP, 1><<<blocks_for((long long)n << lkp), BS, 0, s>>>(
+            k_rows_il<MT, VT, OP, 1><<<blocks_for((long long)n * nc), BS, 0, s>>>(
                 n, rp, ci, av, x, f, d, y, lkp, nc, act);
     } else {
         const int G = A.group;
@@ -865,7 +898,7 @@ void rows_il_op(hipStream_t s, const DevCsr &A, const MT *av, RowOp op, const VT
         rows_il_typed<MT, VT, OP_JACOBI>(s, A, av, x, f, d, y, lkp, nc, act);
 }
 
-inline int il_blocks(int n, int lkp) { return blocks_for((long long)n << lkp); }
+inline int il_blocks(int n, int nc) { return blocks_for((long long)n * nc); }
 
 /* ============================================================== host */
 
@@ -992,13 +1025,14 @@ void launch_rows_il(hipStream_t s, const DevCsr &A, MatType mt, RowOp op, const 
 #define IL_VEC_LAUNCH(KERNEL, VT, ...)                                                   \
     do {                                                                                 \
         constexpr int WM = 16 / sizeof(VT);                                              \
-        const int kp = 1 << lkp;                                                         \
-        if (kp >= WM)                                                                    \
-            KERNEL<VT, WM><<<blocks_for(((long long)n << lkp) / WM), BS, 0, s>>>(__VA_ARGS__); \
-        else if (kp >= 2)                                                                \
-            KERNEL<VT, 2><<<blocks_for(((long long)n << lkp) / 2), BS, 0, s>>>(__VA_ARGS__); \
+        if (nc >= WM)                                                                    \
+            KERNEL<VT, WM><<<blocks_for((long long)n * ((nc + WM - 1) / WM)), BS, 0, s>>>(  \
+                __VA_ARGS__);                                                            \
+        else if (nc >= 2)                                                                \
+            KERNEL<VT, 2><<<blocks_for((long long)n * ((nc + 1) / 2)), BS, 0, s>>>(        \
+                __VA_ARGS__);                                                            \
         else                                                                             \
-            KERNEL<VT, 1><<<blocks_for((long long)n << lkp), BS, 0, s>>>(__VA_ARGS__);   \
+            KERNEL<VT, 1><<<blocks_for((long long)n * nc), BS, 0, s>>>(__VA_ARGS__);     \
     } while (0)
 
 template <typename VT>
@@ -1023,7 +1057,7 @@ template <typename VT>
 void launch_fill_il(hipStream_t s, int n, VT v, VT *x, int lkp, int nc, const int *act)
 {
     if (n <= 0 || nc <= 0) return;
-    k_fill_il<VT><<<il_blocks(n, lkp), BS, 0, s>>>(n, v, x, lkp, nc, act);
+    k_fill_il<VT><<<il_blocks(n, nc), BS, 0, s>>>(n, v, x, lkp, nc, act);
     BCMG_CHECK_LAUNCH();
 }
 
@@ -1031,7 +1065,7 @@ template <typename VT>
 void launch_copy_il(hipStream_t s, int n, const VT *a, VT *b, int lkp, int nc, const int *act)
 {
     if (n <= 0 || nc <= 0 || a == b) return;
-    k_copy_il<VT><<<il_blocks(n, lkp), BS, 0, s>>>(n, a, b, lkp, nc, act);
+    k_copy_il<VT><<<il_blocks(n, nc), BS, 0, s>>>(n, a, b, lkp, nc, act);
     BCMG_CHECK_LAUNCH();
 }
 
@@ -1040,7 +1074,7 @@ void launch_axpy_il(hipStream_t s, int n, VT alpha, const VT *x, VT *y, int lkp,
                     const int *act)
 {
     if (n <= 0 || nc <= 0) return;
-    k_axpy_il<VT><<<il_blocks(n, lkp), BS, 0, s>>>(n, alpha, x, y, lkp, nc, act);
+    k_axpy_il<VT><<<il_blocks(n, nc), BS, 0, s>>>(n, alpha, x, y, lkp, nc, act);
     BCMG_CHECK_LAUNCH();
 }
 
@@ -1048,7 +1082,7 @@ template <typename VT>
 void launch_scale_il(hipStream_t s, int n, VT alpha, VT *x, int lkp, int nc, const int *act)
 {
     if (n <= 0 || nc <= 0) return;
-    k_scale_il<VT><<<il_blocks(n, lkp), BS, 0, s>>>(n, alpha, x, lkp, nc, act);
+    k_scale_il<VT><<<il_blocks(n, nc), BS, 0, s>>>(n, alpha, x, lkp, nc, act);
     BCMG_CHECK_LAUNCH();
 }
 

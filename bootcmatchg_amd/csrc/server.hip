@@ -3,23 +3,25 @@
  * continuous batching (an extension: the reference solves one right-hand
  * side per call, src/SOLVER/solver.cu:20-129).
  *
- * Every registered composite (operator) gets a lane when it has work: a
- * stream of its own and a slot engine (host_multi.hip) of kmax column slots.
- * A step of the loop
- *   1. waits for the iteration each lane queued in the previous step,
+ * A registered composite (operator) gets a lane while it has work: a
+ * stream of its own and a slot engine (host_multi.hip) of column slots.
+ * A step of the loop visits the lanes in turn and, for each,
+ *   1. waits for the iteration it queued in the previous step,
  *   2. collects the slots whose stopping test ended them and retires the
  *      done ones (x copied to the request's output, the result recorded),
  *   3. admits queued requests into free slots - requests that arrived while
  *      others were on the device join those lanes at an iteration boundary,
  *      and a lane is opened for an operator only while the device memory it
  *      needs fits the budget (idle lanes are closed to make room),
- *   4. queues one iteration (a captured graph) on every lane with active
- *      slots; the lanes' streams run concurrently on the device.
+ *   4. queues its next iteration (a captured graph) at once; the lanes'
+ *      streams run concurrently on the device while the host turns to the
+ *      next lane.
  * A slot computes exactly what a single solve of its right-hand side
  * computes, so a result does not depend on what else is being served.
  */
 #include "bcmg_internal.h"
 
+#include <algorithm>
 #include <chrono>
 #include <deque>
 #include <memory>
@@ -67,12 +69,14 @@ struct ServerResult {
 };
 
 struct Server {
+    /* one lane per operator, max_slots wide; its launches span only the
+       slots in use (host_multi.hip), so a lightly used lane stays cheap */
     Ctx &main;
     size_t budget;
     int max_slots;
     std::vector<Boot *> ops;
     std::vector<std::deque<Request>> queue;
-    std::vector<std::unique_ptr<Lane>> lanes;
+    std::vector<std::vector<std::unique_ptr<Lane>>> lanes; /* per operator */
     std::unordered_map<long long, ServerResult> results;
     std::vector<long long> completed_ids; /* finished since the last drain */
     size_t in_use = 0;
@@ -101,40 +105,56 @@ struct Server {
         return r.id;
     }
 
-    bool lane_idle(const Lane &L) const
+    static bool lane_idle(const Lane &L)
     {
         for (auto &s : L.E->slots)
             if (s.state != SLOT_FREE) return false;
         return true;
     }
 
-    void close_lane(int op)
+    int slots_of(int op) const
     {
-        Lane &L = *lanes[op];
-        in_use -= L.bytes;
-        lanes[op].reset();
+        int k = 0;
+        for (auto &L : lanes[op]) k += L->M->kmax;
+        return k;
     }
 
-    /* open a lane for op with as many slots as the budget allows (closing
-       idle lanes, least recently used first, when it does not fit) */
-    bool open_lane(int op)
+    /* close the least recently used idle lane of another operator (an idle
+       lane of op itself would have free slots); false when none is idle */
+    bool evict_one(int op)
     {
-        Boot &B = *ops[op];
-        const int want = max_slots;
-        auto fits = [&](int k) { return in_use + multi_work_bytes(B, k) <= budget; };
-        while (!fits(1)) {
-            int victim = -1;
-            double oldest = 0.0;
-            for (size_t o = 0; o < lanes.size(); o++) {
-                if ((int)o == op || !lanes[o] || !lane_idle(*lanes[o])) continue;
-                if (victim < 0 || lanes[o]->last_used < oldest) {
-                    victim = (int)o;
-                    oldest = lanes[o]->last_used;
+        int vo = -1;
+        size_t vl = 0;
+        double oldest = 0.0;
+        for (size_t o = 0; o < lanes.size(); o++) {
+            for (size_t l = 0; l < lanes[o].size(); l++) {
+                const Lane &L = *lanes[o][l];
+                if (!lane_idle(L)) continue;
+                if ((int)o == op) continue;
+                if (vo < 0 || L.last_used < oldest) {
+                    vo = (int)o;
+                    vl = l;
+                    oldest = L.last_used;
                 }
             }
-            if (victim < 0) return false;
-            close_lane(victim);
         }
+        if (vo < 0) return false;
+        in_use -= lanes[vo][vl]->bytes;
+        lanes[vo].erase(lanes[vo].begin() + vl);
+        return true;
+    }
+
+    /* open a lane for op (up to its slot limit, within the memory budget,
+       closing idle lanes of other operators, least recently used first,
+       when it does not fit) */
+    Lane *open_lane(int op)
+    {
+        Boot &B = *ops[op];
+        const int want = max_slots - slots_of(op);
+        if (want < 1) return nullptr;
+        auto fits = [&](int k) { return in_use + multi_work_bytes(B, k) <= budget; };
+        while (!fits(1))
+            if (!evict_one(op)) return nullptr;
         int k = want;
         while (k > 1 && !fits(k)) k--;
         /* the pool hands blocks out in the order of one stream: the lane's
@@ -150,92 +170,106 @@ struct Server {
         L->slot_req.assign(k, Request());
         L->bytes = multi_work_bytes(B, k);
         in_use += L->bytes;
-        lanes[op] = std::move(L);
-        return true;
+        lanes[op].push_back(std::move(L));
+        return lanes[op].back().get();
     }
 
+    /* Visit the lanes in turn: a lane's previous iteration is waited for,
+       its stopped slots collected and retired, queued requests admitted into
+       its free slots, and its next iteration queued at once, so that it runs
+       while the host turns to the next lane. Requests left over open new
+       lanes while the operator's slot limit and the budget allow. */
     int step()
     {
         int completed = 0;
-        /* 1. the iterations queued by the previous step */
-        for (auto &L : lanes)
-            if (L && L->E->launched) L->E->wait();
-        /* 2. stopped slots: done or restarted; retire the done ones */
-        for (auto &L : lanes) {
-            if (!L) continue;
-            SlotEngine &E = *L->E;
-            E.collect();
-            bool retired = false;
-            for (size_t j = 0; j < E.slots.size(); j++) {
-                if (E.slots[j].state != SLOT_DONE) continue;
-                const SlotInfo S = E.slots[j];
-                const Request &r = L->slot_req[j];
-                E.retire((int)j, r.x);
-                ServerResult &res = results[r.id];
-                res.iters = S.zero_rhs ? 0 : S.iters;
-                res.resid = S.zero_rhs ? 0.0 : S.l2;
-                res.status = S.status;
-                retired = true;
-            }
-            if (retired) {
-                BCMG_HIP(hipStreamSynchronize(L->c.stream));
-                const double t = now_ms();
-                for (size_t j = 0; j < E.slots.size(); j++) {
-                    Request &r = L->slot_req[j];
-                    if (r.id && E.slots[j].state == SLOT_FREE) {
-                        ServerResult &res = results[r.id];
-                        if (!res.done) {
-                            res.done = true;
-                            res.latency_ms = t - r.t_submit;
-                            completed_ids.push_back(r.id);
-                            completed++;
-                        }
-                        r = Request();
-                    }
-                }
-                L->last_used = t;
-            }
-        }
-        /* 3. admission into free slots */
         for (size_t op = 0; op < ops.size(); op++) {
             auto &q = queue[op];
-            if (q.empty()) continue;
-            if (!lanes[op] && !open_lane((int)op)) continue;
-            Lane &L = *lanes[op];
-            if (L.B->prec != BCMG_PREC_FP64 && L.M->prec != L.B->prec) {
-                /* reduced-precision buffers are allocated now (see open_lane) */
-                BCMG_HIP(hipStreamSynchronize(main.stream));
-                ensure_multi_into(L.c, *L.B, *L.M, L.M->kmax);
+            for (auto &L : lanes[op]) {
+                completed += finish(*L);
+                if (!q.empty()) admit(*L, q);
+                L->E->launch();
             }
-            std::vector<int> js;
-            std::vector<const double *> bs;
-            for (size_t j = 0; j < L.E->slots.size() && !q.empty(); j++) {
-                if (L.E->slots[j].state != SLOT_FREE) continue;
-                L.slot_req[j] = q.front();
-                q.pop_front();
-                js.push_back((int)j);
-                bs.push_back(L.slot_req[j].b);
+            while (!q.empty()) {
+                Lane *L = open_lane((int)op);
+                if (!L) break;
+                admit(*L, q);
+                L->E->launch();
             }
-            if (js.empty()) continue;
-            L.E->admit(js, bs);
-            L.last_used = now_ms();
         }
-        /* zero right-hand sides are done at admission: retired next step */
-        /* 4. one iteration of every lane with active slots */
-        for (auto &L : lanes)
-            if (L) L->E->launch();
         return completed;
+    }
+
+    /* wait, collect, retire (results recorded after the copies of x) */
+    int finish(Lane &L)
+    {
+        SlotEngine &E = *L.E;
+        if (E.launched) E.wait();
+        E.collect();
+        bool retired = false;
+        for (size_t j = 0; j < E.slots.size(); j++) {
+            if (E.slots[j].state != SLOT_DONE) continue;
+            const SlotInfo S = E.slots[j];
+            const Request &r = L.slot_req[j];
+            E.retire((int)j, r.x);
+            ServerResult &res = results[r.id];
+            res.iters = S.zero_rhs ? 0 : S.iters;
+            res.resid = S.zero_rhs ? 0.0 : S.l2;
+            res.status = S.status;
+            retired = true;
+        }
+        if (!retired) return 0;
+        BCMG_HIP(hipStreamSynchronize(L.c.stream));
+        const double t = now_ms();
+        int completed = 0;
+        for (size_t j = 0; j < E.slots.size(); j++) {
+            Request &r = L.slot_req[j];
+            if (r.id && E.slots[j].state == SLOT_FREE) {
+                ServerResult &res = results[r.id];
+                if (!res.done) {
+                    res.done = true;
+                    res.latency_ms = t - r.t_submit;
+                    completed_ids.push_back(r.id);
+                    completed++;
+                }
+                r = Request();
+            }
+        }
+        L.last_used = t;
+        return completed;
+    }
+
+    /* queued requests of the lane's operator into its free slots (zero
+       right-hand sides are done at admission and retired next step) */
+    void admit(Lane &L, std::deque<Request> &q)
+    {
+        if (L.B->prec != BCMG_PREC_FP64 && L.M->prec != L.B->prec) {
+            /* reduced-precision buffers are allocated now (see open_lane) */
+            BCMG_HIP(hipStreamSynchronize(main.stream));
+            ensure_multi_into(L.c, *L.B, *L.M, L.M->kmax);
+        }
+        std::vector<int> js;
+        std::vector<const double *> bs;
+        for (size_t j = 0; j < L.E->slots.size() && !q.empty(); j++) {
+            if (L.E->slots[j].state != SLOT_FREE) continue;
+            L.slot_req[j] = q.front();
+            q.pop_front();
+            js.push_back((int)j);
+            bs.push_back(L.slot_req[j].b);
+        }
+        if (js.empty()) return;
+        L.E->admit(js, bs);
+        L.last_used = now_ms();
     }
 
     void info(int *queued, int *running, size_t *bytes, int *nlanes) const
     {
         int qn = 0, rn = 0, ln = 0;
         for (auto &q : queue) qn += (int)q.size();
-        for (auto &L : lanes) {
-            if (!L) continue;
-            ln++;
-            for (auto &s : L->E->slots) rn += s.state != SLOT_FREE;
-        }
+        for (auto &v : lanes)
+            for (auto &L : v) {
+                ln++;
+                for (auto &s : L->E->slots) rn += s.state != SLOT_FREE;
+            }
         if (queued) *queued = qn;
         if (running) *running = rn;
         if (bytes) *bytes = in_use;

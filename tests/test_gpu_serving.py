@@ -96,18 +96,17 @@ def test_alone_equals_mix_equals_single(ctx, boots):
 
 
 def test_budget_closes_idle_lanes(ctx, boots):
-    """A device-memory budget that holds one operator's slots at a time:
-    lanes open as operators get work and idle ones close to make room; the
-    results do not change."""
+    """A device-memory budget one byte short of every operator's one-slot
+    lane at once: lanes open as operators get work and idle ones close to
+    make room; the results do not change."""
     reqs = requests(boots, per_op=2)
     full, _ = serve(ctx, boots, reqs, [0] * len(reqs), slots=2)
     lane_bytes = []
     for op in range(len(boots)):
-        _, info = serve(ctx, boots, [reqs[2 * op]], [0], slots=2)
+        _, info = serve(ctx, boots, [reqs[2 * op]], [0], slots=1)
         lane_bytes.append(info["bytes"])
-    budget = int(max(lane_bytes) * 1.2)
-    tight, info = serve(ctx, boots, reqs, [0] * len(reqs), slots=2, budget=budget)
-    assert sum(lane_bytes) > budget      # not every operator's lane fits at once
+    budget = sum(lane_bytes) - 1
+    tight, info = serve(ctx, boots, reqs, [0] * len(reqs), slots=1, budget=budget)
     assert info["bytes"] <= budget
     assert info["lanes"] < len(boots)
     for (rf, xf), (rt, xt) in zip(full, tight):
