@@ -43,7 +43,7 @@ SETTINGS = dict(solver_type=0, max_hrc=1, conv_ratio=0.8, aggrsweeps=2, aggrtype
 # x 1.999 + WRITE_SIZE x 1.000, factors from a plain copy kernel of known
 # size; summary: profiles/jacobi_sweep_poisson160_traffic.md). Re-measured
 # whenever the sweep kernel changes; reported only for the 160^3 workload.
-SWEEP_TRAFFIC_BYTES_160 = 502.8e6
+SWEEP_TRAFFIC_BYTES_160 = 509.4e6
 
 
 C2_GRID = 128
