@@ -64,6 +64,9 @@ struct orc_boot_s {
     int coarse_solver = 4;
     std::vector<Hierarchy> H;
     double estimated_ratio = 1.0;
+    /* K-cycle breakdown seen during an application (the reference exits:
+       src/SOLVER/FCG.cu:625-628, 640-643) */
+    mutable int kerr = 0;
 };
 
 namespace {
@@ -462,7 +465,63 @@ const std::vector<double> &smoother_divisor(const Level &lv, int type)
     return type == 4 ? lv.M : lv.D;
 }
 
-/* recursive V/H/W cycle (src/SOLVER/GAMG_cycle.cu:25-140); l is 1-based */
+/* inner-iteration tolerance of the K-cycle (src/SOLVER/GAMG_cycle.cu:5) */
+constexpr double KCYCLE_RTOL = 0.25;
+
+void cycle(const orc_boot &B, const Hierarchy &H, std::vector<std::vector<double>> &rhs,
+           std::vector<std::vector<double>> &x, int l);
+
+/* one application of the cycle from level lvl (1-based) to g with a zero
+   start, in buffers of its own (preconditionApplyK, src/SOLVER/FCG.cu:590-615) */
+std::vector<double> apply_from_level(const orc_boot &B, const Hierarchy &H,
+                                     const std::vector<double> &g, int lvl)
+{
+    std::vector<std::vector<double>> rhs(H.L.size()), x(H.L.size());
+    rhs[lvl - 1] = g;
+    x[lvl - 1].assign(g.size(), 0.0);
+    cycle(B, H, rhs, x, lvl);
+    return x[lvl - 1];
+}
+
+/* K-cycle coarse correction on level m (0-based): at most two flexible CG
+   steps on A_m x = f_m preconditioned by the cycle from level m + 1
+   (inneritkcycle, src/SOLVER/FCG.cu:620-723); pow(t, 2) is t * t */
+void kcycle_inner(const orc_boot &B, const Hierarchy &H, const std::vector<double> &f,
+                  std::vector<double> &x, int m)
+{
+    const Csr &A = H.L[m].A;
+    const int n = A.n;
+    std::vector<double> w = f;
+    const double delta0 = nrm2(n, w.data());
+    const std::vector<double> d0 = apply_from_level(B, H, w, m + 1);
+    const double delta_old = dot(n, w.data(), d0.data());
+    if (!(delta_old > 0.0)) B.kerr = 1;
+    std::vector<double> v(n);
+    spmv(A, d0.data(), v.data());
+    const double tau = dot(n, d0.data(), v.data());
+    if (!(tau > 0.0)) B.kerr = 2;
+    double alpha = delta_old / tau;
+    for (int i = 0; i < n; i++) w[i] = -alpha * v[i] + w[i];
+    const double l2 = nrm2(n, w.data());
+    x.assign(n, 0.0);
+    if (l2 <= KCYCLE_RTOL * delta0) {
+        for (int i = 0; i < n; i++) x[i] = alpha * d0[i] + x[i];
+        return;
+    }
+    const std::vector<double> d1 = apply_from_level(B, H, w, m + 1);
+    std::vector<double> v1(n);
+    spmv(A, d1.data(), v1.data());
+    const double tau1 = dot(n, d1.data(), v.data());
+    const double tau2 = dot(n, d1.data(), v1.data());
+    const double tau3 = dot(n, d1.data(), w.data());
+    const double tau4 = tau2 - (tau1 * tau1) / tau;
+    alpha = alpha - (tau1 * tau3) / (tau * tau4);
+    for (int i = 0; i < n; i++) x[i] = alpha * d0[i] + x[i];
+    const double alpha2 = tau3 / tau4;
+    for (int i = 0; i < n; i++) x[i] = alpha2 * d1[i] + x[i];
+}
+
+/* recursive V/H/W/K cycle (src/SOLVER/GAMG_cycle.cu:25-140); l is 1-based */
 void cycle(const orc_boot &B, const Hierarchy &H, std::vector<std::vector<double>> &rhs,
            std::vector<std::vector<double>> &x, int l)
 {
@@ -487,9 +546,13 @@ void cycle(const orc_boot &B, const Hierarchy &H, std::vector<std::vector<double
         rhs[l][I] = t;
     }
     x[l].assign(nc, 0.0);
-    for (int i = 1; i <= B.ad.num_grid_sweeps[l - 1]; i++) {
-        cycle(B, H, rhs, x, l + 1);
-        if (l == L - 1) break;
+    if (B.ad.cycle_type == 3 && l != L - 1) {
+        kcycle_inner(B, H, rhs[l], x[l], l);
+    } else {
+        for (int i = 1; i <= B.ad.num_grid_sweeps[l - 1]; i++) {
+            cycle(B, H, rhs, x, l + 1);
+            if (l == L - 1) break;
+        }
     }
     /* x_l += P x_{l+1} (src/matrix/CSR.cu:965-976) */
     for (int i = 0; i < n; i++) x[l - 1][i] += lv.p[i] * x[l][lv.agg[i]];
@@ -561,7 +624,10 @@ int check_params(const orc_params &p)
 {
     if (p.solver_type < 0 || p.solver_type > 2) return ORC_ESETTINGS;
     if (p.aggrtype != 0 && p.aggrtype != 1) return ORC_ESETTINGS;
-    if (p.cycle_type < 0 || p.cycle_type > 2) return ORC_ESETTINGS;
+    /* cycle_type 3 (K-cycle) is rejected by the reference's settings parser
+       (src/AMG/AMG.cu:506-508) but coded in its cycle; a parameter block may
+       ask for it */
+    if (p.cycle_type < 0 || p.cycle_type > 3) return ORC_ESETTINGS;
     if (p.coarse_solver != 0 && p.coarse_solver != 4) return ORC_ESETTINGS;
     if (p.relax_type != 0 && p.relax_type != 4) return ORC_ESETTINGS;
     if (p.aggrtype == 1) return ORC_EUNSUP;
@@ -607,6 +673,7 @@ int fcg(const orc_boot &B, const double *rhs, std::vector<double> &x, orc_result
     const int n = A.n;
     const int max_iter = B.p.itnlim;
     const double rtol = B.p.rtol;
+    B.kerr = 0;
     std::vector<double> v(n, 0.0), w(n), r(rhs, rhs + n), d, q;
     spmv(A, x.data(), w.data());
     for (int i = 0; i < n; i++) r[i] = -1.0 * w[i] + r[i];
@@ -663,7 +730,7 @@ int fcg(const orc_boot &B, const double *rhs, std::vector<double> &x, orc_result
         res->iterations = iter + 1;
         res->residual = l2;
     }
-    if (!std::isfinite(l2)) return ORC_EBREAKDOWN;
+    if (!std::isfinite(l2) || B.kerr) return ORC_EBREAKDOWN;
     return ORC_OK;
 }
 

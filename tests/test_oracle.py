@@ -424,3 +424,96 @@ def test_zero_rhs_and_unsupported():
     assert rc == -3
     rc, _, _ = orc.solve(A, np.ones(n), orc.params(aggrtype=1))
     assert rc == -4
+
+
+def dense_kcycle(B, r, pre=2, post=2, coarse=16):
+    """The K-cycle (cycle_type 3) written with dense matrices: the coarse
+    correction of a level whose next level is not the coarsest is the
+    Galerkin projection of A_m x = f_m onto span{d0, d1} - what two flexible
+    CG steps compute - with d0 = B f, w = f - alpha A d0 (alpha = f.d0 /
+    d0.A d0), d1 = B w, B the cycle from the next level; only d0 when
+    ||w|| <= 0.25 ||f|| (src/SOLVER/FCG.cu:620-723)."""
+    nl = B.levels(0)
+    As = [ops.to_scipy(B.level_A(0, l)).toarray() for l in range(nl)]
+    Ps = []
+    for l in range(nl - 1):
+        agg, p = B.level_P(0, l)
+        P = np.zeros((len(agg), As[l + 1].shape[0]))
+        P[np.arange(len(agg)), agg] = p
+        Ps.append(P)
+    Ds = [B.level_DM(0, l)[1] for l in range(nl)]
+
+    def relax(l, f, x, k):
+        for _ in range(k):
+            x = x + (f - As[l] @ x) / Ds[l]
+        return x
+
+    def cyc(l, f, x):
+        if l == nl - 1:
+            return relax(l, f, x, coarse)
+        x = relax(l, f, x, pre)
+        fc = Ps[l].T @ (f - As[l] @ x)
+        if l + 1 == nl - 1:
+            xc = cyc(l + 1, fc, np.zeros(len(fc)))
+        else:
+            m = l + 1
+            d0 = cyc(m, fc, np.zeros(len(fc)))
+            alpha = (fc @ d0) / (d0 @ As[m] @ d0)
+            w = fc - alpha * (As[m] @ d0)
+            if np.linalg.norm(w) <= 0.25 * np.linalg.norm(fc):
+                xc = alpha * d0
+            else:
+                d1 = cyc(m, w, np.zeros(len(fc)))
+                V = np.stack([d0, d1], axis=1)
+                xc = V @ np.linalg.solve(V.T @ As[m] @ V, V.T @ fc)
+        x = x + Ps[l] @ xc
+        return relax(l, f, x, post)
+    return cyc(0, r, np.zeros(len(r)))
+
+
+KCYCLE = dict(prerelax_sweeps=2, postrelax_sweeps=2, relaxnumber_coarse=16, cycle_type=3)
+
+
+def test_kcycle_on_two_levels_is_the_v_cycle():
+    """With two levels the next level is the coarsest: no inner iteration."""
+    A = ops.poisson3d(10)
+    Bk = orc.Boot(A, orc.params(**KCYCLE))
+    Bv = orc.Boot(A, orc.params(**dict(KCYCLE, cycle_type=0)))
+    assert Bk.levels(0) == 2
+    r = ops.uniform_pm1(3, len(A[0]) - 1)
+    assert np.array_equal(bits(Bk.precond(r)), bits(Bv.precond(r)))
+
+
+@pytest.mark.parametrize("name", ["poisson", "lognormal"])
+def test_kcycle_matches_galerkin_projection(name):
+    A = {"poisson": lambda: ops.poisson3d(14),
+         "lognormal": lambda: ops.lognormal3d(12, 4.0, seed=5)}[name]()
+    n = len(A[0]) - 1
+    B = orc.Boot(A, orc.params(**dict(KCYCLE, aggrsweeps=1)))
+    assert B.levels(0) >= 4      # two levels with an inner iteration
+    r = ops.uniform_pm1(6, n)
+    z = B.precond(r)
+    zd = dense_kcycle(B, r)
+    assert np.linalg.norm(z - zd) <= 1e-10 * np.linalg.norm(zd)
+
+
+def test_kcycle_fcg_converges_faster_than_v():
+    A = ops.lognormal3d(14, 4.0, seed=7)
+    n = len(A[0]) - 1
+    b = ops.uniform_pm1(0, n)
+    res = {}
+    for ct in (0, 3):
+        B = orc.Boot(A, orc.params(**dict(KCYCLE, cycle_type=ct, aggrsweeps=1, rtol=1e-8)))
+        rc, x, r, _ = B.fcg(b)
+        assert rc == 0
+        assert np.linalg.norm(b - ops.to_scipy(A) @ x) <= 1.01e-8 * np.linalg.norm(b)
+        res[ct] = r.iterations
+    assert res[3] <= res[0], res
+
+
+def test_kcycle_is_rejected_in_a_settings_file(tmp_path):
+    """The reference's parser accepts cycle types 0..2 only (AMG.cu:506-508)."""
+    f = tmp_path / "s.txt"
+    f.write_text("\n".join(["0", "1", "0.8", "2", "0", "39", "3", "4", "4", "16", "4", "4",
+                            "2000", "1e-8"]) + "\n")
+    assert orc.params_from_file(str(f))[0] == -2          # ORC_ESETTINGS
