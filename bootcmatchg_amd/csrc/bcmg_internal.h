@@ -192,6 +192,8 @@ struct Hierarchy {
 template <typename VT>
 struct CycleBufs {
     std::vector<DBuf<VT>> xa, xb, f;
+    /* K-cycle inner iteration per level: w, d0, d1, A d0, A d1 */
+    std::vector<DBuf<VT>> kw, kd0, kd1, kv, kv1;
     DBuf<VT> res;               /* residual buffer, size n0 */
     DBuf<VT> xsave, xadd;       /* additive composite */
 };
@@ -215,6 +217,7 @@ struct Boot {
     /* FCG vectors */
     DBuf<double> v, w, d, q, r;
     DBuf<double> scal;          /* FCG scalars on device */
+    DBuf<double> kscal;         /* K-cycle scalars, KS_STRIDE per level */
     /* setup timing (ms) */
     double t_setup_ms = 0;
     /* captured FCG iterations (even, odd) and the output vector they use */
@@ -300,6 +303,22 @@ void launch_true_residual(Ctx &c, const DevCsr &A, const double *b, const double
    the scalar recurrence, all on the device */
 void launch_spmv_dots(Ctx &c, const DevCsr &A, int k, const double *z, double *t,
                       const double *r, const double *to, double *scal);
+/* K-cycle inner iteration pieces: products of VT vectors in double into
+   out[0..k); alpha and the breakdown flag; w -= alpha v; the coefficients;
+   x = c0 d0 + c1 d1 (device scalars ks, layout in kernels_solve.hip) */
+constexpr int KS_STRIDE = 16;
+constexpr int KS_FLAG_GLOBAL = 15; /* level 0's flag slot: any level's breakdown */
+template <typename VT>
+void launch_kdots(Ctx &c, int n, int k, const VT *a0, const VT *a1, const VT *a2, const VT *b,
+                  double *out);
+void launch_kcycle_alpha(hipStream_t s, double *ks, double *gflag);
+void launch_kcycle_coef(hipStream_t s, double *ks, double rtol);
+template <typename VT>
+void launch_kcycle_axpy(hipStream_t s, int n, const double *ks, const VT *v, VT *w);
+template <typename VT>
+void launch_kcycle_combine(hipStream_t s, int n, const double *ks, const VT *d0, const VT *d1,
+                           VT *x);
+
 /* doubles of reduction scratch needed for an operator of n rows */
 size_t red_size_for(int n);
 /* all sweeps of a small level in one workgroup (when relax_block_ok(A));

@@ -87,11 +87,14 @@ void check_csr_host(int n, int nnz, const int *row, const int *col)
     }
 }
 
-int params_check(const bcmg_params &p)
+/* from_file: the reference's parser (src/AMG/AMG.cu:480-526) accepts cycle
+   types 0..2; a parameter block may also ask for the K-cycle (3), which the
+   reference's cycle implements (src/SOLVER/GAMG_cycle.cu:86-99) */
+int params_check(const bcmg_params &p, bool from_file = false)
 {
     if (p.solver_type < 0 || p.solver_type > 2) return BCMG_ESETTINGS;
     if (p.aggrtype != 0 && p.aggrtype != 1) return BCMG_ESETTINGS;
-    if (p.cycle_type < 0 || p.cycle_type > 2) return BCMG_ESETTINGS;
+    if (p.cycle_type < 0 || p.cycle_type > (from_file ? 2 : 3)) return BCMG_ESETTINGS;
     if (p.coarse_solver != 0 && p.coarse_solver != 4) return BCMG_ESETTINGS;
     if (p.relax_type != 0 && p.relax_type != 4) return BCMG_ESETTINGS;
     if (p.aggrtype == 1) return BCMG_EUNSUP;
@@ -179,7 +182,7 @@ int bcmg_params_from_file(const char *path, bcmg_params *out)
         set_last_error("settings file: expected 14 lines, one value each");
         return BCMG_ESETTINGS;
     }
-    const int rc = params_check(p);
+    const int rc = params_check(p, true);
     if (rc == BCMG_ESETTINGS) {
         set_last_error("settings file: value outside the allowed set");
         return rc;

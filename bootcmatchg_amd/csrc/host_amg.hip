@@ -209,9 +209,61 @@ void relax_level(Ctx &c, const Level &lv, MatType mt, int type, int sweeps, cons
     }
 }
 
-/* recursive cycle (src/SOLVER/GAMG_cycle.cu:25-140); l is 1-based */
+/* inner-iteration tolerance of the K-cycle (src/SOLVER/GAMG_cycle.cu:5) */
+constexpr double KCYCLE_RTOL = 0.25;
+
 template <typename VT>
-void cycle(CycleRun<VT> &R, int l)
+void cycle(CycleRun<VT> &R, int l, const VT *fin = nullptr);
+
+/* K-cycle coarse correction on level m (0-based, not the coarsest): at most
+   two flexible CG steps on A_m x = f_m preconditioned by the cycle from
+   level m + 1 (inneritkcycle, src/SOLVER/FCG.cu:620-723). Every decision
+   stays on the device (the iteration is captured in a graph): the second
+   preconditioner application is always made, and weighted by 0 when the
+   first step already meets the inner tolerance - the result is the
+   reference's either way. The solution lands in the level's iterate. */
+template <typename VT>
+void kcycle_inner(CycleRun<VT> &R, int m)
+{
+    Ctx &c = R.c;
+    Boot &B = R.B;
+    CycleBufs<VT> &W = bufs<VT>(B);
+    const DevCsr &A = R.H.L[m].mat();
+    const MatType mt = level_mt(B, m);
+    const int n = A.n;
+    double *ks = B.kscal.get() + (size_t)m * KS_STRIDE;
+    VT *w = W.kw[m].get(), *d0 = W.kd0[m].get(), *d1 = W.kd1[m].get();
+    VT *v = W.kv[m].get(), *v1 = W.kv1[m].get();
+    LevelState<VT> &s = R.st[m];
+    launch_copy<VT>(c.stream, n, W.f[m].get(), w);
+    launch_kdots<VT>(c, n, 1, w, nullptr, nullptr, w, ks + 0);
+    auto apply = [&](VT *out) {
+        s.zero = true;
+        cycle<VT>(R, m + 1, w);
+        if (s.zero)
+            launch_fill<VT>(c.stream, n, VT(0), out);
+        else
+            launch_copy<VT>(c.stream, n, s.x, out);
+    };
+    apply(d0);
+    launch_kdots<VT>(c, n, 1, w, nullptr, nullptr, d0, ks + 1);
+    launch_rows<VT>(c.stream, A, mt, OP_SPMV, d0, nullptr, nullptr, v);
+    launch_kdots<VT>(c, n, 1, d0, nullptr, nullptr, v, ks + 2);
+    launch_kcycle_alpha(c.stream, ks, B.kscal.get() + KS_FLAG_GLOBAL);
+    launch_kcycle_axpy<VT>(c.stream, n, ks, v, w);
+    launch_kdots<VT>(c, n, 1, w, nullptr, nullptr, w, ks + 4);
+    apply(d1);
+    launch_rows<VT>(c.stream, A, mt, OP_SPMV, d1, nullptr, nullptr, v1);
+    launch_kdots<VT>(c, n, 3, v, v1, w, d1, ks + 5);
+    launch_kcycle_coef(c.stream, ks, KCYCLE_RTOL);
+    launch_kcycle_combine<VT>(c.stream, n, ks, d0, d1, s.x);
+    s.zero = false;
+}
+
+/* recursive cycle (src/SOLVER/GAMG_cycle.cu:25-140); l is 1-based; fin, when
+   given, is the level's right-hand side (an inner K-cycle application) */
+template <typename VT>
+void cycle(CycleRun<VT> &R, int l, const VT *fin)
 {
     Ctx &c = R.c;
     Boot &B = R.B;
@@ -221,7 +273,7 @@ void cycle(CycleRun<VT> &R, int l)
     const Level &lv = H.L[l - 1];
     const MatType mt = level_mt(B, l - 1);
     LevelState<VT> &s = R.st[l - 1];
-    const VT *f = (l == 1) ? R.f0 : W.f[l - 1].get();
+    const VT *f = fin ? fin : ((l == 1) ? R.f0 : W.f[l - 1].get());
     const bool top = (l == 1);
     if (l == L) {
         relax_level<VT>(c, lv, mt, B.coarse_solver, B.prm.relaxnumber_coarse, f, s,
@@ -240,9 +292,13 @@ void cycle(CycleRun<VT> &R, int l)
                     rsrc, nullptr, nullptr, W.f[l].get());
     LevelState<VT> &sc = R.st[l];
     sc.zero = true;
-    for (int i = 1; i <= B.num_grid_sweeps[l - 1]; i++) {
-        cycle<VT>(R, l + 1);
-        if (l == L - 1) break;
+    if (B.prm.cycle_type == 3 && l != L - 1) {
+        kcycle_inner<VT>(R, l);
+    } else {
+        for (int i = 1; i <= B.num_grid_sweeps[l - 1]; i++) {
+            cycle<VT>(R, l + 1);
+            if (l == L - 1) break;
+        }
     }
     if (!sc.zero) {
         if (s.zero) {
@@ -336,6 +392,17 @@ void ensure_bufs(Boot &B, CycleBufs<VT> &W)
         W.f[l].ensure(nl);
     }
     W.res.ensure(B.H[0].L[0].mat().n);
+    if (B.prm.cycle_type == 3) {
+        for (auto *v : {&W.kw, &W.kd0, &W.kd1, &W.kv, &W.kv1}) {
+            if (v->size() < maxL) v->resize(maxL);
+            for (size_t l = 0; l < maxL; l++) {
+                size_t nl = 0;
+                for (auto &h : B.H)
+                    if (l < h.L.size()) nl = std::max(nl, (size_t)h.L[l].mat().n);
+                (*v)[l].ensure(nl);
+            }
+        }
+    }
 }
 
 } // namespace
@@ -358,6 +425,9 @@ Boot::~Boot()
 
 void Boot::ensure_work()
 {
+    size_t maxL = 0;
+    for (auto &h : H) maxL = std::max(maxL, h.L.size());
+    kscal.ensure((maxL + 1) * KS_STRIDE);
     ensure_bufs<double>(*this, b64);
     if (prec != BCMG_PREC_FP64) {
         ensure_bufs<float>(*this, b32);
@@ -499,6 +569,12 @@ int fcg_solve(Ctx &c, Boot &B, const double *b, double *x, int *iters, double *r
     double *v = B.v.get(), *w = B.w.get(), *d = B.d.get(), *q = B.q.get(), *r = B.r.get();
     /* scratch of the fused reductions; graphs hold its address */
     c.ensure_reductions(n);
+    /* K-cycle breakdown flags (the reference exits on them) start clear */
+    const bool kcyc = B.prm.cycle_type == 3;
+    if (kcyc) {
+        launch_fill<double>(c.stream, (int)B.kscal.n, 0.0, B.kscal.get());
+        c.h_scalars[9] = 0.0;
+    }
     /* x0 = 0: r = b - A 0 = b */
     launch_copy(c.stream, n, b, r);
     launch_fill(c.stream, n, 0.0, x);
@@ -533,6 +609,9 @@ int fcg_solve(Ctx &c, Boot &B, const double *b, double *x, int *iters, double *r
         launch_fcg_update(c, n, scal, zo, z, x, to, t, r);
         BCMG_HIP(hipMemcpyAsync(c.h_scalars + 8, scal + 6, sizeof(double), hipMemcpyDeviceToHost,
                                 c.stream));
+        if (kcyc)
+            BCMG_HIP(hipMemcpyAsync(c.h_scalars + 9, B.kscal.get() + KS_FLAG_GLOBAL,
+                                    sizeof(double), hipMemcpyDeviceToHost, c.stream));
     };
     /* The iteration is the same launch sequence every time (two variants for
        the alternating direction slots), so it is captured once per composite
@@ -574,6 +653,9 @@ int fcg_solve(Ctx &c, Boot &B, const double *b, double *x, int *iters, double *r
             }
             BCMG_HIP(hipStreamSynchronize(c.stream));
             l2 = c.h_scalars[8];
+            if (kcyc && c.h_scalars[9] != 0.0)
+                throw Error(BCMG_EBREAKDOWN, "K-cycle inner iteration: indefinite preconditioner "
+                                             "or matrix (the reference exits, FCG.cu:625-643)");
             iter++;
             phase++;
         } while (l2 > rtol * delta0 && iter < max_iter);

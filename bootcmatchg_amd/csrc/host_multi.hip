@@ -70,6 +70,8 @@ MultiWork &ensure_multi_into(Ctx &c, Boot &B, MultiWork &M, int kmax)
 {
     if (kmax < 1 || kmax > MULTI_MAX_COLS)
         throw Error(BCMG_EINVAL, "multi-column solve: 1..32 columns per block");
+    if (B.prm.cycle_type == 3)
+        throw Error(BCMG_EUNSUP, "multi-column solve: the K-cycle runs on single right-hand sides");
     const int n = B.A0->n;
     const bool grow = kmax > M.kmax;
     if (grow) {

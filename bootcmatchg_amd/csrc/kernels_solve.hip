@@ -370,6 +370,83 @@ __global__ __launch_bounds__(BS) void k_fcg_first(int n, const double *__restric
     block_reduce_store<1>(s, part + blockIdx.x);
 }
 
+/* --------------------------------------------------------- K-cycle
+   Device scalars of one level's inner iteration (src/SOLVER/FCG.cu:620-723),
+   ks = [w.w (delta0^2), w.d0 (delta_old), d0.v (tau), alpha, w'.w' (l2^2),
+         d1.v (tau1), d1.v1 (tau2), d1.w' (tau3), c0, c1, ..., flag]. */
+constexpr int KS_FLAG = 15;
+
+/* up to 3 products of VT vectors with b, accumulated in double (fma over a
+   grid-stride partition, then the fixed block tree) */
+template <int K, typename VT>
+__global__ __launch_bounds__(BS) void k_kdots(int n, const VT *__restrict__ a0,
+                                              const VT *__restrict__ a1,
+                                              const VT *__restrict__ a2,
+                                              const VT *__restrict__ b,
+                                              double *__restrict__ part)
+{
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = 0.0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const double bi = (double)b[i];
+        s[0] = fma((double)a0[i], bi, s[0]);
+        if constexpr (K > 1) s[1] = fma((double)a1[i], bi, s[1]);
+        if constexpr (K > 2) s[2] = fma((double)a2[i], bi, s[2]);
+    }
+    block_reduce_store<K>(s, part + (size_t)blockIdx.x * K);
+}
+
+/* alpha = delta_old / tau; a non-positive delta_old (indefinite
+   preconditioner) or tau (indefinite matrix) raises the flag - the
+   reference exits there */
+__global__ void k_kcycle_alpha(double *ks, double *gflag)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (!(ks[1] > 0.0)) ks[KS_FLAG] = *gflag = 1.0;
+    if (!(ks[2] > 0.0)) ks[KS_FLAG] = *gflag = 2.0;
+    ks[3] = ks[1] / ks[2];
+}
+
+/* w = -alpha v + w */
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_kcycle_axpy(int n, const double *__restrict__ ks,
+                                                    const VT *__restrict__ v,
+                                                    VT *__restrict__ w)
+{
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i < n) w[i] = fma(-(VT)ks[3], v[i], w[i]);
+}
+
+/* the combination coefficients of the inner iteration: one step when
+   ||w'|| <= rtol ||w|| (c1 = 0), else the two-step update */
+__global__ void k_kcycle_coef(double *ks, double rtol)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double delta0 = sqrt(ks[0]), l2 = sqrt(ks[4]);
+    const double alpha = ks[3];
+    if (l2 <= rtol * delta0) {
+        ks[8] = alpha;
+        ks[9] = 0.0;
+        return;
+    }
+    const double tau = ks[2], tau1 = ks[5], tau2 = ks[6], tau3 = ks[7];
+    const double tau4 = tau2 - (tau1 * tau1) / tau;
+    ks[8] = alpha - (tau1 * tau3) / (tau * tau4);
+    ks[9] = tau3 / tau4;
+}
+
+/* x = c0 d0 + c1 d1 (x starts at zero) */
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_kcycle_combine(int n, const double *__restrict__ ks,
+                                                       const VT *__restrict__ d0,
+                                                       const VT *__restrict__ d1,
+                                                       VT *__restrict__ x)
+{
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i < n) x[i] = fma((VT)ks[9], d1[i], fma((VT)ks[8], d0[i], VT(0)));
+}
+
 /* r = b - y (the true residual, y = A x) and its squared norm partials */
 __global__ __launch_bounds__(BS) void k_true_residual(int n, const double *__restrict__ b,
                                                       const double *__restrict__ y,
@@ -704,6 +781,65 @@ void launch_true_residual(Ctx &c, const DevCsr &A, const double *b, const double
     k_fcg_final<1><<<1, RED_FINAL_BS, 0, c.stream>>>(nb, c.red.get() + RED_UPD, scal);
     BCMG_CHECK_LAUNCH();
 }
+
+template <typename VT>
+void launch_kdots(Ctx &c, int n, int k, const VT *a0, const VT *a1, const VT *a2, const VT *b,
+                  double *out)
+{
+    const int nb = red_blocks(n);
+    c.red.ensure((size_t)RED_MAX_BLOCKS * 3);
+    double *part = c.red.get();
+    if (k == 1) {
+        k_kdots<1, VT><<<nb, BS, 0, c.stream>>>(n, a0, a1, a2, b, part);
+        k_reduce_final<1, false><<<1, BS, 0, c.stream>>>(nb, part, out);
+    } else if (k == 2) {
+        k_kdots<2, VT><<<nb, BS, 0, c.stream>>>(n, a0, a1, a2, b, part);
+        k_reduce_final<2, false><<<1, BS, 0, c.stream>>>(nb, part, out);
+    } else {
+        k_kdots<3, VT><<<nb, BS, 0, c.stream>>>(n, a0, a1, a2, b, part);
+        k_reduce_final<3, false><<<1, BS, 0, c.stream>>>(nb, part, out);
+    }
+    BCMG_CHECK_LAUNCH();
+}
+
+void launch_kcycle_alpha(hipStream_t s, double *ks, double *gflag)
+{
+    k_kcycle_alpha<<<1, 64, 0, s>>>(ks, gflag);
+    BCMG_CHECK_LAUNCH();
+}
+
+void launch_kcycle_coef(hipStream_t s, double *ks, double rtol)
+{
+    k_kcycle_coef<<<1, 64, 0, s>>>(ks, rtol);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_kcycle_axpy(hipStream_t s, int n, const double *ks, const VT *v, VT *w)
+{
+    if (n <= 0) return;
+    k_kcycle_axpy<VT><<<blocks_for(n), BS, 0, s>>>(n, ks, v, w);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_kcycle_combine(hipStream_t s, int n, const double *ks, const VT *d0, const VT *d1,
+                           VT *x)
+{
+    if (n <= 0) return;
+    k_kcycle_combine<VT><<<blocks_for(n), BS, 0, s>>>(n, ks, d0, d1, x);
+    BCMG_CHECK_LAUNCH();
+}
+
+#define BCMG_INSTANTIATE_K(VT)                                                               \
+    template void launch_kdots<VT>(Ctx &, int, int, const VT *, const VT *, const VT *,      \
+                                   const VT *, double *);                                    \
+    template void launch_kcycle_axpy<VT>(hipStream_t, int, const double *, const VT *, VT *); \
+    template void launch_kcycle_combine<VT>(hipStream_t, int, const double *, const VT *,    \
+                                            const VT *, VT *);
+BCMG_INSTANTIATE_K(double)
+BCMG_INSTANTIATE_K(float)
+#undef BCMG_INSTANTIATE_K
 
 size_t red_size_for(int n) { return (size_t)RED_SPMV + 3 * (size_t)((n + BS - 1) / BS) + 64; }
 

@@ -39,4 +39,7 @@ SETTINGS = {
     "boot2_add_mixed": dict(solver_type=2, max_hrc=2, conv_ratio=0.5, aggrsweeps=2, cycle_type=1,
                             coarse_solver=0, relax_type=4, relaxnumber_coarse=10,
                             prerelax_sweeps=3, postrelax_sweeps=2, itnlim=2000, rtol=1e-8),
+    "K_L1_1sweep_agg": dict(solver_type=0, max_hrc=1, conv_ratio=0.8, aggrsweeps=1, cycle_type=3,
+                            coarse_solver=4, relax_type=4, relaxnumber_coarse=16,
+                            prerelax_sweeps=2, postrelax_sweeps=2, itnlim=2000, rtol=1e-8),
 }
