@@ -10,7 +10,10 @@
  *   - the Galerkin product sums in a fixed order: AP(i,J) over j ascending in
  *     row i, then A_c(I,J) over the members i of I ascending (the reference's
  *     hash SpGEMM adds with shared-memory atomics in arrival order,
- *     src/EXTERNAL/nsparse-master/cuda-c/src/kernel/kernel_spgemm_hash_d.cu:686-692).
+ *     src/EXTERNAL/nsparse-master/cuda-c/src/kernel/kernel_spgemm_hash_d.cu:686-692);
+ *   - the general sparse product of smoothed aggregation sums every entry
+ *     C(i,j) in the order of the products: the left row's entries in storage
+ *     order, each over the right row's entries in storage order.
  * The matching weight uses the portable bcmg_log (include/bcmg_log.h) instead
  * of CUDA's log so that both sides produce the same weights.
  * Build with -ffp-contract=off: the setup arithmetic must not be fused.
@@ -42,6 +45,11 @@ struct Level {
     std::vector<int> agg;          /* P: fine row -> coarse column */
     std::vector<double> p;         /* P values (one per fine row) */
     Csr R;                         /* R = P^T, columns ascending */
+    /* smoothed aggregation: P = (I - omega D^-1 A) P_tent replaces agg/p in
+       the cycle (agg/p keep the tentative prolongator) */
+    bool smoothed = false;
+    Csr P;
+    double omega = 0.0;
 };
 
 struct Hierarchy {
@@ -329,6 +337,134 @@ Csr galerkin(const Csr &A, const int *agg, const double *p, int nc)
     return Ac;
 }
 
+/* ---------------------------------------------- smoothed aggregation */
+
+/* C = L Rm (CSRm::CSRCSR_product, src/matrix/CSR.cu:1039-1079, a hash
+   SpGEMM). Pattern: the symbolic union of the right rows named by each left
+   row (no cancellation drops an entry), columns ascending. Values:
+   C(i,j) = 0 + sum of L(i,e) Rm(e,j) over the entries e of L's row i in
+   storage order, and within Rm's row e in storage order. */
+Csr spgemm(const Csr &L, const Csr &Rm)
+{
+    Csr C;
+    C.n = L.n;
+    C.m = Rm.m;
+    C.row.assign(L.n + 1, 0);
+    std::vector<int> mark(Rm.m, -1), cols;
+    std::vector<double> acc(Rm.m, 0.0);
+    for (int i = 0; i < L.n; i++) {
+        cols.clear();
+        for (int e = L.row[i]; e < L.row[i + 1]; e++) {
+            const int k = L.col[e];
+            for (int f = Rm.row[k]; f < Rm.row[k + 1]; f++) {
+                const int j = Rm.col[f];
+                const double t = L.val[e] * Rm.val[f];
+                if (mark[j] != i) {
+                    mark[j] = i;
+                    acc[j] = 0.0;
+                    cols.push_back(j);
+                }
+                acc[j] = acc[j] + t;
+            }
+        }
+        std::sort(cols.begin(), cols.end());
+        for (int j : cols) {
+            C.col.push_back(j);
+            C.val.push_back(acc[j]);
+        }
+        C.row[i + 1] = (int)C.col.size();
+    }
+    return C;
+}
+
+/* A^T with the entries of each row in ascending column order
+   (CSRm::T, src/matrix/CSR.cu:272-285) */
+Csr transpose(const Csr &A)
+{
+    Csr T;
+    T.n = A.m;
+    T.m = A.n;
+    T.row.assign(A.m + 1, 0);
+    for (int e = 0; e < A.nnz(); e++) T.row[A.col[e] + 1]++;
+    for (int j = 0; j < A.m; j++) T.row[j + 1] += T.row[j];
+    T.col.resize(A.nnz());
+    T.val.resize(A.nnz());
+    std::vector<int> fill(T.row.begin(), T.row.end() - 1);
+    for (int i = 0; i < A.n; i++)
+        for (int e = A.row[i]; e < A.row[i + 1]; e++) {
+            const int pos = fill[A.col[e]]++;
+            T.col[pos] = i;
+            T.val[pos] = A.val[e];
+        }
+    return T;
+}
+
+/* ||A||_inf = max_i (0 + sum_j |a_ij| in storage order): the body the
+   reference left commented out in CSRm::infinityNorm (src/matrix/CSR.cu:
+   405-416, _row_sum :386-400); the live function asserts (:402-404). */
+double infinity_norm(const Csr &A)
+{
+    double m = 0.0;
+    for (int i = 0; i < A.n; i++) {
+        double s = 0.0;
+        for (int e = A.row[i]; e < A.row[i + 1]; e++) s += std::fabs(A.val[e]);
+        if (s > m) m = s;
+    }
+    return m;
+}
+
+/* P_tent as CSR: one entry per fine row (agg(i), p_i) */
+Csr tentative(int n, int nc, const std::vector<int> &agg, const std::vector<double> &p)
+{
+    Csr P;
+    P.n = n;
+    P.m = nc;
+    P.row.resize(n + 1);
+    for (int i = 0; i <= n; i++) P.row[i] = i;
+    P.col = agg;
+    P.val = p;
+    return P;
+}
+
+void relax_prepare(Level &lv, int relax_type);
+
+/* Smoothed prolongators for the levels built by the unsmoothed coarsening
+   (src/AMG/matchingAggregation.cu:353-390): on each level j,
+     S = D^-1 A_j (each row divided by its diagonal, CSR.cu:520-545),
+     omega = 4 / (3 ||S||_inf),
+     S = -omega S, plus 1 on the diagonal (_applyOmega :64-80),
+     P_j = S P_tent, R_j = P_j^T, A_{j+1} = (R_j A_j) P_j,
+   and the smoother data of level j + 1 is prepared from the new A_{j+1}
+   before level j + 1 is smoothed. The aggregates (and so every level size)
+   are the unsmoothed coarsening's. */
+void smooth_prolongators(const orc_params &p, Hierarchy &H)
+{
+    const int levels = (int)H.L.size();
+    for (int j = 0; j + 1 < levels; j++) {
+        Level &lv = H.L[j];
+        Csr S = lv.A;
+        for (int i = 0; i < S.n; i++)
+            for (int e = S.row[i]; e < S.row[i + 1]; e++) S.val[e] /= lv.D[i];
+        const double omega = 4.0 / (3.0 * infinity_norm(S));
+        for (int i = 0; i < S.n; i++)
+            for (int e = S.row[i]; e < S.row[i + 1]; e++) {
+                S.val[e] = -omega * S.val[e];
+                if (S.col[e] == i) S.val[e] += 1.;
+            }
+        const int nc = H.L[j + 1].A.n;
+        lv.P = spgemm(S, tentative(S.n, nc, lv.agg, lv.p));
+        lv.smoothed = true;
+        lv.omega = omega;
+        lv.R = transpose(lv.P);
+        const Csr T = spgemm(lv.R, lv.A);
+        Level &next = H.L[j + 1];
+        next.A = spgemm(T, lv.P);
+        next.D.clear();
+        next.M.clear();
+        relax_prepare(next, p.relax_type);
+    }
+}
+
 /* ---------------------------------------------------------------- setup */
 
 /* src/AMG/matchingAggregation.cu:18-60 */
@@ -431,6 +567,7 @@ int adaptive_coarsening(const orc_params &p, BuildState &bs, const Csr &A,
         /* the reference reaches finalize_level(H, 0) and asserts */
         return ORC_EINVAL;
     }
+    if (p.aggrtype == 1 && level > 1) smooth_prolongators(p, H);
     if (p.coarse_solver != p.relax_type) relax_prepare(H.L[level - 1], p.coarse_solver);
     double c = 0, cw = 0;
     for (int i = 0; i < level; i++) {
@@ -554,8 +691,18 @@ void cycle(const orc_boot &B, const Hierarchy &H, std::vector<std::vector<double
             if (l == L - 1) break;
         }
     }
-    /* x_l += P x_{l+1} (src/matrix/CSR.cu:965-976) */
-    for (int i = 0; i < n; i++) x[l - 1][i] += lv.p[i] * x[l][lv.agg[i]];
+    if (lv.smoothed) {
+        /* x_l = 1 P x_{l+1} + 1 x_l (src/SOLVER/GAMG_cycle.cu:115-117) */
+        const Csr &P = lv.P;
+        for (int i = 0; i < n; i++) {
+            double t = 0.0;
+            for (int e = P.row[i]; e < P.row[i + 1]; e++) t += P.val[e] * x[l][P.col[e]];
+            x[l - 1][i] = t + x[l - 1][i];
+        }
+    } else {
+        /* x_l += P x_{l+1} (src/matrix/CSR.cu:965-976) */
+        for (int i = 0; i < n; i++) x[l - 1][i] += lv.p[i] * x[l][lv.agg[i]];
+    }
     relax(lv.A, smoother_divisor(lv, B.ad.relax_type), rhs[l - 1].data(), x[l - 1], B.ad.postrelax);
 }
 
@@ -630,7 +777,6 @@ int check_params(const orc_params &p)
     if (p.cycle_type < 0 || p.cycle_type > 3) return ORC_ESETTINGS;
     if (p.coarse_solver != 0 && p.coarse_solver != 4) return ORC_ESETTINGS;
     if (p.relax_type != 0 && p.relax_type != 4) return ORC_ESETTINGS;
-    if (p.aggrtype == 1) return ORC_EUNSUP;
     if (p.max_hrc < 1 || p.aggrsweeps < 1 || p.max_levels < 1) return ORC_EINVAL;
     if (p.prerelax_sweeps < 0 || p.postrelax_sweeps < 0 || p.relaxnumber_coarse < 0)
         return ORC_EINVAL;
@@ -924,6 +1070,56 @@ int orc_level_P(const orc_boot *b, int k, int l, int *agg, double *p)
     if (!lv || lv->agg.empty()) return ORC_EINVAL;
     if (agg) std::copy(lv->agg.begin(), lv->agg.end(), agg);
     if (p) std::copy(lv->p.begin(), lv->p.end(), p);
+    return ORC_OK;
+}
+
+int orc_level_Pinfo(const orc_boot *b, int k, int l, int *nnz, int *smoothed, double *omega)
+{
+    const Level *lv = get_level(b, k, l);
+    if (!lv || lv->agg.empty()) return ORC_EINVAL;
+    if (nnz) *nnz = lv->smoothed ? lv->P.nnz() : (int)lv->agg.size();
+    if (smoothed) *smoothed = lv->smoothed ? 1 : 0;
+    if (omega) *omega = lv->omega;
+    return ORC_OK;
+}
+
+int orc_level_Pcsr(const orc_boot *b, int k, int l, int *row, int *col, double *val)
+{
+    const Level *lv = get_level(b, k, l);
+    if (!lv || lv->agg.empty()) return ORC_EINVAL;
+    const Csr P = lv->smoothed ? lv->P
+                               : tentative((int)lv->agg.size(), lv->R.n, lv->agg, lv->p);
+    if (row) std::copy(P.row.begin(), P.row.end(), row);
+    if (col) std::copy(P.col.begin(), P.col.end(), col);
+    if (val) std::copy(P.val.begin(), P.val.end(), val);
+    return ORC_OK;
+}
+
+int orc_spgemm(int nl, int ml, const int *lrow, const int *lcol, const double *lval, int nr,
+               int mr, const int *rrow, const int *rcol, const double *rval, int *crow,
+               int *ccol, double *cval, int *nnz_c)
+{
+    if (nl < 0 || ml != nr || mr < 0 || !lrow || !rrow || !crow || !nnz_c) return ORC_EINVAL;
+    Csr L, R;
+    L.n = nl;
+    L.m = ml;
+    L.row.assign(lrow, lrow + nl + 1);
+    L.col.assign(lcol, lcol + lrow[nl]);
+    L.val.assign(lval, lval + lrow[nl]);
+    R.n = nr;
+    R.m = mr;
+    R.row.assign(rrow, rrow + nr + 1);
+    R.col.assign(rcol, rcol + rrow[nr]);
+    R.val.assign(rval, rval + rrow[nr]);
+    for (int c : L.col)
+        if (c < 0 || c >= nr) return ORC_EINVAL;
+    for (int c : R.col)
+        if (c < 0 || c >= mr) return ORC_EINVAL;
+    const Csr C = spgemm(L, R);
+    *nnz_c = C.nnz();
+    std::copy(C.row.begin(), C.row.end(), crow);
+    if (ccol) std::copy(C.col.begin(), C.col.end(), ccol);
+    if (cval) std::copy(C.val.begin(), C.val.end(), cval);
     return ORC_OK;
 }
 

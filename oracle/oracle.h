@@ -23,7 +23,7 @@ extern "C" {
 #define ORC_EINVAL -1     /* bad argument / malformed matrix */
 #define ORC_ESETTINGS -2  /* settings file missing, short or out of range */
 #define ORC_EZERORHS -3   /* ||b - A x0|| <= DBL_EPSILON ||b|| (reference exits) */
-#define ORC_EUNSUP -4     /* feature the reference aborts on (aggr_type 1, ...) */
+#define ORC_EUNSUP -4     /* a feature the restatement does not cover */
 #define ORC_EBREAKDOWN -5 /* non-finite residual */
 
 /* Field order of the reference's struct params (src/AMG/AMG.cu:20-37). */
@@ -82,6 +82,12 @@ int orc_galerkin(int n, const int *row, const int *col, const double *val,
                  const int *agg, const double *p, int n_c,
                  int *row_c, int *col_c, double *val_c, int *nnz_c);
 
+/* C = L R (nl x ml times nr x mr, ml == nr) in the fixed order of smoothed
+   aggregation's products. First call with ccol = NULL for crow and nnz_c. */
+int orc_spgemm(int nl, int ml, const int *lrow, const int *lcol, const double *lval,
+               int nr, int mr, const int *rrow, const int *rcol, const double *rval,
+               int *crow, int *ccol, double *cval, int *nnz_c);
+
 /* kernels of the solve phase ------------------------------------------- */
 int orc_spmv(int n, const int *row, const int *col, const double *val,
              const double *x, double *y);
@@ -99,6 +105,10 @@ int orc_level_size(const orc_boot *b, int k, int l, int *n, int *nnz);
 int orc_level_A(const orc_boot *b, int k, int l, int *row, int *col, double *val);
 /* prolongator of level l -> l+1 (composed over the aggregation sweeps) */
 int orc_level_P(const orc_boot *b, int k, int l, int *agg, double *p);
+/* prolongator of level l as CSR: the smoothed one when aggrtype is 1, else
+   the tentative one (one entry per row); omega = 4 / (3 ||D^-1 A||_inf) */
+int orc_level_Pinfo(const orc_boot *b, int k, int l, int *nnz, int *smoothed, double *omega);
+int orc_level_Pcsr(const orc_boot *b, int k, int l, int *row, int *col, double *val);
 /* smoother divisors: D = diag, M = L1 row sums (NULL if not prepared) */
 int orc_level_DM(const orc_boot *b, int k, int l, double *D, double *M, int *hasD, int *hasM);
 /* smooth vector w used to build hierarchy k */

@@ -70,7 +70,7 @@ def lib():
         L.orc_boot_free.restype = None
         for name in ("orc_boot_info", "orc_hier_levels", "orc_level_size", "orc_level_A",
                      "orc_level_P", "orc_level_DM", "orc_boot_w", "orc_precond_apply",
-                     "orc_fcg"):
+                     "orc_fcg", "orc_level_Pinfo", "orc_level_Pcsr", "orc_spgemm"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -168,6 +168,33 @@ def galerkin(A, agg, p, nc):
     return rc_, cc, vc
 
 
+def spgemm(Lm, Rm, m_right):
+    """C = L R for CSR triples (L is n x len(R row)-1, R has m_right columns),
+    in the fixed order of smoothed aggregation's products."""
+    lrow, lcol, lval = [np.ascontiguousarray(a) for a in Lm]
+    rrow, rcol, rval = [np.ascontiguousarray(a) for a in Rm]
+    lrow = lrow.astype(np.int32)
+    lcol = lcol.astype(np.int32)
+    rrow = rrow.astype(np.int32)
+    rcol = rcol.astype(np.int32)
+    lval = lval.astype(np.float64)
+    rval = rval.astype(np.float64)
+    nl = len(lrow) - 1
+    nr = len(rrow) - 1
+    L = lib()
+    crow = np.empty(nl + 1, dtype=np.int32)
+    nnz = ctypes.c_int(0)
+    args = (nl, nr, _p(lrow, _i32p), _p(lcol, _i32p), _p(lval, _f64p), nr, m_right,
+            _p(rrow, _i32p), _p(rcol, _i32p), _p(rval, _f64p), _p(crow, _i32p))
+    rc = L.orc_spgemm(*args, None, None, ctypes.byref(nnz))
+    if rc:
+        raise ValueError("orc_spgemm: %d" % rc)
+    ccol = np.empty(nnz.value, dtype=np.int32)
+    cval = np.empty(nnz.value)
+    assert L.orc_spgemm(*args, _p(ccol, _i32p), _p(cval, _f64p), ctypes.byref(nnz)) == 0
+    return crow, ccol, cval
+
+
 def spmv(A, x):
     row, col, val = _csr(A)
     n = len(row) - 1
@@ -235,6 +262,22 @@ class Boot:
         p = np.empty(n.value)
         assert lib().orc_level_P(self.h, k, l, _p(agg, _i32p), _p(p, _f64p)) == 0
         return agg, p
+
+    def level_Pcsr(self, k, l):
+        """(row, col, val, smoothed, omega) of level l's prolongator."""
+        nnz = ctypes.c_int(0)
+        sm = ctypes.c_int(0)
+        om = ctypes.c_double(0)
+        assert lib().orc_level_Pinfo(self.h, k, l, ctypes.byref(nnz), ctypes.byref(sm),
+                                     ctypes.byref(om)) == 0
+        n = ctypes.c_int(0)
+        assert lib().orc_level_size(self.h, k, l, ctypes.byref(n), None) == 0
+        row = np.empty(n.value + 1, dtype=np.int32)
+        col = np.empty(nnz.value, dtype=np.int32)
+        val = np.empty(nnz.value)
+        assert lib().orc_level_Pcsr(self.h, k, l, _p(row, _i32p), _p(col, _i32p),
+                                    _p(val, _f64p)) == 0
+        return row, col, val, bool(sm.value), om.value
 
     def level_DM(self, k, l):
         n = ctypes.c_int(0)

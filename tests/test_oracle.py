@@ -417,13 +417,11 @@ def test_bootstrap_second_hierarchy_repeats_the_first():
     assert np.array_equal(B.w(1), np.ones(len(A[0]) - 1))
 
 
-def test_zero_rhs_and_unsupported():
+def test_zero_rhs():
     A = ops.poisson3d(4)
     n = len(A[0]) - 1
     rc, _, _ = orc.solve(A, np.zeros(n), orc.params())
     assert rc == -3
-    rc, _, _ = orc.solve(A, np.ones(n), orc.params(aggrtype=1))
-    assert rc == -4
 
 
 def dense_kcycle(B, r, pre=2, post=2, coarse=16):
