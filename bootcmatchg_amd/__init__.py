@@ -291,6 +291,21 @@ class Boot:
                                   p.ctypes.data_as(_f64p)))
         return agg, p
 
+    def level_Pcsr(self, k, l):
+        """(row, col, val, smoothed, omega) of level l's prolongator."""
+        n, _ = self.level_size(k, l)
+        nnz = ctypes.c_int()
+        sm = ctypes.c_int()
+        om = ctypes.c_double()
+        _check(lib().bcmg_level_Pinfo(self.h, k, l, ctypes.byref(nnz), ctypes.byref(sm),
+                                      ctypes.byref(om)))
+        row = np.empty(n + 1, np.int32)
+        col = np.empty(nnz.value, np.int32)
+        val = np.empty(nnz.value)
+        _check(lib().bcmg_level_Pcsr(self.h, k, l, row.ctypes.data_as(_i32p),
+                                     col.ctypes.data_as(_i32p), val.ctypes.data_as(_f64p)))
+        return row, col, val, bool(sm.value), om.value
+
     def level_DM(self, k, l):
         n, _ = self.level_size(k, l)
         D = np.empty(n)
@@ -488,6 +503,23 @@ def time_spmv(ctx, A, x_dev, y_dev, reps):
     ms = ctypes.c_double()
     _check(lib().bcmg_time_spmv(ctx.h, A.h, x_dev.ptr, y_dev.ptr, reps, ctypes.byref(ms)))
     return ms.value
+
+
+def spgemm(ctx, Lm, Rm, m_right, products_per_pass=0):
+    """C = L R for host CSR triples (uploaded here); returns C as a Csr
+    handle (rectangular: n = L's rows, columns 0..m_right-1)."""
+    lrow, lcol, lval = _as(Lm[0], np.int32), _as(Lm[1], np.int32), _as(Lm[2], np.float64)
+    rrow, rcol, rval = _as(Rm[0], np.int32), _as(Rm[1], np.int32), _as(Rm[2], np.float64)
+    nl, ml = len(lrow) - 1, len(rrow) - 1
+    dev = [DeviceArray.from_numpy(ctx, a) for a in (lrow, lcol, lval, rrow, rcol, rval)]
+    h = _vp()
+    try:
+        _check(lib().bcmg_spgemm(ctx.h, nl, ml, m_right, *[d.ptr for d in dev],
+                                 ctypes.c_longlong(products_per_pass), ctypes.byref(h)))
+    finally:
+        for d in dev:
+            d.free()
+    return Csr(ctx, handle=h)
 
 
 def galerkin(ctx, A, agg_dev, p_dev, nc):

@@ -179,6 +179,13 @@ struct Level {
     DBuf<double> p;
     /* R = P^T as CSR (n_next rows) */
     DevCsr R;
+    /* smoothed aggregation: the prolongator (I - omega D^-1 A) P_tent,
+       stored NEGATED so that the residual row operation f - (-P) x gives
+       the prolongation f + P x exactly (negation commutes with rounding);
+       agg/p keep the tentative prolongator */
+    bool smoothed = false;
+    double omega = 0.0;
+    DevCsr Pneg;
 };
 
 struct Hierarchy {
@@ -526,6 +533,30 @@ void transpose_P(Ctx &c, int n, int nc, const int *agg, const double *p, DevCsr 
 void members_from_agg(Ctx &c, int n, const int *agg, int nc, int *members);
 /* exclusive scan of int32 counts into out[0..n], out[n] = total; returns total */
 long long exclusive_scan(Ctx &c, const int *in, int *out, int n, bool read_total = true);
+
+/* smoothed aggregation (kernels_spgemm.hip) */
+/* a CSR operand by pointers: n rows, m columns */
+struct CsrRef {
+    int n = 0, m = 0, nnz = 0;
+    const int *rp = nullptr, *ci = nullptr;
+    const double *v = nullptr;
+};
+inline CsrRef ref_of(const DevCsr &A)
+{
+    return CsrRef{A.n, A.m, A.nnz, A.row.get(), A.col.get(), A.val.get()};
+}
+/* C = L R, symbolic pattern, columns ascending, each entry 0 + its products
+   in the order of L's row then R's row (storage order); rows are taken in
+   batches of at most per_pass products (0: the library's default);
+   right_unique: every row of R has distinct columns */
+void spgemm(Ctx &c, const CsrRef &L, const CsrRef &R, DevCsr &C, long long per_pass = 0,
+            bool right_unique = true);
+/* T = A^T, each row in ascending column order */
+void transpose_csr(Ctx &c, const CsrRef &A, DevCsr &T);
+/* S = I - omega D^-1 A in A's pattern, omega = 4 / (3 ||D^-1 A||_inf) */
+double sa_smoother(Ctx &c, const DevCsr &A, const double *D, DBuf<double> &S);
+void launch_iota(hipStream_t s, int n, int *a);
+void launch_negate(hipStream_t s, long long n, double *v);
 
 /* ------------------------------------------------------------ host algo */
 /* switch a built composite to BCMG_PREC_* (converts the hierarchy copies) */

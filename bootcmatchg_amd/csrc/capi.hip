@@ -97,7 +97,6 @@ int params_check(const bcmg_params &p, bool from_file = false)
     if (p.cycle_type < 0 || p.cycle_type > (from_file ? 2 : 3)) return BCMG_ESETTINGS;
     if (p.coarse_solver != 0 && p.coarse_solver != 4) return BCMG_ESETTINGS;
     if (p.relax_type != 0 && p.relax_type != 4) return BCMG_ESETTINGS;
-    if (p.aggrtype == 1) return BCMG_EUNSUP;
     if (p.max_hrc < 1 || p.aggrsweeps < 1 || p.max_levels < 1 || p.itnlim < 1)
         return BCMG_EINVAL;
     if (p.prerelax_sweeps < 0 || p.postrelax_sweeps < 0 || p.relaxnumber_coarse < 0)
@@ -362,8 +361,7 @@ int bcmg_setup(bcmg_ctx ctx, bcmg_csr A, const bcmg_params *p, bcmg_boot *out)
     *out = nullptr;
     const int rc = params_check(*p);
     if (rc) {
-        set_last_error(rc == BCMG_EUNSUP ? "aggr_type 1 (smoothed aggregation) is not supported"
-                                         : "invalid settings");
+        set_last_error("invalid settings");
         return rc;
     }
     API_BEGIN
@@ -606,6 +604,41 @@ int bcmg_level_P(bcmg_boot B, int k, int l, int *agg, double *p)
     API_END
 }
 
+int bcmg_level_Pinfo(bcmg_boot B, int k, int l, int *nnz, int *smoothed, double *omega)
+{
+    API_BEGIN
+    const Level &lv = get_level(B, k, l);
+    REQUIRE(l + 1 < B->B.H[k].num_levels(), "the coarsest level has no prolongator");
+    if (nnz) *nnz = lv.smoothed ? lv.Pneg.nnz : lv.mat().n;
+    if (smoothed) *smoothed = lv.smoothed ? 1 : 0;
+    if (omega) *omega = lv.omega;
+    API_END
+}
+
+int bcmg_level_Pcsr(bcmg_boot B, int k, int l, int *row, int *col, double *val)
+{
+    API_BEGIN
+    const Level &lv = get_level(B, k, l);
+    REQUIRE(l + 1 < B->B.H[k].num_levels(), "the coarsest level has no prolongator");
+    CtxScope scope_(B->ctx);
+    const int n = lv.mat().n;
+    if (lv.smoothed) {
+        const DevCsr &P = lv.Pneg;
+        d2h(B->ctx, row, P.row.get(), sizeof(int) * (n + 1));
+        d2h(B->ctx, col, P.col.get(), sizeof(int) * P.nnz);
+        if (val) {
+            d2h(B->ctx, val, P.val.get(), sizeof(double) * P.nnz);
+            for (int e = 0; e < P.nnz; e++) val[e] = -val[e];
+        }
+    } else {
+        if (row)
+            for (int i = 0; i <= n; i++) row[i] = i;
+        d2h(B->ctx, col, lv.agg.get(), sizeof(int) * n);
+        d2h(B->ctx, val, lv.p.get(), sizeof(double) * n);
+    }
+    API_END
+}
+
 int bcmg_level_DM(bcmg_boot B, int k, int l, double *D, double *M, int *hasD, int *hasM)
 {
     API_BEGIN
@@ -705,6 +738,56 @@ int bcmg_galerkin(bcmg_ctx ctx, bcmg_csr A, const int *agg, const double *p, int
         throw;
     }
     *Ac = h;
+    API_END
+}
+
+int bcmg_spgemm(bcmg_ctx ctx, int nl, int ml, int mr, const int *lrow, const int *lcol,
+                const double *lval, const int *rrow, const int *rcol, const double *rval,
+                long long products_per_pass, bcmg_csr *C)
+{
+    if (!ctx || nl < 0 || ml < 0 || mr < 0 || !lrow || !rrow || !C) return BCMG_EINVAL;
+    *C = nullptr;
+    API_BEGIN
+    CtxScope scope_(ctx);
+    /* the operands' indices are checked on the host before any kernel reads
+       through them */
+    std::vector<int> hl((size_t)nl + 1), hr((size_t)ml + 1);
+    d2h(ctx, hl.data(), lrow, sizeof(int) * hl.size());
+    d2h(ctx, hr.data(), rrow, sizeof(int) * hr.size());
+    auto check_rows = [](const std::vector<int> &rp) {
+        REQUIRE(rp[0] == 0, "CSR: row pointers must start at 0");
+        for (size_t i = 1; i < rp.size(); i++)
+            REQUIRE(rp[i] >= rp[i - 1], "CSR: row pointers must not decrease");
+    };
+    check_rows(hl);
+    check_rows(hr);
+    const int lnnz = hl[nl], rnnz = hr[ml];
+    REQUIRE(lnnz == 0 || (lcol && lval), "null left operand arrays");
+    REQUIRE(rnnz == 0 || (rcol && rval), "null right operand arrays");
+    std::vector<int> cl(lnnz), cr(rnnz);
+    d2h(ctx, cl.data(), lcol, sizeof(int) * lnnz);
+    d2h(ctx, cr.data(), rcol, sizeof(int) * rnnz);
+    for (int c : cl) REQUIRE(c >= 0 && c < ml, "left operand: column index out of range");
+    for (int c : cr) REQUIRE(c >= 0 && c < mr, "right operand: column index out of range");
+    bool right_unique = true;
+    for (int i = 0; i < ml && right_unique; i++)
+        for (int e = hr[i] + 1; e < hr[i + 1]; e++)
+            if (cr[e] <= cr[e - 1]) {
+                right_unique = false;
+                break;
+            }
+    const CsrRef L{nl, ml, lnnz, lrow, lcol, lval};
+    const CsrRef R{ml, mr, rnnz, rrow, rcol, rval};
+    auto *h = new bcmg_csr_s();
+    try {
+        h->ctx = ctx;
+        spgemm(ctx->c, L, R, h->A, products_per_pass, right_unique);
+        resolve_tiles(ctx->c, {&h->A});
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    *C = h;
     API_END
 }
 

@@ -93,6 +93,44 @@ void matching_aggregation(Ctx &c, const bcmg_params &p, BuildState &bs, const De
     Aout = std::move(cur);
 }
 
+/* Smoothed prolongators for the levels the unsmoothed coarsening built
+   (src/AMG/matchingAggregation.cu:353-390): on level j,
+   S = I - omega D^-1 A_j with omega = 4 / (3 ||D^-1 A_j||_inf),
+   P_j = S P_tent, R_j = P_j^T, A_{j+1} = (R_j A_j) P_j, and level j + 1's
+   smoother data from the new A_{j+1} before level j + 1 is smoothed. The
+   reference's infinity norm asserts (src/matrix/CSR.cu:402-404); this takes
+   the body it left commented out there (largest sum of |s_ij| over a row). */
+void smooth_prolongators(Ctx &c, const bcmg_params &p, Hierarchy &H)
+{
+    const int levels = H.num_levels();
+    for (int j = 0; j + 1 < levels; j++) {
+        Level &lv = H.L[j];
+        Level &next = H.L[j + 1];
+        const DevCsr &A = lv.mat();
+        const int n = A.n, nc = next.mat().n;
+        DBuf<double> S;
+        const double omega = sa_smoother(c, A, lv.D.get(), S);
+        DBuf<int> iota((size_t)n + 1);
+        launch_iota(c.stream, n + 1, iota.get());
+        const CsrRef Sr{n, n, A.nnz, A.row.get(), A.col.get(), S.get()};
+        const CsrRef Pt{n, nc, n, iota.get(), lv.agg.get(), lv.p.get()};
+        spgemm(c, Sr, Pt, lv.Pneg);
+        S.release();
+        iota.release();
+        transpose_csr(c, ref_of(lv.Pneg), lv.R);
+        DevCsr T, Ac;
+        spgemm(c, ref_of(lv.R), ref_of(A), T);
+        spgemm(c, ref_of(T), ref_of(lv.Pneg), Ac);
+        T = DevCsr();
+        launch_negate(c.stream, lv.Pneg.nnz, lv.Pneg.val.get());
+        lv.smoothed = true;
+        lv.omega = omega;
+        next.Aown = std::move(Ac);
+        next.hasD = next.hasM = false;
+        relax_prepare(c, next, next.Aown, p.relax_type);
+    }
+}
+
 /* src/AMG/matchingAggregation.cu:298-423 */
 void adaptive_coarsening(Ctx &c, const bcmg_params &p, BuildState &bs, DevCsr &A,
                          const double *w_in, Hierarchy &H)
@@ -126,9 +164,20 @@ void adaptive_coarsening(Ctx &c, const bcmg_params &p, BuildState &bs, DevCsr &A
         level++;
         if (size_coarse <= (double)bs.ftcoarse * bs.maxcoarsesize) break;
     }
+    std::vector<DevCsr *> pending;
+    if (p.aggrtype == 1 && level > 1) {
+        /* the unsmoothed R tiles are replaced: resolve them first */
+        for (int i = 0; i + 1 < level; i++) pending.push_back(&H.L[i].R);
+        resolve_tiles(c, pending);
+        pending.clear();
+        smooth_prolongators(c, p, H);
+        for (int i = 0; i + 1 < level; i++) {
+            pending.push_back(&H.L[i].Pneg);
+            pending.push_back(&H.L[i + 1].Aown);
+        }
+    }
     if (p.coarse_solver != p.relax_type)
         relax_prepare(c, H.L[level - 1], H.L[level - 1].mat(), p.coarse_solver);
-    std::vector<DevCsr *> pending;
     for (int i = 0; i + 1 < level; i++) pending.push_back(&H.L[i].R);
     resolve_tiles(c, pending);
     double cx = 0, cw = 0;
@@ -305,7 +354,14 @@ void cycle(CycleRun<VT> &R, int l, const VT *fin)
             launch_fill<VT>(c.stream, A.n, VT(0), s.x);
             s.zero = false;
         }
-        launch_prolong<VT>(c.stream, A.n, lv.agg.get(), pvals<VT>(lv), sc.x, s.x);
+        if (lv.smoothed) {
+            /* x = x - (-P) x_c: the stored negated prolongator */
+            launch_rows<VT>(c.stream, lv.Pneg, std::is_same<VT, double>::value ? MT_F64 : MT_F32,
+                            OP_RESID, sc.x, s.x, nullptr, s.x2);
+            std::swap(s.x, s.x2);
+        } else {
+            launch_prolong<VT>(c.stream, A.n, lv.agg.get(), pvals<VT>(lv), sc.x, s.x);
+        }
     }
     relax_level<VT>(c, lv, mt, B.prm.relax_type, B.prm.postrelax_sweeps, f, s,
                     top ? R.final_out : nullptr, top);
@@ -493,6 +549,7 @@ void set_precision(Ctx &c, Boot &B, int prec)
                 lv.p32.ensure(n);
                 launch_d2f(c.stream, n, lv.p.get(), lv.p32.get());
                 lv.R.make_lowp(c.stream, false);
+                if (lv.smoothed) lv.Pneg.make_lowp(c.stream, false);
             }
         }
     }

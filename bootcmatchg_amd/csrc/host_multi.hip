@@ -248,8 +248,16 @@ void cycle_m(MRun<VT> &R, int l)
             launch_fill_il<VT>(c.stream, A.n, VT(0), s.x, lkp, R.nc, R.act);
             s.zero = false;
         }
-        launch_prolong_il<VT>(c.stream, A.n, lv.agg.get(), pvals<VT>(lv), sc.x, s.x, lkp, R.nc,
-                              R.act);
+        if (lv.smoothed) {
+            /* x = x - (-P) x_c: the stored negated prolongator */
+            launch_rows_il<VT>(c.stream, lv.Pneg,
+                               std::is_same<VT, double>::value ? MT_F64 : MT_F32, OP_RESID,
+                               sc.x, s.x, nullptr, s.x2, lkp, R.nc, R.act);
+            std::swap(s.x, s.x2);
+        } else {
+            launch_prolong_il<VT>(c.stream, A.n, lv.agg.get(), pvals<VT>(lv), sc.x, s.x, lkp,
+                                  R.nc, R.act);
+        }
     }
     relax_level_m<VT>(R, lv, mt, B.prm.relax_type, B.prm.postrelax_sweeps, f, s,
                       top ? R.final_out : nullptr);

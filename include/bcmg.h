@@ -28,7 +28,7 @@ extern "C" {
 #define BCMG_EINVAL -1       /* bad argument or malformed matrix */
 #define BCMG_ESETTINGS -2    /* settings file missing, short or out of range */
 #define BCMG_EZERORHS -3     /* ||b - A x0|| <= DBL_EPSILON ||b||: the reference exits (FCG.cu:459-462) */
-#define BCMG_EUNSUP -4       /* a setting the reference aborts on (aggr_type 1: CSR.cu:402-404) */
+#define BCMG_EUNSUP -4       /* a setting or size this library does not run */
 #define BCMG_EBREAKDOWN -5   /* non-finite residual (the reference asserts: FCG.cu:567) */
 #define BCMG_EHIP -6         /* HIP runtime error */
 #define BCMG_ENOMEM -7
@@ -42,7 +42,7 @@ typedef struct {
     double conv_ratio;      /* rho */
     int matchtype;          /* always 3 (AMG.cu:497) */
     int aggrsweeps;
-    int aggrtype;           /* 0 unsmoothed (1 is rejected: BCMG_EUNSUP) */
+    int aggrtype;           /* 0 unsmoothed, 1 smoothed (omega = 4 / (3 ||D^-1 A||_inf)) */
     int max_levels;
     int cycle_type;         /* 0 V, 1 H, 2 W */
     int coarse_solver;      /* 0 Jacobi, 4 L1 */
@@ -179,6 +179,12 @@ int bcmg_level_size(bcmg_boot B, int k, int l, int *n, int *nnz);
 int bcmg_level_A(bcmg_boot B, int k, int l, int *row, int *col, double *val);
 /* prolongator from level l to l+1: coarse index and value per fine row */
 int bcmg_level_P(bcmg_boot B, int k, int l, int *agg, double *p);
+/* prolongator from level l to l+1 as CSR (n_l + 1 row pointers, nnz
+   entries): the smoothed one (I - omega D^-1 A) P_tent when aggrtype is 1
+   (src/AMG/matchingAggregation.cu:353-390), else the tentative one (one
+   entry per row). omega = 4 / (3 ||D^-1 A||_inf); 0 when not smoothed. */
+int bcmg_level_Pinfo(bcmg_boot B, int k, int l, int *nnz, int *smoothed, double *omega);
+int bcmg_level_Pcsr(bcmg_boot B, int k, int l, int *row, int *col, double *val);
 /* smoother divisors: D = diag(A), M = L1 row sums; has* tell which exist */
 int bcmg_level_DM(bcmg_boot B, int k, int l, double *D, double *M, int *hasD, int *hasM);
 
@@ -210,6 +216,17 @@ int bcmg_aggregate(bcmg_ctx ctx, int n, const int *M_sym, const double *w, int *
    (src/AMG/matchingAggregation.cu:241-244). */
 int bcmg_galerkin(bcmg_ctx ctx, bcmg_csr A, const int *agg, const double *p, int n_coarse,
                   bcmg_csr *Ac);
+/* C = L R (nl x ml times ml x mr; device CSR arrays), each entry the sum of
+   its products in the order of L's row, then R's row (storage order), the
+   pattern symbolic with columns ascending. Replaces CSRm::CSRCSR_product
+   (src/matrix/CSR.cu:1069-1079) in smoothed aggregation's three products
+   (src/AMG/matchingAggregation.cu:375,383,387). C may be rectangular: read it
+   with bcmg_csr_info / bcmg_csr_download. Rows are processed in batches of at
+   most products_per_pass products (0: 2^28), which bounds the temporary
+   device memory to about 24 bytes per product of a batch. */
+int bcmg_spgemm(bcmg_ctx ctx, int nl, int ml, int mr, const int *lrow, const int *lcol,
+                const double *lval, const int *rrow, const int *rcol, const double *rval,
+                long long products_per_pass, bcmg_csr *C);
 /* x <- B(r; x): one application of the composite preconditioner with x as
    the initial guess. Replaces preconditionApply (src/SOLVER/FCG.cu:112-187). */
 int bcmg_precond_apply(bcmg_ctx ctx, bcmg_boot B, const double *r, double *x);

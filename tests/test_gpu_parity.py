@@ -226,9 +226,6 @@ def test_solve_host_api_and_errors(ctx):
     with pytest.raises(bc.BcmgError) as e:
         bc.solve(A, prm, np.zeros(n))
     assert e.value.code == bc.EZERORHS
-    with pytest.raises(bc.BcmgError) as e:
-        bc.solve(A, bc.make_params(aggrtype=1), np.ones(n))
-    assert e.value.code == bc.EUNSUP
 
 
 def test_graph_replay_matches_eager_launches(ctx):
