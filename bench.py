@@ -17,7 +17,9 @@ and on the bootstrap composite workload (cell-coefficient diffusion on 128^3,
 k = exp(g), g ~ N(0, 4), max_hrc 3, rho 0.5); the multi-right-hand-side
 form (32 columns through every launch, anisotropic 2048^2, W-cycle); and the
 serving loop (8 cached operators, Poisson arrivals, continuous batching)
-against one FP64 solve at a time on the same arrival trace.
+against one FP64 solve at a time on the same arrival trace; and smoothed
+aggregation (aggr_type 1) against the unsmoothed drop-in on 32^3, 64^3 and
+100^3 Poisson.
 """
 import argparse
 import json
@@ -224,6 +226,56 @@ def serving(bc, ctx, quick=False):
     return out
 
 
+SA_GRIDS = (32, 64, 100)
+
+
+def smoothed(bc, ctx, steps, grids=SA_GRIDS):
+    """Smoothed aggregation (aggr_type 1, omega = 4 / (3 ||D^-1 A||_inf))
+    against the unsmoothed drop-in: 3D 7-point Poisson at each grid, the
+    drop-in settings with rtol 1e-8, b uniform[-1,1] seed 0; full solves
+    (setup + FCG) timed per step."""
+    from bootcmatchg_amd import operators as ops
+    out = {"settings": "0,1,0.8,2,{0|1},39,0,4,4,16,4,4,2000,1e-8"}
+    for N in grids:
+        A = ops.poisson3d(N)
+        n = len(A[0]) - 1
+        b = ops.uniform_pm1(0, n)
+        S = ops.to_scipy(A)
+        dA = bc.Csr(ctx, A)
+        db = bc.DeviceArray.from_numpy(ctx, b)
+        dx = bc.DeviceArray(ctx, n)
+        res = {}
+        xs = {}
+        for agg in (0, 1):
+            prm = bc.make_params(**dict(SETTINGS, aggrtype=agg))
+            setup, solve = [], []
+            for k in range(steps + 1):
+                ctx.sync()
+                t0 = time.perf_counter()
+                B = bc.Boot(ctx, dA, prm)
+                ctx.sync()
+                t1 = time.perf_counter()
+                it, _ = B.fcg(db, dx)
+                t2 = time.perf_counter()
+                lv = [B.level_size(0, l) for l in range(B.levels(0))]
+                B.free()
+                if k > 0:
+                    setup.append((t1 - t0) * 1e3)
+                    solve.append((t2 - t1) * 1e3)
+            x = dx.numpy()
+            xs[agg] = x
+            res["smoothed" if agg else "unsmoothed"] = {
+                "setup_ms": float(np.mean(setup)), "solve_ms": float(np.mean(solve)),
+                "solves_per_s": 1e3 / (float(np.mean(setup)) + float(np.mean(solve))),
+                "iterations": it, "levels": len(lv),
+                "operator_complexity": sum(z[1] for z in lv) / lv[0][1],
+                "true_rel_residual": float(np.linalg.norm(b - S @ x) / np.linalg.norm(b))}
+        res["solution_rel_diff"] = float(np.linalg.norm(xs[1] - xs[0]) / np.linalg.norm(xs[0]))
+        out["poisson3d_%d^3" % N] = res
+        del dA, db, dx
+    return out
+
+
 def sweep_bytes(n, nnz):
     """Algorithmic HBM bytes of one Jacobi/L1 sweep u' = u + (f - A u) / d:
     val + col (12 B/nnz), row pointer, u (gathered, counted once), f, d, u'."""
@@ -251,7 +303,9 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every FCG kernel from the host instead of replaying "
                          "captured HIP graphs (used for kernel-trace profiling)")
-    ap.add_argument("--only", choices=("multi", "serving"), default=None,
+    ap.add_argument("--no-smoothed", action="store_true",
+                    help="skip the smoothed-aggregation measurement")
+    ap.add_argument("--only", choices=("multi", "serving", "smoothed"), default=None,
                     help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -268,6 +322,8 @@ def main():
             bc.use_graphs(ctx, False)
         if args.only == "multi":
             res = multi_rhs(bc, ctx, max(1, args.steps // 2))
+        elif args.only == "smoothed":
+            res = smoothed(bc, ctx, max(1, args.steps // 2))
         else:
             res = serving(bc, ctx, quick=args.steps <= 1)
         print(json.dumps({args.only: res}))
@@ -372,6 +428,7 @@ def main():
 
     multi = None if args.no_multi else multi_rhs(bc, ctx, max(1, args.steps // 2))
     served = None if args.no_serving else serving(bc, ctx)
+    sa = None if args.no_smoothed else smoothed(bc, ctx, max(1, args.steps // 2))
 
     out = {
         "metric": METRIC,
@@ -409,6 +466,7 @@ def main():
         "precision_form": precision,
         "multi_rhs": multi,
         "serving": served,
+        "smoothed_aggregation": sa,
     }
     print(json.dumps(out))
 
