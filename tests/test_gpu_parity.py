@@ -244,3 +244,44 @@ def test_graph_replay_matches_eager_launches(ctx):
         it_g, r_g = B.fcg(b, x_g)
         assert it_g == it_e and r_g == r_e
         assert np.array_equal(bits(x_g.numpy()), bits(x_e.numpy()))
+
+
+def permuted_poisson(N, seed):
+    """3D Poisson with rows and columns permuted at random: no band, entries
+    anywhere in a row."""
+    import scipy.sparse as sp
+    S = ops.to_scipy(ops.poisson3d(N)).tocsr()
+    p = np.random.default_rng(seed).permutation(S.shape[0])
+    T = S[p][:, p].tocsr()
+    T.sort_indices()
+    return (T.indptr.astype(np.int32), T.indices.astype(np.int32), T.data.astype(np.float64))
+
+
+def test_randomly_ordered_operator(ctx):
+    """A randomly permuted operator (columns up to n apart, gathers without
+    locality): row kernels, hierarchy bits and FCG against the oracle."""
+    A = permuted_poisson(34, 5)
+    n = len(A[0]) - 1
+    rows = np.repeat(np.arange(n), np.diff(A[0]))
+    assert np.abs(A[1] - rows).max() > 32767
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, n)
+    f = rng.uniform(-1, 1, n)
+    dA = bc.Csr(ctx, A)
+    dx, df, dy = dev(ctx, x), dev(ctx, f), bc.DeviceArray(ctx, n)
+    bound = RTOL_ROW * (absmv(A, x) + np.abs(f)) + 1e-300
+    bc.spmv(ctx, dA, dx, dy)
+    assert np.all(np.abs(dy.numpy() - orc.spmv(A, x)) <= bound)
+    bc.residual(ctx, dA, dx, df, dy)
+    assert np.all(np.abs(dy.numpy() - (f - orc.spmv(A, x))) <= bound)
+    prm = SETTINGS["V_L1_2sweeps"]
+    O = orc.Boot(A, orc.params(**prm))
+    B = bc.Boot(ctx, dA, bc.make_params(**prm))
+    _compare_hierarchy(B, O, 0)
+    b = ops.uniform_pm1(0, n)
+    rc, xo, res_o, _ = O.fcg(b)
+    assert rc == 0
+    dxs = bc.DeviceArray(ctx, n)
+    it, _ = B.fcg(dev(ctx, b), dxs)
+    assert abs(it - res_o.iterations) <= 2, (it, res_o.iterations)
+    assert np.linalg.norm(dxs.numpy() - xo) <= 1e-5 * np.linalg.norm(xo)
