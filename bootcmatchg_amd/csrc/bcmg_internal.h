@@ -158,6 +158,7 @@ struct Ctx {
     bool use_graphs = true;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
+    std::vector<int> ev_launches;  /* launches between each pair of events */
     hipEvent_t next_event();
     /* grow the reduction scratch for an n-row operator (outside any graph
        capture) */

@@ -813,6 +813,7 @@ int bcmg_time_smoother(bcmg_ctx ctx, int enable)
     if (!ctx) return BCMG_EINVAL;
     ctx->c.time_smoother = enable != 0;
     ctx->c.ev_used = 0;
+    ctx->c.ev_launches.clear();
     return BCMG_OK;
 }
 
@@ -823,14 +824,17 @@ int bcmg_smoother_timing(bcmg_ctx ctx, double *total_ms, int *launches)
     CtxScope scope_(ctx);
     BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
     double tot = 0.0;
+    int count = 0;
     for (size_t i = 0; i + 1 < ctx->c.ev_used; i += 2) {
         float ms = 0.f;
         BCMG_HIP(hipEventElapsedTime(&ms, ctx->c.ev_pool[i], ctx->c.ev_pool[i + 1]));
         tot += ms;
+        count += ctx->c.ev_launches[i / 2];
     }
     if (total_ms) *total_ms = tot;
-    if (launches) *launches = (int)(ctx->c.ev_used / 2);
+    if (launches) *launches = count;
     ctx->c.ev_used = 0;
+    ctx->c.ev_launches.clear();
     API_END
 }
 

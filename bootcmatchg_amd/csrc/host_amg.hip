@@ -237,24 +237,30 @@ void relax_level(Ctx &c, const Level &lv, MatType mt, int type, int sweeps, cons
         s.zero = false;
         return;
     }
+    /* timed: one event pair around the run of consecutive sweep launches */
+    hipEvent_t e1 = nullptr;
+    int timed = 0;
     for (int k = 0; k < sweeps; k++) {
         VT *out = s.x2;
         if (final_out && k == sweeps - 1 && s.x != final_out) out = final_out;
         if (s.zero) {
             launch_jacobi_zero<VT>(c.stream, A.n, f, d, out);
         } else {
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            if (timing) {
-                e0 = c.next_event();
+            if (timing && !timed) {
+                hipEvent_t e0 = c.next_event();
                 e1 = c.next_event();
                 BCMG_HIP(hipEventRecord(e0, c.stream));
             }
             launch_rows<VT>(c.stream, A, mt, OP_JACOBI, s.x, f, d, out);
-            if (timing) BCMG_HIP(hipEventRecord(e1, c.stream));
+            timed++;
         }
         s.x2 = s.x;
         s.x = out;
         s.zero = false;
+    }
+    if (timing && timed) {
+        BCMG_HIP(hipEventRecord(e1, c.stream));
+        c.ev_launches.push_back(timed);
     }
 }
 

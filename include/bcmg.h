@@ -236,9 +236,10 @@ int bcmg_precond_apply(bcmg_ctx ctx, bcmg_boot B, const double *r, double *x);
 int bcmg_use_graphs(bcmg_ctx ctx, int enable);
 
 /* ---- measurement -------------------------------------------------------- */
-/* When enabled, every fine-level (level 0) smoother sweep launched by
-   bcmg_fcg is bracketed by HIP events on the context stream; the totals are
-   read (and reset) with bcmg_smoother_timing. */
+/* When enabled, each run of consecutive fine-level (level 0) smoother sweeps
+   launched by bcmg_fcg is bracketed by one pair of HIP events on the context
+   stream (nothing else runs between them); bcmg_smoother_timing reads (and
+   resets) the summed time and the number of sweeps it covers. */
 int bcmg_time_smoother(bcmg_ctx ctx, int enable);
 int bcmg_smoother_timing(bcmg_ctx ctx, double *total_ms, int *launches);
 /* `reps` back-to-back launches of y = A x on the context stream between two
