@@ -505,7 +505,7 @@ def time_spmv(ctx, A, x_dev, y_dev, reps):
     return ms.value
 
 
-def spgemm(ctx, Lm, Rm, m_right, products_per_pass=0):
+def spgemm(ctx, Lm, Rm, m_right, slots_per_pass=0):
     """C = L R for host CSR triples (uploaded here); returns C as a Csr
     handle (rectangular: n = L's rows, columns 0..m_right-1)."""
     lrow, lcol, lval = _as(Lm[0], np.int32), _as(Lm[1], np.int32), _as(Lm[2], np.float64)
@@ -515,7 +515,7 @@ def spgemm(ctx, Lm, Rm, m_right, products_per_pass=0):
     h = _vp()
     try:
         _check(lib().bcmg_spgemm(ctx.h, nl, ml, m_right, *[d.ptr for d in dev],
-                                 ctypes.c_longlong(products_per_pass), ctypes.byref(h)))
+                                 ctypes.c_longlong(slots_per_pass), ctypes.byref(h)))
     finally:
         for d in dev:
             d.free()

@@ -548,7 +548,8 @@ inline CsrRef ref_of(const DevCsr &A)
 }
 /* C = L R, symbolic pattern, columns ascending, each entry 0 + its products
    in the order of L's row then R's row (storage order); rows are taken in
-   batches of at most per_pass products (0: the library's default);
+   batches of at most per_pass output slots (min(products, columns of C) per
+   row) plus packing weights (0: the library's default);
    right_unique: every row of R has distinct columns */
 void spgemm(Ctx &c, const CsrRef &L, const CsrRef &R, DevCsr &C, long long per_pass = 0,
             bool right_unique = true);

@@ -743,7 +743,7 @@ int bcmg_galerkin(bcmg_ctx ctx, bcmg_csr A, const int *agg, const double *p, int
 
 int bcmg_spgemm(bcmg_ctx ctx, int nl, int ml, int mr, const int *lrow, const int *lcol,
                 const double *lval, const int *rrow, const int *rcol, const double *rval,
-                long long products_per_pass, bcmg_csr *C)
+                long long slots_per_pass, bcmg_csr *C)
 {
     if (!ctx || nl < 0 || ml < 0 || mr < 0 || !lrow || !rrow || !C) return BCMG_EINVAL;
     *C = nullptr;
@@ -781,7 +781,7 @@ int bcmg_spgemm(bcmg_ctx ctx, int nl, int ml, int mr, const int *lrow, const int
     auto *h = new bcmg_csr_s();
     try {
         h->ctx = ctx;
-        spgemm(ctx->c, L, R, h->A, products_per_pass, right_unique);
+        spgemm(ctx->c, L, R, h->A, slots_per_pass, right_unique);
         resolve_tiles(ctx->c, {&h->A});
     } catch (...) {
         delete h;

@@ -44,12 +44,13 @@ constexpr int SG_SORT = 2 * SG_CH;    /* touched columns sorted in LDS, at most 
 constexpr long long SG_DENSE_BYTES = 2ll << 30; /* dense accumulators, all workgroups */
 constexpr int SG_DENSE_BLOCKS = 1024;
 constexpr int SG_SERIAL_MIN = 64;     /* products per left entry for the serial mode */
+constexpr int SG_LDS_COLS = 4096;     /* columns of C for LDS accumulators */
 constexpr int SHORT_ROW = 64;         /* transpose rows ranked by one lane */
 constexpr int RANK_CHUNK = 4096;      /* columns per LDS load when ranking */
 constexpr unsigned long long SG_NONE = ~0ull;
 constexpr int POS_BITS = 11;          /* position field: < 2048 */
 constexpr int ROW_SHIFT = 42;         /* row field above 31 column bits */
-constexpr long long SG_PER_PASS = 1ll << 28; /* products per batch of rows */
+constexpr long long SG_PER_PASS = 1ll << 28; /* slots + weights per batch of rows */
 
 inline int blocks_for(long long w) { return (int)((w + BS - 1) / BS); }
 
@@ -118,23 +119,31 @@ __device__ inline int pow2_at_least(int v)
     return N;
 }
 
-/* products per row, packing weights, the long-row list, the total */
+/* per row: products u, output slots sl = min(products, columns of C) (a
+   row has at most that many entries), packing weight ws, the long-row
+   list; the total of slots plus weights (what a batch of rows is bounded
+   by); *big = 1 if a row has 2^30 products or more */
 __global__ __launch_bounds__(BS) void k_sg_count(int n, const int *__restrict__ Lrp,
                                                  const int *__restrict__ Lci,
-                                                 const int *__restrict__ Rrp,
-                                                 int *__restrict__ u, int *__restrict__ ws,
-                                                 int *__restrict__ cnt, int *__restrict__ large,
+                                                 const int *__restrict__ Rrp, long long m,
+                                                 int *__restrict__ u, int *__restrict__ sl,
+                                                 int *__restrict__ ws, int *__restrict__ cnt,
+                                                 int *__restrict__ large,
                                                  int *__restrict__ nlarge,
-                                                 unsigned long long *__restrict__ total)
+                                                 unsigned long long *__restrict__ total,
+                                                 int *__restrict__ big)
 {
     const int i = blockIdx.x * BS + threadIdx.x;
-    long long s = 0;
+    long long s = 0, slots = 0;
     if (i < n) {
         for (int e = Lrp[i]; e < Lrp[i + 1]; e++) {
             const int k = Lci[e];
             s += Rrp[k + 1] - Rrp[k];
         }
-        u[i] = (int)(s < INT_MAX ? s : INT_MAX);
+        if (s >= (1ll << 30)) atomicOr(big, 1);
+        u[i] = (int)(s < (1ll << 30) ? s : (1ll << 30));
+        slots = s < m ? s : m;
+        sl[i] = (int)slots;
         cnt[i] = 0;
         if (s <= SG_CAP) {
             ws[i] = (int)s + 1;
@@ -143,7 +152,8 @@ __global__ __launch_bounds__(BS) void k_sg_count(int n, const int *__restrict__ 
             large[atomicAdd(nlarge, 1)] = i;
         }
     }
-    unsigned long long t = (unsigned long long)s;
+    /* the batch cost of a row: its slots plus its packing weight */
+    unsigned long long t = (unsigned long long)slots + (i < n && s <= SG_CAP ? s + 1 : 0);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
     if ((threadIdx.x & 63) == 0 && t) atomicAdd(total, t);
@@ -301,7 +311,7 @@ __global__ __launch_bounds__(BS) void k_sg_dense(int nlarge, const int *__restri
                                                  const int *__restrict__ uoff,
                                                  int *__restrict__ tcol, double *__restrict__ tval,
                                                  double *acc_all, int *stamp_all, long long m,
-                                                 int row_base, int serial_min,
+                                                 int row_base, int serial_min, int skip_serial,
                                                  int *__restrict__ cnt)
 {
     __shared__ unsigned long long key[SG_CH];   /* also the sort buffer of SG_SORT ints */
@@ -317,6 +327,10 @@ __global__ __launch_bounds__(BS) void k_sg_dense(int nlarge, const int *__restri
         const int r = large[q];
         const int id = row_base + r;
         const int U = u[r];
+        const int e_end = Lrp[r + 1];
+        int ew0 = Lrp[r], pb0 = 0;   /* first window some later chunk needs */
+        const bool serial = serial_min > 0 && (long long)U >= (long long)serial_min * (e_end - ew0);
+        if (serial && skip_serial) continue;   /* k_sg_serial_lds takes it */
         int *const ocol = tcol + uoff[r];
         double *const oval = tval + uoff[r];
         if (threadIdx.x == 0) {
@@ -324,9 +338,6 @@ __global__ __launch_bounds__(BS) void k_sg_dense(int nlarge, const int *__restri
             cmin = INT_MAX;
             cmax = -1;
         }
-        const int e_end = Lrp[r + 1];
-        int ew0 = Lrp[r], pb0 = 0;   /* first window some later chunk needs */
-        const bool serial = serial_min > 0 && (long long)U >= (long long)serial_min * (e_end - ew0);
         /* long right rows (distinct columns): one left entry at a time, its
            products added column-parallel - the same order per column */
         for (int e0 = ew0; serial && e0 < e_end; e0 += BS) {
@@ -339,12 +350,22 @@ __global__ __launch_bounds__(BS) void k_sg_dense(int nlarge, const int *__restri
             }
             __syncthreads();
             const int ne = min(BS, e_end - e0);
+            /* the lane's first product of the next entry is loaded before
+               the current entry's accumulation, so its latency overlaps */
+            int pf = wf0[0] + threadIdx.x;
+            bool have = pf < ws0[0];
+            int pc = have ? Rci[pf] : 0;
+            double pv = have ? Rv[pf] : 0.0;
             for (int t = 0; t < ne; t++) {
                 const int f1 = ws0[t];
                 const double lv = wlv[t];
-                for (int f = wf0[t] + threadIdx.x; f < f1; f += BS) {
-                    const int col = Rci[f];
-                    const double prod = lv * Rv[f];
+                const int nf = t + 1 < ne ? wf0[t + 1] + threadIdx.x : 0;
+                const bool nhave = t + 1 < ne && nf < ws0[t + 1];
+                const int nc = nhave ? Rci[nf] : 0;
+                const double nv = nhave ? Rv[nf] : 0.0;
+                for (int f = pf; have && f < f1; f += BS) {
+                    const int col = f == pf ? pc : Rci[f];
+                    const double prod = lv * (f == pf ? pv : Rv[f]);
                     const bool first = stamp[col] != id;
                     acc[col] = (first ? 0.0 : acc[col]) + prod;
                     if (first) {
@@ -356,6 +377,10 @@ __global__ __launch_bounds__(BS) void k_sg_dense(int nlarge, const int *__restri
                 }
                 __threadfence_block();
                 __syncthreads();
+                pf = nf;
+                have = nhave;
+                pc = nc;
+                pv = nv;
             }
         }
         for (int c0 = 0; !serial && c0 < U; c0 += SG_CH) {
@@ -456,6 +481,90 @@ __global__ __launch_bounds__(BS) void k_sg_dense(int nlarge, const int *__restri
         }
         if (threadIdx.x == 0) cnt[r] = k;
         __threadfence_block();
+        __syncthreads();
+    }
+}
+
+/* The serial-mode long rows when C has at most SG_LDS_COLS columns: the
+   accumulator and the row marks live in LDS; the row's entries come out in
+   column order from a scan of the marks (no sort). Same order per column as
+   the serial mode of k_sg_dense. */
+__global__ __launch_bounds__(BS) void k_sg_serial_lds(int nlarge, const int *__restrict__ large,
+                                                      const int *__restrict__ Lrp,
+                                                      const int *__restrict__ Lci,
+                                                      const double *__restrict__ Lv,
+                                                      const int *__restrict__ Rrp,
+                                                      const int *__restrict__ Rci,
+                                                      const double *__restrict__ Rv,
+                                                      const int *__restrict__ u,
+                                                      const int *__restrict__ uoff,
+                                                      int *__restrict__ tcol,
+                                                      double *__restrict__ tval, int m,
+                                                      int serial_min, int *__restrict__ cnt)
+{
+    __shared__ double acc[SG_LDS_COLS];
+    __shared__ int mark[SG_LDS_COLS];
+    __shared__ int ws0[BS], wf0[BS];
+    __shared__ double wlv[BS];
+    __shared__ int sh[BS / 64];
+    for (int j = threadIdx.x; j < m; j += BS) mark[j] = -1;
+    __syncthreads();
+    for (int q = blockIdx.x; q < nlarge; q += gridDim.x) {
+        const int r = large[q];
+        const int U = u[r];
+        const int e_end = Lrp[r + 1];
+        const int eb = Lrp[r];
+        if (!((long long)U >= (long long)serial_min * (e_end - eb))) continue;
+        for (int e0 = eb; e0 < e_end; e0 += BS) {
+            const int e = e0 + threadIdx.x;
+            if (e < e_end) {
+                const int k = Lci[e];
+                wf0[threadIdx.x] = Rrp[k];
+                ws0[threadIdx.x] = Rrp[k + 1];
+                wlv[threadIdx.x] = Lv[e];
+            }
+            __syncthreads();
+            const int ne = min(BS, e_end - e0);
+            int pf = wf0[0] + threadIdx.x;
+            bool have = pf < ws0[0];
+            int pc = have ? Rci[pf] : 0;
+            double pv = have ? Rv[pf] : 0.0;
+            for (int t = 0; t < ne; t++) {
+                const int f1 = ws0[t];
+                const double lv = wlv[t];
+                const int nf = t + 1 < ne ? wf0[t + 1] + threadIdx.x : 0;
+                const bool nhave = t + 1 < ne && nf < ws0[t + 1];
+                const int nc = nhave ? Rci[nf] : 0;
+                const double nv = nhave ? Rv[nf] : 0.0;
+                for (int f = pf; have && f < f1; f += BS) {
+                    const int col = f == pf ? pc : Rci[f];
+                    const double prod = lv * (f == pf ? pv : Rv[f]);
+                    const bool first = mark[col] != q;
+                    acc[col] = (first ? 0.0 : acc[col]) + prod;
+                    if (first) mark[col] = q;
+                }
+                __syncthreads();
+                pf = nf;
+                have = nhave;
+                pc = nc;
+                pv = nv;
+            }
+        }
+        int *const ocol = tcol + uoff[r];
+        double *const oval = tval + uoff[r];
+        int base = 0;
+        for (int jb = 0; jb < m; jb += BS) {
+            const int j = jb + threadIdx.x;
+            const int fl = (j < m && mark[j] == q) ? 1 : 0;
+            int tot;
+            const int pos = base + block_scan_excl(fl, sh, &tot);
+            if (fl) {
+                ocol[pos] = j;
+                oval[pos] = acc[j];
+            }
+            base += tot;
+        }
+        if (threadIdx.x == 0) cnt[r] = base;
         __syncthreads();
     }
 }
@@ -665,9 +774,9 @@ void gather_rows(Ctx &c, int n, const int *uoff, const int *rp, const int *tcol,
 
 } // namespace
 
-/* Rows are taken in batches of at most per_pass products (a longer row is a
-   batch of its own), so that the product slots stay bounded and their
-   offsets fit 32 bits; one batch writes C directly. */
+/* Rows are taken in batches of at most per_pass output slots plus packing
+   weights (a row over it is a batch of its own), so that the temporaries
+   stay bounded and their offsets fit 32 bits; one batch writes C directly. */
 void spgemm(Ctx &c, const CsrRef &L, const CsrRef &R, DevCsr &C, long long per_pass,
             bool right_unique)
 {
@@ -686,50 +795,56 @@ void spgemm(Ctx &c, const CsrRef &L, const CsrRef &R, DevCsr &C, long long per_p
         launch_block_nnz(c, C);
         return;
     }
-    DBuf<int> u(n), ws(n), cnt(n), large(n);
-    DBuf<int> nlarge(1);
+    const long long mcols = R.m > 0 ? R.m : 1;
+    DBuf<int> u(n), sl(n), ws(n), cnt(n), large(n);
+    DBuf<int> flags(2);   /* long-row count, row with too many products */
     DBuf<unsigned long long> total(1);
-    BCMG_HIP(hipMemsetAsync(nlarge.get(), 0, sizeof(int), c.stream));
+    BCMG_HIP(hipMemsetAsync(flags.get(), 0, 2 * sizeof(int), c.stream));
     BCMG_HIP(hipMemsetAsync(total.get(), 0, sizeof(unsigned long long), c.stream));
-    k_sg_count<<<blocks_for(n), BS, 0, c.stream>>>(n, L.rp, L.ci, R.rp, u.get(), ws.get(),
-                                                   cnt.get(), large.get(), nlarge.get(),
-                                                   total.get());
+    k_sg_count<<<blocks_for(n), BS, 0, c.stream>>>(n, L.rp, L.ci, R.rp, mcols, u.get(), sl.get(),
+                                                   ws.get(), cnt.get(), large.get(), flags.get(),
+                                                   total.get(), flags.get() + 1);
     BCMG_CHECK_LAUNCH();
     unsigned long long h_total = 0;
-    int h_nlarge = 0;
+    int h_flags[2] = {0, 0};
     BCMG_HIP(hipMemcpyAsync(&h_total, total.get(), sizeof(h_total), hipMemcpyDeviceToHost,
                             c.stream));
-    BCMG_HIP(hipMemcpyAsync(&h_nlarge, nlarge.get(), sizeof(int), hipMemcpyDeviceToHost,
+    BCMG_HIP(hipMemcpyAsync(h_flags, flags.get(), 2 * sizeof(int), hipMemcpyDeviceToHost,
                             c.stream));
     BCMG_HIP(hipStreamSynchronize(c.stream));
-    /* batches of rows [lo, hi) and, per batch, its long rows (local ids) */
+    const int h_nlarge = h_flags[0];
+    if (h_flags[1]) throw Error(BCMG_EUNSUP, "sparse product: a row with 2^30 products or more");
+    /* batches of rows [lo, hi) by output slots and, per batch, its long rows
+       (local ids) */
     std::vector<int> blo, bhi;
     std::vector<long long> bprod;
     std::vector<std::vector<int>> blarge;
     if ((long long)h_total <= per_pass) {
         blo.push_back(0);
         bhi.push_back(n);
-        bprod.push_back((long long)h_total);
+        bprod.push_back((long long)h_total);   /* bounds the slots from above */
     } else {
-        std::vector<int> hu(n);
-        BCMG_HIP(hipMemcpy(hu.data(), u.get(), sizeof(int) * n, hipMemcpyDeviceToHost));
-        long long acc = 0;
+        std::vector<int> hu(n), hw(n);
+        BCMG_HIP(hipMemcpy(hu.data(), sl.get(), sizeof(int) * n, hipMemcpyDeviceToHost));
+        BCMG_HIP(hipMemcpy(hw.data(), ws.get(), sizeof(int) * n, hipMemcpyDeviceToHost));
+        long long acc = 0, slots = 0;
         int lo = 0;
         for (int i = 0; i < n; i++) {
-            if (hu[i] >= INT_MAX / 2)
-                throw Error(BCMG_EUNSUP, "sparse product: a row with more than 2^30 products");
-            if (i > lo && acc + hu[i] > per_pass) {
+            const long long cost = (long long)hu[i] + hw[i];
+            if (i > lo && acc + cost > per_pass) {
                 blo.push_back(lo);
                 bhi.push_back(i);
-                bprod.push_back(acc);
+                bprod.push_back(slots);
                 lo = i;
                 acc = 0;
+                slots = 0;
             }
-            acc += hu[i];
+            acc += cost;
+            slots += hu[i];
         }
         blo.push_back(lo);
         bhi.push_back(n);
-        bprod.push_back(acc);
+        bprod.push_back(slots);
         std::vector<int> hl(h_nlarge);
         if (h_nlarge > 0)
             BCMG_HIP(hipMemcpy(hl.data(), large.get(), sizeof(int) * h_nlarge,
@@ -753,14 +868,13 @@ void spgemm(Ctx &c, const CsrRef &L, const CsrRef &R, DevCsr &C, long long per_p
     std::vector<DBuf<double>> oval(nbatch > 1 ? nbatch : 0);
     std::vector<long long> onnz(nbatch, 0);
     if (nbatch > 1) brp.alloc((size_t)maxrows + 1);
-    const long long mcols = R.m > 0 ? R.m : 1;
     DBuf<double> acc;
     DBuf<int> stamp;
     int dblocks = 0;
     for (size_t b = 0; b < nbatch; b++) {
         const int lo = blo[b], nr = bhi[b] - blo[b];
         const int *Lrp = L.rp + lo;
-        exclusive_scan(c, u.get() + lo, uoff.get(), nr, false);
+        exclusive_scan(c, sl.get() + lo, uoff.get(), nr, false);
         const long long S = exclusive_scan(c, ws.get() + lo, soff.get(), nr, true);
         const int nb = (int)((S + SG_CAP - 1) / SG_CAP);
         if (nb > 0)
@@ -795,11 +909,16 @@ void spgemm(Ctx &c, const CsrRef &L, const CsrRef &R, DevCsr &C, long long per_p
                                         c.stream));
             }
             const int g = std::min(nl, dblocks);
+            const int smin = right_unique ? SG_SERIAL_MIN : 0;
+            const bool lds = smin > 0 && mcols <= SG_LDS_COLS;
+            if (lds)
+                k_sg_serial_lds<<<std::min(nl, SG_DENSE_BLOCKS), BS, 0, c.stream>>>(
+                    nl, lrows, Lrp, L.ci, L.v, R.rp, R.ci, R.v, u.get() + lo, uoff.get(),
+                    tcol.get(), tval.get(), (int)mcols, smin, cnt.get() + lo);
             k_sg_dense<<<g, BS, 0, c.stream>>>(nl, lrows, Lrp, L.ci, L.v, R.rp, R.ci, R.v,
                                                u.get() + lo, uoff.get(), tcol.get(), tval.get(),
-                                               acc.get(), stamp.get(), mcols, lo,
-                                               right_unique ? SG_SERIAL_MIN : 0,
-                                               cnt.get() + lo);
+                                               acc.get(), stamp.get(), mcols, lo, smin,
+                                               lds ? 1 : 0, cnt.get() + lo);
             BCMG_CHECK_LAUNCH();
         }
         if (nbatch == 1) {

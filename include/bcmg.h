@@ -221,12 +221,13 @@ int bcmg_galerkin(bcmg_ctx ctx, bcmg_csr A, const int *agg, const double *p, int
    pattern symbolic with columns ascending. Replaces CSRm::CSRCSR_product
    (src/matrix/CSR.cu:1069-1079) in smoothed aggregation's three products
    (src/AMG/matchingAggregation.cu:375,383,387). C may be rectangular: read it
-   with bcmg_csr_info / bcmg_csr_download. Rows are processed in batches of at
-   most products_per_pass products (0: 2^28), which bounds the temporary
-   device memory to about 24 bytes per product of a batch. */
+   with bcmg_csr_info / bcmg_csr_download. Rows are processed in batches whose
+   temporary slots (per row: min(its products, mr) plus a packing weight of
+   products + 1 for rows of at most 1024 products) stay within slots_per_pass
+   (0: 2^28); a slot takes 12 bytes of device memory. */
 int bcmg_spgemm(bcmg_ctx ctx, int nl, int ml, int mr, const int *lrow, const int *lcol,
                 const double *lval, const int *rrow, const int *rcol, const double *rval,
-                long long products_per_pass, bcmg_csr *C);
+                long long slots_per_pass, bcmg_csr *C);
 /* x <- B(r; x): one application of the composite preconditioner with x as
    the initial guess. Replaces preconditionApply (src/SOLVER/FCG.cu:112-187). */
 int bcmg_precond_apply(bcmg_ctx ctx, bcmg_boot B, const double *r, double *x);

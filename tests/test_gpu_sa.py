@@ -91,12 +91,12 @@ def test_spgemm_bitexact(ctx, name):
 @pytest.mark.parametrize("name,per_pass", [("mixed", 5000), ("short_rows", 700),
                                            ("chunk_spanning", 1)])
 def test_spgemm_row_batches(ctx, name, per_pass):
-    """Rows taken in batches of a few thousand products (long rows alone)
-    give the same bits as one pass."""
+    """Rows taken in batches of a few thousand slots (long rows alone) give
+    the same bits as one pass."""
     L, R, m = CASES[name]
     Lt, Rt = triple(L), triple(R)
     print("spgemm batches", name, per_pass, flush=True)
-    row, col, val = bc.spgemm(ctx, Lt, Rt, m, products_per_pass=per_pass).download()
+    row, col, val = bc.spgemm(ctx, Lt, Rt, m, slots_per_pass=per_pass).download()
     orow, ocol, oval = orc.spgemm(Lt, Rt, m)
     assert np.array_equal(row, orow)
     assert np.array_equal(col, ocol)
