@@ -18,8 +18,8 @@ k = exp(g), g ~ N(0, 4), max_hrc 3, rho 0.5); the multi-right-hand-side
 form (32 columns through every launch, anisotropic 2048^2, W-cycle); and the
 serving loop (8 cached operators, Poisson arrivals, continuous batching)
 against one FP64 solve at a time on the same arrival trace; and smoothed
-aggregation (aggr_type 1) against the unsmoothed drop-in on 32^3, 64^3 and
-100^3 Poisson.
+aggregation (aggr_type 1) against the unsmoothed drop-in on 32^3, 64^3,
+100^3 and 160^3 Poisson.
 """
 import argparse
 import json
@@ -226,7 +226,7 @@ def serving(bc, ctx, quick=False):
     return out
 
 
-SA_GRIDS = (32, 64, 100)
+SA_GRIDS = (32, 64, 100, 160)
 
 
 def smoothed(bc, ctx, steps, grids=SA_GRIDS):
