@@ -120,12 +120,19 @@ def _as(a, dtype):
 
 
 class Context:
-    """One GPU and one HIP stream (replaces Handles, src/utility/handles.cu)."""
+    """One GPU and one HIP stream (replaces Handles, src/utility/handles.cu).
+    stream: a hipStream_t (an int address, e.g. torch's
+    `torch.cuda.Stream().cuda_stream`) the caller created on `device`; the
+    context then enqueues all its work there and leaves the stream alive."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, stream=None):
         self.h = _vp()
         self._kids = []
-        _check(lib().bcmg_ctx_create(device, ctypes.byref(self.h)))
+        if stream is None:
+            _check(lib().bcmg_ctx_create(device, ctypes.byref(self.h)))
+        else:
+            _check(lib().bcmg_ctx_create_on_stream(device, _vp(int(stream)),
+                                                   ctypes.byref(self.h)))
 
     def _adopt(self, obj):
         """Track an object that lives in this context: close() frees the

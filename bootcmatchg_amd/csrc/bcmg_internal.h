@@ -147,6 +147,7 @@ struct Ctx {
     int device = 0;
     Pool pool;
     hipStream_t stream = nullptr;
+    bool own_stream = true;     /* false: the caller's stream, not destroyed here */
     /* pinned host scalars used for the few device->host reads */
     double *h_scalars = nullptr;
     int *h_ints = nullptr;

@@ -190,7 +190,9 @@ int bcmg_params_from_file(const char *path, bcmg_params *out)
     return BCMG_OK;
 }
 
-int bcmg_ctx_create(int device, bcmg_ctx *out)
+namespace {
+
+int ctx_create(int device, hipStream_t ext, bcmg_ctx *out)
 {
     if (!out) return BCMG_EINVAL;
     *out = nullptr;
@@ -207,7 +209,15 @@ int bcmg_ctx_create(int device, bcmg_ctx *out)
     try {
         ctx->c.device = device;
         BCMG_HIP(hipSetDevice(device));
-        BCMG_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+        if (ext) {
+            int sdev = -1;
+            BCMG_HIP(hipStreamGetDevice(ext, &sdev));
+            if (sdev != device) throw Error(BCMG_EINVAL, "the stream belongs to another device");
+            ctx->c.stream = ext;
+            ctx->c.own_stream = false;
+        } else {
+            BCMG_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+        }
         BCMG_HIP(hipHostMalloc((void **)&ctx->c.h_scalars, 64 * sizeof(double), hipHostMallocDefault));
         BCMG_HIP(hipHostMalloc((void **)&ctx->c.h_ints, 256 * sizeof(int), hipHostMallocDefault));
         ctx->c.ensure_reductions(1);
@@ -218,6 +228,19 @@ int bcmg_ctx_create(int device, bcmg_ctx *out)
     }
     *out = ctx;
     API_END
+}
+
+} // namespace
+
+int bcmg_ctx_create(int device, bcmg_ctx *out) { return ctx_create(device, nullptr, out); }
+
+int bcmg_ctx_create_on_stream(int device, void *stream, bcmg_ctx *out)
+{
+    if (!stream) {
+        set_last_error("a created stream is required (not the null stream)");
+        return BCMG_EINVAL;
+    }
+    return ctx_create(device, static_cast<hipStream_t>(stream), out);
 }
 
 int bcmg_ctx_destroy(bcmg_ctx ctx)

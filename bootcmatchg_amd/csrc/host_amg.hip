@@ -773,7 +773,7 @@ Ctx::~Ctx()
     if (h_ints) (void)hipHostFree(h_ints);
     red.release();
     iscratch.release();
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 
 } // namespace bcmg

@@ -68,6 +68,12 @@ int bcmg_params_from_file(const char *path, bcmg_params *out);
 /* Context on GPU `device`. Replaces Handles::init / Handles::free
    (src/utility/handles.cu:12-48): one HIP stream, no library handles. */
 int bcmg_ctx_create(int device, bcmg_ctx *out);
+/* A context whose work is ordered on a stream the caller created (a
+   hipStream_t, passed as a pointer; not the null stream) on `device`: every
+   call on it enqueues there, so the library composes with the caller's own
+   work and dependencies; bcmg_ctx_destroy leaves the stream alive. The
+   reference runs on the default stream (src/utility/handles.cu:12-48). */
+int bcmg_ctx_create_on_stream(int device, void *stream, bcmg_ctx *out);
 int bcmg_ctx_destroy(bcmg_ctx ctx);
 int bcmg_sync(bcmg_ctx ctx);
 

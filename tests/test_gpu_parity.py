@@ -285,3 +285,32 @@ def test_randomly_ordered_operator(ctx):
     it, _ = B.fcg(dev(ctx, b), dxs)
     assert abs(it - res_o.iterations) <= 2, (it, res_o.iterations)
     assert np.linalg.norm(dxs.numpy() - xo) <= 1e-5 * np.linalg.norm(xo)
+
+
+def test_context_on_a_caller_stream(ctx):
+    """A context on a stream the caller created gives the same bits as one on
+    its own stream, and leaves the stream alive when it closes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    s = ctypes.c_void_p()
+    assert hip.hipStreamCreateWithFlags(ctypes.byref(s), ctypes.c_uint(1)) == 0
+    try:
+        A = OPERATORS["lognormal3d_10"]()
+        n = len(A[0]) - 1
+        b = ops.uniform_pm1(8, n)
+        prm = bc.make_params(**SETTINGS["V_L1_2sweeps"])
+        B0 = bc.Boot(ctx, bc.Csr(ctx, A), prm)
+        x0 = bc.DeviceArray(ctx, n)
+        it0, r0 = B0.fcg(dev(ctx, b), x0)
+        c2 = bc.Context(0, stream=s.value)
+        B = bc.Boot(c2, bc.Csr(c2, A), prm)
+        x = bc.DeviceArray(c2, n)
+        it, r = B.fcg(bc.DeviceArray.from_numpy(c2, b), x)
+        assert it == it0 and r == r0
+        assert np.array_equal(bits(x.numpy()), bits(x0.numpy()))
+        c2.close()
+        assert hip.hipStreamSynchronize(s) == 0
+        with pytest.raises(bc.BcmgError) as e:
+            bc.Context(0, stream=0)
+    finally:
+        hip.hipStreamDestroy(s)
