@@ -70,7 +70,7 @@ def test_settings_missing_and_short(tmp_path):
     assert orc.params_from_file(path)[0] == -2
 
 
-def test_smoothed_aggregation_is_parsed_then_rejected_at_setup(tmp_path):
+def test_smoothed_aggregation_is_parsed(tmp_path):
     vals = ["0", "1", "0.8", "2", "1", "39", "0", "4", "4", "16", "4", "4", "2000", "1e-6"]
     path = str(tmp_path / "s")
     write_settings(path, vals)
@@ -142,3 +142,10 @@ def test_extension_entry_points_validate_arguments():
     assert L.bcmg_server_completed(vp(), None, 0, ctypes.byref(i)) == bc.EINVAL
     assert L.bcmg_server_info(vp(), None, None, None, None) == bc.EINVAL
     assert L.bcmg_server_destroy(vp()) == bc.OK
+    # context on a caller's stream: the null stream is refused
+    assert L.bcmg_ctx_create_on_stream(0, vp(), ctypes.byref(vp())) == bc.EINVAL
+    # smoothed aggregation's product and prolongator accessors
+    assert L.bcmg_spgemm(vp(), 1, 1, 1, vp(), vp(), vp(), vp(), vp(), vp(),
+                         ctypes.c_longlong(0), ctypes.byref(vp())) == bc.EINVAL
+    assert L.bcmg_level_Pinfo(vp(), 0, 0, None, None, None) == bc.EINVAL
+    assert L.bcmg_level_Pcsr(vp(), 0, 0, None, None, None) == bc.EINVAL
