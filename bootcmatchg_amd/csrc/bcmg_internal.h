@@ -408,6 +408,7 @@ struct MultiFcg {
     int *act = nullptr, *par = nullptr, *it = nullptr;
     double *tol = nullptr;
     int maxit = 0;
+    double *kflag = nullptr;   /* K-cycle breakdown flag of column j at [j KS_STRIDE] */
 };
 /* work space of the multi-column solve on one composite: kmax column slots
    of every cycle and FCG vector, per-slot FCG state, pinned staging */
@@ -425,6 +426,7 @@ struct MultiWork {
     DBuf<double> vec;          /* V0 V1 W0 W1 Z R X B Y */
     DBuf<double> scal, tol, part;
     DBuf<int> flags;           /* act, par, it, mask: kmax each */
+    DBuf<double> kscal;        /* K-cycle scalars: level l, column j at (l kmax + j) KS_STRIDE */
     MultiFcg F;
     int *mask = nullptr;       /* device mask of admission / check steps */
     /* pinned staging */
@@ -448,6 +450,22 @@ struct MultiWork {
         if (h_flags) (void)hipHostFree(h_flags);
     }
 };
+
+/* the K-cycle inner iteration on interleaved blocks: per column the
+   arithmetic of launch_kdots / launch_kcycle_* (ks: column j at
+   ks + j KS_STRIDE) */
+template <typename VT>
+void launch_kdots_il(hipStream_t s, int n, int k, const VT *a0, const VT *a1, const VT *a2,
+                     const VT *b, int lkp, int nc, const int *act, double *part, double *ks,
+                     int at);
+void launch_kcycle_alpha_il(hipStream_t s, double *ks, double *gflag, int nc, const int *act);
+void launch_kcycle_coef_il(hipStream_t s, double *ks, double rtol, int nc, const int *act);
+template <typename VT>
+void launch_kcycle_axpy_il(hipStream_t s, int n, const double *ks, const VT *v, VT *w, int lkp,
+                           int nc, const int *act);
+template <typename VT>
+void launch_kcycle_combine_il(hipStream_t s, int n, const double *ks, const VT *d0, const VT *d1,
+                              VT *x, int lkp, int nc, const int *act);
 
 MultiWork &ensure_multi_into(Ctx &c, Boot &B, MultiWork &M, int kmax);
 MultiWork &multi_work(Ctx &c, Boot &B, MultiWork *&holder, int kmax);

@@ -54,6 +54,12 @@ void ensure_mbufs(const Boot &B, const MultiWork &M, CycleBufs<VT> &W)
         W.xadd.ensure(s0);
         W.xsave.ensure(s0);
     }
+    if (B.prm.cycle_type == 3) {
+        for (auto *v : {&W.kw, &W.kd0, &W.kd1, &W.kv, &W.kv1}) {
+            if (v->size() < L) v->resize(L);
+            for (size_t l = 0; l < L; l++) (*v)[l].ensure((size_t)M.nl[l] << M.lkp);
+        }
+    }
 }
 
 inline int log2_cols(int k)
@@ -70,8 +76,6 @@ MultiWork &ensure_multi_into(Ctx &c, Boot &B, MultiWork &M, int kmax)
 {
     if (kmax < 1 || kmax > MULTI_MAX_COLS)
         throw Error(BCMG_EINVAL, "multi-column solve: 1..32 columns per block");
-    if (B.prm.cycle_type == 3)
-        throw Error(BCMG_EUNSUP, "multi-column solve: the K-cycle runs on single right-hand sides");
     const int n = B.A0->n;
     const bool grow = kmax > M.kmax;
     if (grow) {
@@ -132,6 +136,18 @@ MultiWork &ensure_multi_into(Ctx &c, Boot &B, MultiWork &M, int kmax)
         M.r64.ensure((size_t)M.nl[0] << M.lkp);
         M.z64.ensure((size_t)M.nl[0] << M.lkp);
         M.prec64 = true;
+    }
+    if (B.prm.cycle_type == 3) {
+        /* K-cycle scalars per level and column; level 0's flag slot of a
+           column is its breakdown flag (cleared at admission) */
+        const size_t need = M.nl.size() * (size_t)M.kmax * KS_STRIDE;
+        if (M.kscal.n < need || !M.kscal.get()) {
+            M.kscal.alloc(need);
+            BCMG_HIP(hipMemsetAsync(M.kscal.get(), 0, need * sizeof(double), c.stream));
+        }
+        M.F.kflag = M.kscal.get() + KS_FLAG_GLOBAL;
+    } else {
+        M.F.kflag = nullptr;
     }
     M.F.nc = M.kmax;
     M.F.maxit = B.prm.itnlim;
@@ -207,9 +223,62 @@ void relax_level_m(MRun<VT> &R, const Level &lv, MatType mt, int type, int sweep
     }
 }
 
-/* cycle of host_amg.hip on interleaved blocks (src/SOLVER/GAMG_cycle.cu:25-140) */
+/* inner-iteration tolerance of the K-cycle (src/SOLVER/GAMG_cycle.cu:5) */
+constexpr double KCYCLE_RTOL_M = 0.25;
+
 template <typename VT>
-void cycle_m(MRun<VT> &R, int l)
+void cycle_m(MRun<VT> &R, int l, const VT *fin = nullptr);
+
+/* kcycle_inner of host_amg.hip on interleaved blocks: per column, at most
+   two flexible CG steps on A_m x = f_m preconditioned by the cycle from level
+   m + 1 (src/SOLVER/FCG.cu:620-723), every decision on the device */
+template <typename VT>
+void kcycle_inner_m(MRun<VT> &R, int m)
+{
+    Ctx &c = R.c;
+    Boot &B = R.B;
+    MultiWork &M = R.M;
+    CycleBufs<VT> &W = mbufs<VT>(M);
+    const DevCsr &A = R.H.L[m].mat();
+    const MatType mt = level_mt(B, m);
+    const int n = A.n, lkp = R.lkp, nc = R.nc;
+    const int *act = R.act;
+    double *ks = M.kscal.get() + (size_t)m * M.kmax * KS_STRIDE;
+    double *gflag = M.kscal.get() + KS_FLAG_GLOBAL;
+    double *part = M.part.get();
+    VT *w = W.kw[m].get(), *d0 = W.kd0[m].get(), *d1 = W.kd1[m].get();
+    VT *v = W.kv[m].get(), *v1 = W.kv1[m].get();
+    MState<VT> &s = R.st[m];
+    launch_copy_il<VT>(c.stream, n, W.f[m].get(), w, lkp, nc, act);
+    launch_kdots_il<VT>(c.stream, n, 1, w, nullptr, nullptr, w, lkp, nc, act, part, ks, 0);
+    auto apply = [&](VT *out) {
+        s.zero = true;
+        cycle_m<VT>(R, m + 1, w);
+        if (s.zero)
+            launch_fill_il<VT>(c.stream, n, VT(0), out, lkp, nc, act);
+        else
+            launch_copy_il<VT>(c.stream, n, s.x, out, lkp, nc, act);
+    };
+    apply(d0);
+    launch_kdots_il<VT>(c.stream, n, 1, w, nullptr, nullptr, d0, lkp, nc, act, part, ks, 1);
+    launch_rows_il<VT>(c.stream, A, mt, OP_SPMV, d0, nullptr, nullptr, v, lkp, nc, act);
+    launch_kdots_il<VT>(c.stream, n, 1, d0, nullptr, nullptr, v, lkp, nc, act, part, ks, 2);
+    launch_kcycle_alpha_il(c.stream, ks, gflag, nc, act);
+    launch_kcycle_axpy_il<VT>(c.stream, n, ks, v, w, lkp, nc, act);
+    launch_kdots_il<VT>(c.stream, n, 1, w, nullptr, nullptr, w, lkp, nc, act, part, ks, 4);
+    apply(d1);
+    launch_rows_il<VT>(c.stream, A, mt, OP_SPMV, d1, nullptr, nullptr, v1, lkp, nc, act);
+    launch_kdots_il<VT>(c.stream, n, 3, v, v1, w, d1, lkp, nc, act, part, ks, 5);
+    launch_kcycle_coef_il(c.stream, ks, KCYCLE_RTOL_M, nc, act);
+    launch_kcycle_combine_il<VT>(c.stream, n, ks, d0, d1, s.x, lkp, nc, act);
+    s.zero = false;
+}
+
+/* cycle of host_amg.hip on interleaved blocks (src/SOLVER/GAMG_cycle.cu:25-140);
+   fin, when given, is the level's right-hand side (an inner K-cycle
+   application) */
+template <typename VT>
+void cycle_m(MRun<VT> &R, int l, const VT *fin)
 {
     Ctx &c = R.c;
     Boot &B = R.B;
@@ -220,7 +289,7 @@ void cycle_m(MRun<VT> &R, int l)
     const Level &lv = H.L[l - 1];
     const MatType mt = level_mt(B, l - 1);
     MState<VT> &s = R.st[l - 1];
-    const VT *f = (l == 1) ? R.f0 : W.f[l - 1].get();
+    const VT *f = fin ? fin : ((l == 1) ? R.f0 : W.f[l - 1].get());
     const bool top = (l == 1);
     if (l == L) {
         relax_level_m<VT>(R, lv, mt, B.coarse_solver, B.prm.relaxnumber_coarse, f, s,
@@ -239,9 +308,13 @@ void cycle_m(MRun<VT> &R, int l)
                        OP_SPMV, rsrc, nullptr, nullptr, W.f[l].get(), lkp, R.nc, R.act);
     MState<VT> &sc = R.st[l];
     sc.zero = true;
-    for (int i = 1; i <= B.num_grid_sweeps[l - 1]; i++) {
-        cycle_m<VT>(R, l + 1);
-        if (l == L - 1) break;
+    if (B.prm.cycle_type == 3 && l != L - 1) {
+        kcycle_inner_m<VT>(R, l);
+    } else {
+        for (int i = 1; i <= B.num_grid_sweeps[l - 1]; i++) {
+            cycle_m<VT>(R, l + 1);
+            if (l == L - 1) break;
+        }
     }
     if (!sc.zero) {
         if (s.zero) {
@@ -428,6 +501,9 @@ void SlotEngine::admit(const std::vector<int> &js, const std::vector<const doubl
         h_par[j] = 0;
         h_it[j] = 0;
         launch_copy<double>(c.stream, n, bs[q], F.B + (size_t)j * ld);
+        if (F.kflag)
+            BCMG_HIP(hipMemsetAsync(F.kflag + (size_t)j * KS_STRIDE, 0, sizeof(double),
+                                    c.stream));
     }
     upload_flags(c, M);
     launch_copy_multi<double, double>(c.stream, n, F.B, ld, F.R, ld, km, M.mask);
@@ -605,18 +681,21 @@ size_t multi_work_bytes(const Boot &B, int kmax)
     size_t maxL = 0;
     for (auto &h : B.H) maxL = std::max(maxL, h.L.size());
     const size_t kp = (size_t)1 << log2_cols(kmax);
+    /* per level: x, x2, f (and, for the K-cycle, its five inner vectors) */
+    const size_t per_level = B.prm.cycle_type == 3 ? 8 : 3;
     size_t cyc = 0, n0 = 0;
     for (size_t l = 0; l < maxL; l++) {
         size_t nl = 0;
         for (auto &h : B.H)
             if (l < h.L.size()) nl = std::max(nl, (size_t)h.L[l].mat().n);
         if (l == 0) n0 = nl;
-        cyc += 3 * nl * kp;
+        cyc += per_level * nl * kp;
     }
     cyc += n0 * kp * (3 + (B.prm.solver_type == 2 ? 2 : 0)); /* res, r, z (+ additive) */
+    const size_t ksc = B.prm.cycle_type == 3 ? maxL * (size_t)kmax * KS_STRIDE * 8 : 0;
     const size_t vt = B.prec == BCMG_PREC_FP64 ? 8 : 4;
     const size_t fcg = 9 * (size_t)round_ld(n0) * kmax * 8;
-    return cyc * vt + fcg + multi_part_size(B.A0->n, kmax) * 8 + (size_t)kmax * 80;
+    return cyc * vt + fcg + ksc + multi_part_size(B.A0->n, kmax) * 8 + (size_t)kmax * 80;
 }
 
 /* k right-hand sides through one slot engine: admit all, iterate until

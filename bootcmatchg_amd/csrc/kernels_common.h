@@ -119,6 +119,39 @@ __device__ __forceinline__ void staged_lds(int tile, int *&lcol, MT *&lval)
 
 constexpr int RED_FINAL_BS = 1024; /* threads of the one-block final reductions */
 
+/* K-cycle inner iteration scalars of one level (ks, KS_STRIDE doubles):
+   [0] w.w  [1] w.d0  [2] d0.v  [3] alpha  [4] w'.w'  [5] d1.v  [6] d1.v1
+   [7] d1.w'  [8] c0  [9] c1, [KS_FLAG] the level's breakdown flag */
+constexpr int KS_FLAG = 15;
+
+/* alpha = delta_old / tau; a non-positive delta_old (indefinite
+   preconditioner) or tau (indefinite matrix) raises the flags - the
+   reference exits there (src/SOLVER/FCG.cu:625-628, 640-643) */
+__device__ __forceinline__ void kcycle_alpha_apply(double *ks, double *gflag)
+{
+    if (!(ks[1] > 0.0)) ks[KS_FLAG] = *gflag = 1.0;
+    if (!(ks[2] > 0.0)) ks[KS_FLAG] = *gflag = 2.0;
+    ks[3] = ks[1] / ks[2];
+}
+
+/* the combination coefficients of the inner iteration: one step when
+   ||w'|| <= rtol ||w|| (c1 = 0), else the two-step update
+   (src/SOLVER/FCG.cu:650-723) */
+__device__ __forceinline__ void kcycle_coef_apply(double *ks, double rtol)
+{
+    const double delta0 = sqrt(ks[0]), l2 = sqrt(ks[4]);
+    const double alpha = ks[3];
+    if (l2 <= rtol * delta0) {
+        ks[8] = alpha;
+        ks[9] = 0.0;
+        return;
+    }
+    const double tau = ks[2], tau1 = ks[5], tau2 = ks[6], tau3 = ks[7];
+    const double tau4 = tau2 - (tau1 * tau1) / tau;
+    ks[8] = alpha - (tau1 * tau3) / (tau * tau4);
+    ks[9] = tau3 / tau4;
+}
+
 /* fixed-order block reduction of K partial sums; thread 0 writes out[0..K) */
 template <int K>
 __device__ __forceinline__ void block_reduce_store(double (&s)[K], double *out)

@@ -519,6 +519,86 @@ __global__ __launch_bounds__(BS) void k_transpose_il(int n, const S *__restrict_
     }
 }
 
+/* --------------------------------------------------------- K-cycle
+   The inner iteration of host_amg.hip's kcycle_inner on interleaved blocks:
+   column j keeps its scalars at ks + j KS_STRIDE (the single-column layout)
+   and gets the single path's arithmetic, reduction order included. */
+
+/* k_kdots<K, VT> per column (column j = blockIdx.y): the same grid-stride
+   partition and block tree, partials at part[(j * nb + b) * K + k] */
+template <int K, typename VT>
+__global__ __launch_bounds__(BS) void k_kdots_il(int n, const VT *__restrict__ a0,
+                                                 const VT *__restrict__ a1,
+                                                 const VT *__restrict__ a2,
+                                                 const VT *__restrict__ b, int lkp, int nc,
+                                                 const int *__restrict__ act,
+                                                 double *__restrict__ part)
+{
+    const int j = blockIdx.y;
+    if (!col_on(j, nc, act)) return;
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = 0.0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const size_t o = ((size_t)i << lkp) + j;
+        const double bi = (double)b[o];
+        s[0] = fma((double)a0[o], bi, s[0]);
+        if constexpr (K > 1) s[1] = fma((double)a1[o], bi, s[1]);
+        if constexpr (K > 2) s[2] = fma((double)a2[o], bi, s[2]);
+    }
+    block_reduce_store<K>(s, part + ((size_t)j * gridDim.x + blockIdx.x) * K);
+}
+
+/* k_reduce_final<K, false> per column (block j) into ks + j KS_STRIDE + at */
+template <int K>
+__global__ __launch_bounds__(BS) void k_kdots_final_il(int nb, const double *__restrict__ part,
+                                                       double *__restrict__ ks, int at, int nc,
+                                                       const int *__restrict__ act)
+{
+    const int j = blockIdx.x;
+    if (!col_on(j, nc, act)) return;
+    reduce_final_body<K, false>(nb, part + (size_t)j * nb * K, ks + (size_t)j * KS_STRIDE + at);
+}
+
+/* alpha per column; a breakdown raises the column's global flag */
+__global__ void k_kcycle_alpha_il(double *ks, double *gflag, int nc, const int *__restrict__ act)
+{
+    const int j = threadIdx.x;
+    if (!col_on(j, nc, act)) return;
+    kcycle_alpha_apply(ks + (size_t)j * KS_STRIDE, gflag + (size_t)j * KS_STRIDE);
+}
+
+__global__ void k_kcycle_coef_il(double *ks, double rtol, int nc, const int *__restrict__ act)
+{
+    const int j = threadIdx.x;
+    if (!col_on(j, nc, act)) return;
+    kcycle_coef_apply(ks + (size_t)j * KS_STRIDE, rtol);
+}
+
+/* w = -alpha_j v + w */
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_kcycle_axpy_il(int n, const double *__restrict__ ks,
+                                                       const VT *__restrict__ v,
+                                                       VT *__restrict__ w, int lkp, int nc,
+                                                       const int *__restrict__ act)
+{
+    IL_BEGIN
+    w[o] = fma(-(VT)ks[(size_t)j * KS_STRIDE + 3], v[o], w[o]);
+}
+
+/* x = c0_j d0 + c1_j d1 */
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_kcycle_combine_il(int n, const double *__restrict__ ks,
+                                                          const VT *__restrict__ d0,
+                                                          const VT *__restrict__ d1,
+                                                          VT *__restrict__ x, int lkp, int nc,
+                                                          const int *__restrict__ act)
+{
+    IL_BEGIN
+    const double *k = ks + (size_t)j * KS_STRIDE;
+    x[o] = fma((VT)k[9], d1[o], fma((VT)k[8], d0[o], VT(0)));
+}
+
 /* ------------------------------------------------------------------ FCG */
 
 /* Per-column FCG state. Directions live in two slot pairs (V[s], W[s]); the
@@ -536,6 +616,8 @@ struct FcgCols {
     int *act, *par, *it;
     const double *tol;
     int maxit;
+    /* K-cycle: column j's breakdown flag at kflag[j * KS_STRIDE] (or null) */
+    const double *kflag;
 };
 
 __device__ __forceinline__ double *slot(double *s0, double *s1, int s, int j, long ld)
@@ -723,6 +805,12 @@ __global__ __launch_bounds__(RED_FINAL_BS) void k_fcg_final_multi(int nb,
         F.it[j] = it;
         F.par[j] ^= 1;
         if (!(scal[6] > F.tol[j] && it < F.maxit)) F.act[j] = 0;
+        if (F.kflag && F.kflag[(size_t)j * KS_STRIDE] != 0.0) {
+            /* an inner K-cycle iteration broke down: the column stops with a
+               non-finite residual (the single-column solve's error) */
+            scal[6] = __builtin_nan("");
+            F.act[j] = 0;
+        }
     }
 }
 
@@ -857,6 +945,7 @@ inline FcgCols fcg_cols(const MultiFcg &M)
     F.it = M.it;
     F.tol = M.tol;
     F.maxit = M.maxit;
+    F.kflag = M.kflag;
     return F;
 }
 
@@ -1132,6 +1221,70 @@ template void launch_from_il<double, double>(hipStream_t, int, const double *, d
                                              int, int, const int *);
 template void launch_from_il<float, double>(hipStream_t, int, const float *, double *, long,
                                             int, int, const int *);
+
+/* ---------------------------------------------------- K-cycle launchers */
+
+template <typename VT>
+void launch_kdots_il(hipStream_t s, int n, int k, const VT *a0, const VT *a1, const VT *a2,
+                     const VT *b, int lkp, int nc, const int *act, double *part, double *ks,
+                     int at)
+{
+    const int nb = red_blocks(n);
+    const dim3 g(nb, nc);
+    if (k == 1) {
+        k_kdots_il<1, VT><<<g, BS, 0, s>>>(n, a0, a1, a2, b, lkp, nc, act, part);
+        k_kdots_final_il<1><<<nc, BS, 0, s>>>(nb, part, ks, at, nc, act);
+    } else if (k == 2) {
+        k_kdots_il<2, VT><<<g, BS, 0, s>>>(n, a0, a1, a2, b, lkp, nc, act, part);
+        k_kdots_final_il<2><<<nc, BS, 0, s>>>(nb, part, ks, at, nc, act);
+    } else {
+        k_kdots_il<3, VT><<<g, BS, 0, s>>>(n, a0, a1, a2, b, lkp, nc, act, part);
+        k_kdots_final_il<3><<<nc, BS, 0, s>>>(nb, part, ks, at, nc, act);
+    }
+    BCMG_CHECK_LAUNCH();
+}
+
+void launch_kcycle_alpha_il(hipStream_t s, double *ks, double *gflag, int nc, const int *act)
+{
+    k_kcycle_alpha_il<<<1, 64, 0, s>>>(ks, gflag, nc, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+void launch_kcycle_coef_il(hipStream_t s, double *ks, double rtol, int nc, const int *act)
+{
+    k_kcycle_coef_il<<<1, 64, 0, s>>>(ks, rtol, nc, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_kcycle_axpy_il(hipStream_t s, int n, const double *ks, const VT *v, VT *w, int lkp,
+                           int nc, const int *act)
+{
+    if (n <= 0) return;
+    k_kcycle_axpy_il<VT><<<il_blocks(n, nc), BS, 0, s>>>(n, ks, v, w, lkp, nc, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_kcycle_combine_il(hipStream_t s, int n, const double *ks, const VT *d0, const VT *d1,
+                              VT *x, int lkp, int nc, const int *act)
+{
+    if (n <= 0) return;
+    k_kcycle_combine_il<VT><<<il_blocks(n, nc), BS, 0, s>>>(n, ks, d0, d1, x, lkp, nc, act);
+    BCMG_CHECK_LAUNCH();
+}
+
+#define BCMG_INST_KCYCLE_IL(VT)                                                             \
+    template void launch_kdots_il<VT>(hipStream_t, int, int, const VT *, const VT *,          \
+                                      const VT *, const VT *, int, int, const int *, double *, \
+                                      double *, int);                                         \
+    template void launch_kcycle_axpy_il<VT>(hipStream_t, int, const double *, const VT *, VT *, \
+                                            int, int, const int *);                           \
+    template void launch_kcycle_combine_il<VT>(hipStream_t, int, const double *, const VT *,    \
+                                               const VT *, VT *, int, int, const int *);
+BCMG_INST_KCYCLE_IL(double)
+BCMG_INST_KCYCLE_IL(float)
+#undef BCMG_INST_KCYCLE_IL
 
 size_t multi_part_size(int n, int nc)
 {

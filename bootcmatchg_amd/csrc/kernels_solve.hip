@@ -374,7 +374,6 @@ __global__ __launch_bounds__(BS) void k_fcg_first(int n, const double *__restric
    Device scalars of one level's inner iteration (src/SOLVER/FCG.cu:620-723),
    ks = [w.w (delta0^2), w.d0 (delta_old), d0.v (tau), alpha, w'.w' (l2^2),
          d1.v (tau1), d1.v1 (tau2), d1.w' (tau3), c0, c1, ..., flag]. */
-constexpr int KS_FLAG = 15;
 
 /* up to 3 products of VT vectors with b, accumulated in double (fma over a
    grid-stride partition, then the fixed block tree) */
@@ -403,9 +402,7 @@ __global__ __launch_bounds__(BS) void k_kdots(int n, const VT *__restrict__ a0,
 __global__ void k_kcycle_alpha(double *ks, double *gflag)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (!(ks[1] > 0.0)) ks[KS_FLAG] = *gflag = 1.0;
-    if (!(ks[2] > 0.0)) ks[KS_FLAG] = *gflag = 2.0;
-    ks[3] = ks[1] / ks[2];
+    kcycle_alpha_apply(ks, gflag);
 }
 
 /* w = -alpha v + w */
@@ -423,17 +420,7 @@ __global__ __launch_bounds__(BS) void k_kcycle_axpy(int n, const double *__restr
 __global__ void k_kcycle_coef(double *ks, double rtol)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const double delta0 = sqrt(ks[0]), l2 = sqrt(ks[4]);
-    const double alpha = ks[3];
-    if (l2 <= rtol * delta0) {
-        ks[8] = alpha;
-        ks[9] = 0.0;
-        return;
-    }
-    const double tau = ks[2], tau1 = ks[5], tau2 = ks[6], tau3 = ks[7];
-    const double tau4 = tau2 - (tau1 * tau1) / tau;
-    ks[8] = alpha - (tau1 * tau3) / (tau * tau4);
-    ks[9] = tau3 / tau4;
+    kcycle_coef_apply(ks, rtol);
 }
 
 /* x = c0 d0 + c1 d1 (x starts at zero) */

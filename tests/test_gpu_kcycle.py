@@ -2,7 +2,9 @@
 next level is not the coarsest, src/SOLVER/FCG.cu:620-723) on the GPU against
 the oracle. The hierarchy is bit-exact as for the other cycles; one
 preconditioner application agrees with the oracle's to 1e-10 (different
-summation orders in the inner products); FCG iterations within +-2."""
+summation orders in the inner products); FCG iterations within +-2. The
+multi-column and serving forms give every column the single solve's bits.
+"""
 import numpy as np
 import pytest
 
@@ -71,10 +73,56 @@ def test_kcycle_reduced_precision_and_graphs(ctx):
     assert np.linalg.norm(bh - ops.to_scipy(A) @ xe.numpy()) <= K["rtol"] * np.linalg.norm(bh) * 1.000001
 
 
-def test_kcycle_multi_column_is_unsupported(ctx):
-    A = OPERATORS["poisson3d_12"]()
+@pytest.mark.parametrize("prec", [bc.PREC_FP64, bc.PREC_FP32])
+def test_kcycle_multi_column_equals_single(ctx, prec):
+    """Several right-hand sides through one K-cycle composite: every column
+    bit-identical to its single solve (per-column inner iterations, dots in
+    the single path's order), also with more columns than one block holds."""
+    A = OPERATORS["lognormal3d_14"]()
     n = len(A[0]) - 1
     B = bc.Boot(ctx, bc.Csr(ctx, A), bc.make_params(**K))
-    with pytest.raises(bc.BcmgError) as e:
-        B.fcg_multi([dev(ctx, np.ones(n))], [bc.DeviceArray(ctx, n)])
-    assert e.value.code == bc.EUNSUP
+    B.set_precision(prec)
+    k = 5
+    bs = [dev(ctx, ops.uniform_pm1(40 + j, n)) for j in range(k)]
+    xs = [bc.DeviceArray(ctx, n) for _ in range(k)]
+    its, res, st = B.fcg_multi(bs, xs)
+    assert list(st) == [0] * k
+    for j in range(k):
+        x1 = bc.DeviceArray(ctx, n)
+        it1, r1 = B.fcg(bs[j], x1)
+        assert its[j] == it1 and res[j] == r1, (j, its[j], it1)
+        assert np.array_equal(bits(xs[j].numpy()), bits(x1.numpy()))
+
+
+def test_kcycle_serving_alone_equals_mix(ctx):
+    """A K-cycle composite served next to a V-cycle one, requests arriving
+    over several steps: each result equals its single solve."""
+    Ak = OPERATORS["poisson3d_17"]()
+    Av = OPERATORS["aniso2d_48"]()
+    Bk = bc.Boot(ctx, bc.Csr(ctx, Ak), bc.make_params(**K))
+    Bv = bc.Boot(ctx, bc.Csr(ctx, Av), bc.make_params(**SETTINGS["V_L1_2sweeps"]))
+    S = bc.Server(ctx, 8 << 30, 4)
+    ok, ov = S.add(Bk), S.add(Bv)
+    reqs = []
+    for q in range(6):
+        B, op, A = (Bk, ok, Ak) if q % 2 == 0 else (Bv, ov, Av)
+        n = len(A[0]) - 1
+        reqs.append([B, op, dev(ctx, ops.uniform_pm1(60 + q, n)), bc.DeviceArray(ctx, n), n,
+                     None, None])
+    step = 0
+    while any(r[6] is None for r in reqs):
+        for q, r in enumerate(reqs):
+            if r[5] is None and q <= step:
+                r[5] = S.submit(r[1], r[2], r[3])
+        S.step()
+        for r in reqs:
+            if r[5] is not None and r[6] is None:
+                r[6] = S.result(r[5])
+        step += 1
+        assert step < 20000
+    S.close()
+    for B, _, b, x, n, _, res in reqs:
+        x1 = bc.DeviceArray(ctx, n)
+        it1, r1 = B.fcg(b, x1)
+        assert res["status"] == 0 and res["iterations"] == it1 and res["residual"] == r1
+        assert np.array_equal(bits(x.numpy()), bits(x1.numpy()))
