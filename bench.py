@@ -17,7 +17,9 @@ and on the bootstrap composite workload (cell-coefficient diffusion on 128^3,
 k = exp(g), g ~ N(0, 4), max_hrc 3, rho 0.5); the multi-right-hand-side
 form (32 columns through every launch, anisotropic 2048^2, W-cycle); and the
 serving loop (8 cached operators, Poisson arrivals, continuous batching)
-against one FP64 solve at a time on the same arrival trace; and smoothed
+against one FP64 solve at a time on the same arrival trace; the small
+case the reference runs on a CPU (32^3, b = 1, rtol 1e-6) on the GPU and on
+the serial oracle; and smoothed
 aggregation (aggr_type 1) against the unsmoothed drop-in on 32^3, 64^3,
 100^3 and 160^3 Poisson.
 """
@@ -224,6 +226,46 @@ def serving(bc, ctx, quick=False):
     for o in pool.ops:
         o["B"].free()
     return out
+
+
+def small_case(bc, ctx, steps):
+    """The reference's CPU-runnable case: 3D Poisson 32^3, b = 1, x0 = 0,
+    the drop-in settings with rtol 1e-6; full solves on the GPU against the
+    serial oracle on one core (median of 5 after one warm-up)."""
+    from bootcmatchg_amd import operators as ops
+    from oracle import oracle as orc
+    A = ops.poisson3d(32)
+    n = len(A[0]) - 1
+    b = np.ones(n)
+    settings = dict(SETTINGS, rtol=1e-6)
+    prm = bc.make_params(**settings)
+    dA = bc.Csr(ctx, A)
+    db = bc.DeviceArray.from_numpy(ctx, b)
+    dx = bc.DeviceArray(ctx, n)
+    lat = []
+    for k in range(steps + 1):
+        ctx.sync()
+        t0 = time.perf_counter()
+        B = bc.Boot(ctx, dA, prm)
+        it, _ = B.fcg(db, dx)
+        t1 = time.perf_counter()
+        B.free()
+        if k:
+            lat.append(t1 - t0)
+    x = dx.numpy()
+    cpu = []
+    for k in range(6):
+        t0 = time.perf_counter()
+        rc, xo, res = orc.solve(A, b, orc.params(**settings))
+        t1 = time.perf_counter()
+        if k:
+            cpu.append(t1 - t0)
+    return {"workload": "poisson3d_7pt_32^3 b=1 rtol 1e-6, settings "
+                        "0,1,0.8,2,0,39,0,4,4,16,4,4,2000,1e-6",
+            "gpu_solves_per_s": 1.0 / float(np.median(lat)), "gpu_iterations": it,
+            "cpu_oracle_solves_per_s": 1.0 / float(np.median(cpu)), "cpu_cores": 1,
+            "cpu_iterations": res.iterations,
+            "solution_rel_diff": float(np.linalg.norm(x - xo) / np.linalg.norm(xo))}
 
 
 SA_GRIDS = (32, 64, 100, 160)
@@ -470,6 +512,7 @@ def main():
     multi = None if args.no_multi else multi_rhs(bc, ctx, max(1, args.steps // 2))
     served = None if args.no_serving else serving(bc, ctx)
     sa = None if args.no_smoothed else smoothed(bc, ctx, max(1, args.steps // 2))
+    small = None if args.no_cpu_baseline else small_case(bc, ctx, args.steps)
 
     out = {
         "metric": METRIC,
@@ -514,6 +557,7 @@ def main():
         "multi_rhs": multi,
         "serving": served,
         "smoothed_aggregation": sa,
+        "small_case": small,
     }
     print(json.dumps(out))
 
