@@ -308,8 +308,17 @@ __global__ __launch_bounds__(BS) void k_rows_il(int n, const int *__restrict__ r
         }
         return;
     }
-    /* PF entries at a time: their indices and values first, then all their
-       gathers in flight, then the sums in entry order */
+    /* the row's own operands first, so that their latency overlaps the
+       gathers; then PF entries at a time: their indices and values, all
+       their gathers in flight, the sums in entry order */
+    const size_t o = ((size_t)r << lkp) + jg * W;
+    V fo = V(0), xo = V(0);
+    VT dr = VT(1);
+    if (OP != OP_SPMV) fo = *reinterpret_cast<const V *>(f + o);
+    if (OP == OP_JACOBI) {
+        xo = *reinterpret_cast<const V *>(x + o);
+        dr = d[r];
+    }
     for (int e0 = ra; e0 < re; e0 += PF) {
         int cc[PF];
         VT vv[PF];
@@ -328,16 +337,13 @@ __global__ __launch_bounds__(BS) void k_rows_il(int n, const int *__restrict__ r
         for (int u = 0; u < PF; u++)
             if (e0 + u < re) acc = acc + vv[u] * gg[u];
     }
-    const size_t o = ((size_t)r << lkp) + jg * W;
     V res;
-    if (OP == OP_SPMV) {
+    if (OP == OP_SPMV)
         res = acc;
-    } else if (OP == OP_RESID) {
-        res = *reinterpret_cast<const V *>(f + o) - acc;
-    } else {
-        const VT dr = d[r];
-        res = (*reinterpret_cast<const V *>(f + o) - acc) / dr + *reinterpret_cast<const V *>(x + o);
-    }
+    else if (OP == OP_RESID)
+        res = fo - acc;
+    else
+        res = (fo - acc) / dr + xo;
     if (all) {
         *reinterpret_cast<V *>(y + o) = res;
     } else {
