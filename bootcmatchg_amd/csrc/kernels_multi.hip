@@ -250,10 +250,15 @@ __device__ __forceinline__ void group_on(int jg, int W, int nc, const int *act, 
 /* stream-order row sums (k_rows_staged's arithmetic) on interleaved blocks:
    a lane owns W adjacent columns of one row and gathers them with one
    vector load per entry; per column p = a_ij x_j rounded, added left to
-   right from 0. Inactive columns of a partly active group are computed but
-   not stored. */
-template <typename MT, typename VT, int OP, int W>
-__global__ __launch_bounds__(BS) void k_rows_il(int n, const int *__restrict__ rp,
+   right from 0. A partly active group computes all W columns (they share
+   the gathered cache lines) and stores only the active ones; the columns of
+   a group lie inside the block's 2^lkp stride, so the idle ones are
+   allocated memory. */
+/* PF entries of a row are gathered at once. 7 waves per SIMD (at most 72
+   VGPRs) hold without spills for every instance launched: PF 8 only with
+   FP32 values and vectors, PF 6 otherwise. */
+template <typename MT, typename VT, int OP, int W, int PF>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_rows_il(int n, const int *__restrict__ rp,
                                                 const int *__restrict__ ci,
                                                 const MT *__restrict__ av,
                                                 const VT *__restrict__ x,
@@ -274,40 +279,6 @@ __global__ __launch_bounds__(BS) void k_rows_il(int n, const int *__restrict__ r
     const int ra = rp[r], re = rp[r + 1];
     V acc = V(0);
     const VT *xg = x + jg * W;
-    constexpr int PF = 8;
-    if (!all) {
-        /* a partly active group gathers only its active columns (a lane's
-           vector would fetch the idle ones too) */
-        for (int u = 0; u < W; u++) {
-            if (!col_on(jg * W + u, nc, act)) continue;
-            VT t = 0;
-            for (int e0 = ra; e0 < re; e0 += PF) {
-                int cc[PF];
-                VT vv[PF], gg[PF];
-#pragma unroll
-                for (int q = 0; q < PF; q++) {
-                    if (e0 + q < re) {
-                        cc[q] = ci[e0 + q];
-                        vv[q] = mval<VT>(av[e0 + q]);
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < PF; q++)
-                    if (e0 + q < re) gg[q] = xg[((size_t)cc[q] << lkp) + u];
-#pragma unroll
-                for (int q = 0; q < PF; q++)
-                    if (e0 + q < re) t = t + vv[q] * gg[q];
-            }
-            const size_t o = ((size_t)r << lkp) + jg * W + u;
-            if (OP == OP_SPMV)
-                y[o] = t;
-            else if (OP == OP_RESID)
-                y[o] = f[o] - t;
-            else
-                y[o] = (f[o] - t) / d[r] + x[o];
-        }
-        return;
-    }
     /* the row's own operands first, so that their latency overlaps the
        gathers; then PF entries at a time: their indices and values, all
        their gathers in flight, the sums in entry order */
@@ -964,15 +935,28 @@ void rows_il_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x, co
     const int n = A.n;
     const int *rp = A.row.get(), *ci = A.col.get();
     if (stream_ok<VT>(A)) {
+        /* W columns per lane: a lane's columns stay inside the 2^lkp stride
+           (W <= nc <= 2^lkp, both powers of two, or W = 1). Entries in
+           flight: 6 keeps 8 waves per SIMD and covers the short rows of 2D
+           stencils in one pass; all-FP32 rows of 7 or 8 entries (3D
+           stencils) take 8 at once at 7 waves rather than two passes (the
+           other types would spill at 8). The sums are the same for any
+           choice. */
         constexpr int WM = 16 / sizeof(VT);
-        if (nc >= WM)
-            k_rows_il<MT, VT, OP, WM><<<blocks_for((long long)n * ((nc + WM - 1) / WM)), BS, 0,
-                                        s>>>(n, rp, ci, av, x, f, d, y, lkp, nc, act);
+        constexpr bool all32 = std::is_same<MT, float>::value && std::is_same<VT, float>::value;
+        const bool pf8 = all32 && A.maxrow > 6 && A.maxrow <= 8;
+        const unsigned nb = (unsigned)blocks_for((long long)n * ((nc + WM - 1) / WM));
+        if (nc >= WM && pf8)
+            k_rows_il<MT, VT, OP, WM, (all32 ? 8 : 6)><<<nb, BS, 0, s>>>(
+                n, rp, ci, av, x, f, d, y, lkp, nc, act);
+        else if (nc >= WM)
+            k_rows_il<MT, VT, OP, WM, 6><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y, lkp, nc,
+                                                           act);
         else if (nc >= 2)
-            k_rows_il<MT, VT, OP, 2><<<blocks_for((long long)n * ((nc + 1) / 2)), BS, 0, s>>>(
+            k_rows_il<MT, VT, OP, 2, 6><<<blocks_for((long long)n * ((nc + 1) / 2)), BS, 0, s>>>(
                 n, rp, ci, av, x, f, d, y, lkp, nc, act);
         else
-            k_rows_il<MT, VT, OP, 1><<<blocks_for((long long)n * nc), BS, 0, s>>>(
+            k_rows_il<MT, VT, OP, 1, 6><<<blocks_for((long long)n * nc), BS, 0, s>>>(
                 n, rp, ci, av, x, f, d, y, lkp, nc, act);
     } else {
         const int G = A.group;
