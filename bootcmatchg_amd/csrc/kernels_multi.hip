@@ -665,7 +665,10 @@ __global__ void k_fcg_scalars_multi(double *scal, int nc, int first, const int *
 /* SpMV fused with the FCG products (the arithmetic of k_spmv_dots): t = A z,
    per column the block partials of (r.z, t.z[, to.z]). K = 2: first step, t
    into pair 0; K = 3: t into pair par ^ 1, to from pair par. One row per
-   thread, CH columns per block row (chunk blockIdx.y). */
+   thread, CH columns per block. The grid is one-dimensional: the nch column
+   chunks of a row block are consecutive blocks of one XCD, so the matrix
+   rows that the first chunk brings into that XCD's L2 serve the others. */
+constexpr int SD_PF = 8; /* entries whose indices are loaded at once */
 template <int K>
 __global__ __launch_bounds__(BS) void k_spmv_dots_multi(int n, const int *__restrict__ rp,
                                                         const int *__restrict__ ci,
@@ -674,7 +677,10 @@ __global__ __launch_bounds__(BS) void k_spmv_dots_multi(int n, const int *__rest
                                                         double *__restrict__ part)
 {
     __shared__ double sh[CH][K][BS / 64];
-    const int j0 = blockIdx.y * CH;
+    const int L = xcd_block(blockIdx.x, gridDim.x);
+    const int nch = (nc + CH - 1) / CH, nb = (int)(gridDim.x / nch);
+    const int b = L / nch;
+    const int j0 = (L - b * nch) * CH;
     bool on[CH];
     bool any = false, all = true;
 #pragma unroll
@@ -684,7 +690,6 @@ __global__ __launch_bounds__(BS) void k_spmv_dots_multi(int n, const int *__rest
         all &= on[u];
     }
     if (!any) return;
-    const int b = xcd_block(blockIdx.x, gridDim.x);
     const int r = b * BS + threadIdx.x;
     const bool own = r < n;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -699,14 +704,26 @@ __global__ __launch_bounds__(BS) void k_spmv_dots_multi(int n, const int *__rest
     for (int u = 0; u < CH; u++) t[u] = 0.0;
     const double *zj = F.Z + (size_t)j0 * ld;
     if (all) {
-        for (int e = ra; e < re; e++) {
-            const int c = ci[e];
-            const double v = av[e];
-            double g[CH];
+        /* SD_PF entries' indices and values at once, then their gathers */
+        for (int e0 = ra; e0 < re; e0 += SD_PF) {
+            int cc[SD_PF];
+            double vv[SD_PF];
 #pragma unroll
-            for (int u = 0; u < CH; u++) g[u] = zj[(size_t)u * ld + c];
+            for (int q = 0; q < SD_PF; q++) {
+                if (e0 + q < re) {
+                    cc[q] = ci[e0 + q];
+                    vv[q] = av[e0 + q];
+                }
+            }
 #pragma unroll
-            for (int u = 0; u < CH; u++) t[u] = t[u] + v * g[u];
+            for (int q = 0; q < SD_PF; q++) {
+                if (e0 + q >= re) break;
+                double g[CH];
+#pragma unroll
+                for (int u = 0; u < CH; u++) g[u] = zj[(size_t)u * ld + cc[q]];
+#pragma unroll
+                for (int u = 0; u < CH; u++) t[u] = t[u] + vv[q] * g[u];
+            }
         }
     } else {
         for (int e = ra; e < re; e++) {
@@ -750,7 +767,7 @@ __global__ __launch_bounds__(BS) void k_spmv_dots_multi(int n, const int *__rest
     /* the cross-wave part: thread u sums column j0 + u's wave results */
     const int u = threadIdx.x;
     if (u < CH && col_on(j0 + u, nc, mask)) {
-        double *out = part + ((size_t)(j0 + u) * gridDim.x + b) * K;
+        double *out = part + ((size_t)(j0 + u) * nb + b) * K;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             double tt = 0.0;
@@ -1317,13 +1334,14 @@ void launch_spmv_dots_multi(Ctx &c, const DevCsr &A, int k, MultiFcg &M, const i
     }
     const int nb = (n + BS - 1) / BS;
     const int nch = (M.nc + CH - 1) / CH;
+    const unsigned grid = (unsigned)nb * (unsigned)nch;
     if (k == 2) {
-        k_spmv_dots_multi<2><<<dim3(nb, nch), BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(),
-                                                       F, M.nc, mask, part);
+        k_spmv_dots_multi<2><<<grid, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(),
+                                                        A.val.get(), F, M.nc, mask, part);
         k_fcg_final_multi<2, false><<<M.nc, RED_FINAL_BS, 0, c.stream>>>(nb, part, F, mask);
     } else {
-        k_spmv_dots_multi<3><<<dim3(nb, nch), BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), A.val.get(),
-                                                       F, M.nc, mask, part);
+        k_spmv_dots_multi<3><<<grid, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(),
+                                                        A.val.get(), F, M.nc, mask, part);
         k_fcg_final_multi<3, false><<<M.nc, RED_FINAL_BS, 0, c.stream>>>(nb, part, F, mask);
     }
     BCMG_CHECK_LAUNCH();
