@@ -246,6 +246,27 @@ def test_graph_replay_matches_eager_launches(ctx):
         assert np.array_equal(bits(x_g.numpy()), bits(x_e.numpy()))
 
 
+@pytest.mark.parametrize("setting", ["boot3_mult", "boot3_sym", "boot2_add_mixed"])
+def test_bootstrap_graph_replay_matches_eager_launches(ctx, setting):
+    """The bootstrap's composite applications replayed from a captured graph
+    give the same hierarchies bit for bit as host launches, for the three
+    composite kinds."""
+    A = OPERATORS["lognormal3d_14"]()
+    dA = bc.Csr(ctx, A)
+    prm = bc.make_params(**SETTINGS[setting])
+    bc.use_graphs(ctx, False)
+    Be = bc.Boot(ctx, dA, prm)
+    bc.use_graphs(ctx, True)
+    Bg = bc.Boot(ctx, dA, prm)
+    assert Bg.info() == Be.info() and Be.info()[0] >= 2
+    for k in range(Be.info()[0]):
+        assert Bg.levels(k) == Be.levels(k)
+        for l in range(Be.levels(k)):
+            g, e = Bg.level_A(k, l), Be.level_A(k, l)
+            assert np.array_equal(g[0], e[0]) and np.array_equal(g[1], e[1]), (k, l)
+            assert np.array_equal(bits(g[2]), bits(e[2])), (k, l)
+
+
 def permuted_poisson(N, seed):
     """3D Poisson with rows and columns permuted at random: no band, entries
     anywhere in a row."""
