@@ -153,7 +153,13 @@ class Context:
             self.h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module's globals (lib) may already be gone
+        # and the process is ending: a failed release is not reported there;
+        # explicit free()/close() calls still raise
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def sync(self):
         _check(lib().bcmg_sync(self.h))
@@ -196,6 +202,9 @@ class DeviceArray:
     _release = free
 
     def __del__(self):
+        # at interpreter exit the module's globals (lib) may already be gone
+        # and the process is ending: a failed release is not reported there;
+        # explicit free()/close() calls still raise
         try:
             self.free()
         except Exception:
@@ -239,7 +248,13 @@ class Csr:
     _release = free
 
     def __del__(self):
-        self.free()
+        # at interpreter exit the module's globals (lib) may already be gone
+        # and the process is ending: a failed release is not reported there;
+        # explicit free()/close() calls still raise
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Boot:
@@ -261,7 +276,13 @@ class Boot:
     _release = free
 
     def __del__(self):
-        self.free()
+        # at interpreter exit the module's globals (lib) may already be gone
+        # and the process is ending: a failed release is not reported there;
+        # explicit free()/close() calls still raise
+        try:
+            self.free()
+        except Exception:
+            pass
 
     def info(self):
         nh = ctypes.c_int()
@@ -384,7 +405,13 @@ class Server:
     _release = close
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module's globals (lib) may already be gone
+        # and the process is ending: a failed release is not reported there;
+        # explicit free()/close() calls still raise
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def add(self, boot):
         op = ctypes.c_int()
