@@ -544,7 +544,9 @@ __global__ __launch_bounds__(BS) void k_stage(int n, const int *__restrict__ rp,
 }
 
 /* first-occurrence flags of the coarse columns of each segment, and the
-   number of distinct columns (the coarse row length) */
+   number of distinct columns (the coarse row length). A segment of at most
+   G entries is read once, one entry per lane, and compared through
+   shuffles; a longer one by the group's loop over the staged arrays. */
 template <int G>
 __global__ __launch_bounds__(BS) void k_rap_count(int nc, const int *__restrict__ stoff,
                                                   const int *__restrict__ Jst,
@@ -554,26 +556,43 @@ __global__ __launch_bounds__(BS) void k_rap_count(int nc, const int *__restrict_
     const int tid = blockIdx.x * BS + threadIdx.x;
     const int I = tid / G;
     const int lane = threadIdx.x & (G - 1);
-    if (I >= nc) return;
+    if (I >= nc) return; /* the whole group: its lanes share I */
     const int s0 = stoff[I], s1 = stoff[I + 1];
     int c = 0;
-    for (int e = s0 + lane; e < s1; e += G) {
-        const int J = Jst[e];
-        bool f = true;
-        for (int e2 = s0; e2 < e; e2++)
-            if (Jst[e2] == J) {
-                f = false;
-                break;
-            }
-        first[e] = f ? 1 : 0;
-        c += f ? 1 : 0;
+    if (s1 - s0 <= G) {
+        const bool valid = lane < s1 - s0;
+        const int J = valid ? Jst[s0 + lane] : -1;
+        bool f = valid;
+        for (int k = 0; k < G; k++) {
+            const int Jk = __shfl(J, k, G);
+            if (k < lane && Jk == J) f = false;
+        }
+        if (valid) first[s0 + lane] = f ? 1 : 0;
+        c = f ? 1 : 0;
+    } else {
+        for (int e = s0 + lane; e < s1; e += G) {
+            const int J = Jst[e];
+            bool f = true;
+            for (int e2 = s0; e2 < e; e2++)
+                if (Jst[e2] == J) {
+                    f = false;
+                    break;
+                }
+            first[e] = f ? 1 : 0;
+            c += f ? 1 : 0;
+        }
     }
 #pragma unroll
     for (int o = G >> 1; o > 0; o >>= 1) c += __shfl_xor(c, o, G);
     if (lane == 0) cnt[I] = c;
 }
 
-/* coarse entries in ascending column order, values summed in the fixed order */
+/* coarse entries in ascending column order, values summed in the fixed order:
+   for each distinct column J of the segment, its rank among the distinct
+   columns and the two members' sums of their staged products, each in
+   staging order. A segment of at most G entries is read once and scanned
+   through shuffles (the same additions in the same order); a longer one by
+   the group's loop over the staged arrays. */
 template <int G>
 __global__ __launch_bounds__(BS) void k_rap_fill(int nc, const int *__restrict__ rp,
                                                  const int *__restrict__ members,
@@ -588,13 +607,46 @@ __global__ __launch_bounds__(BS) void k_rap_fill(int nc, const int *__restrict__
     const int tid = blockIdx.x * BS + threadIdx.x;
     const int I = tid / G;
     const int lane = threadIdx.x & (G - 1);
-    if (I >= nc) return;
+    if (I >= nc) return; /* the whole group: its lanes share I */
     const int s0 = stoff[I], s1 = stoff[I + 1];
     const int m0 = members[2 * I], m1 = members[2 * I + 1];
-    const int split = s0 + (rp[m0 + 1] - rp[m0]);
+    const int len0 = rp[m0 + 1] - rp[m0]; /* member 0's entries come first */
     const double p0 = p[m0];
     const double p1 = m1 >= 0 ? p[m1] : 0.0;
     const int out0 = rpc[I];
+    if (s1 - s0 <= G) {
+        const bool valid = lane < s1 - s0;
+        const int J = valid ? Jst[s0 + lane] : -1;
+        const double t = valid ? tst[s0 + lane] : 0.0;
+        const int f = valid ? first[s0 + lane] : 0;
+        int rank = 0;
+        double sa = 0.0, sb = 0.0;
+        bool ha = false, hb = false;
+        for (int k = 0; k < G; k++) {
+            const int Jk = __shfl(J, k, G);
+            const double tk = __shfl(t, k, G);
+            const int fk = __shfl(f, k, G);
+            rank += (fk && Jk < J) ? 1 : 0;
+            if (Jk == J) {
+                if (k < len0) {
+                    sa = sa + tk;
+                    ha = true;
+                } else {
+                    sb = sb + tk;
+                    hb = true;
+                }
+            }
+        }
+        if (valid && f) {
+            double acc = 0.0;
+            if (ha) acc = acc + p0 * sa;
+            if (hb) acc = acc + p1 * sb;
+            colc[out0 + rank] = J;
+            valc[out0 + rank] = acc;
+        }
+        return;
+    }
+    const int split = s0 + len0;
     for (int e = s0 + lane; e < s1; e += G) {
         if (!first[e]) continue;
         const int J = Jst[e];
