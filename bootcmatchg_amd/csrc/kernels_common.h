@@ -47,7 +47,7 @@ __device__ __forceinline__ void stage_entries(const int *__restrict__ ci,
                                               const MT *__restrict__ av, int e0, int cnt,
                                               int *__restrict__ lcol, MT *__restrict__ lval)
 {
-    constexpr int U = 4;
+    constexpr int U = 8;
     for (int k0 = threadIdx.x; k0 < cnt; k0 += BS * U) {
         int c[U];
         MT v[U];
