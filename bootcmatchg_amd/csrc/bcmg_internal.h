@@ -371,6 +371,13 @@ void launch_scale_multi(hipStream_t s, int n, VT alpha, VT *x, long ld, int nc, 
 template <typename VT>
 void launch_rows_il(hipStream_t s, const DevCsr &A, MatType mt, RowOp op, const VT *x,
                     const VT *f, const VT *d, VT *y, int lkp, int nc, const int *act);
+/* the last Jacobi/L1 sweep of a cycle with its result widened into the
+   column-major FP64 block z (column j at z + j ldz); false when the operator
+   takes the group kernel or the block is narrower than one vector (the
+   caller then sweeps and converts) */
+template <typename VT>
+bool launch_jacobi_il_c64(hipStream_t s, const DevCsr &A, MatType mt, const VT *x, const VT *f,
+                          const VT *d, double *z, long ldz, int lkp, int nc, const int *act);
 template <typename VT>
 void launch_jacobi_zero_il(hipStream_t s, int n, const VT *f, const VT *d, VT *y, int lkp,
                            int nc, const int *act);

@@ -187,6 +187,11 @@ struct MRun {
     int nc;
     int lkp;    /* interleave stride 2^lkp of this application */
     const int *act;
+    /* when set, the sweep that writes final_out writes its result widened
+       into this column-major FP64 block instead (and sets *fused) */
+    double *z64 = nullptr;
+    long ldz = 0;
+    bool *fused = nullptr;
 };
 
 /* relax_level of host_amg.hip on an interleaved block */
@@ -212,11 +217,19 @@ void relax_level_m(MRun<VT> &R, const Level &lv, MatType mt, int type, int sweep
     }
     for (int k = 0; k < sweeps; k++) {
         VT *out = s.x2;
-        if (final_out && k == sweeps - 1 && s.x != final_out) out = final_out;
-        if (s.zero)
+        const bool last = final_out && k == sweeps - 1;
+        if (last && s.x != final_out) out = final_out;
+        if (s.zero) {
             launch_jacobi_zero_il<VT>(c.stream, A.n, f, d, out, lkp, R.nc, R.act);
-        else
+        } else if (last && R.z64 &&
+                   launch_jacobi_il_c64<VT>(c.stream, A, mt, s.x, f, d, R.z64, R.ldz, lkp, R.nc,
+                                            R.act)) {
+            /* the result is in z64 only; final_out is not written */
+            out = final_out;
+            *R.fused = true;
+        } else {
             launch_rows_il<VT>(c.stream, A, mt, OP_JACOBI, s.x, f, d, out, lkp, R.nc, R.act);
+        }
         s.x2 = s.x;
         s.x = out;
         s.zero = false;
@@ -346,14 +359,19 @@ void run_hierarchy_m(MRun<VT> &R, int k)
         R.st[l].x2 = W.xb[l].get();
         R.st[l].zero = true;
     }
-    MRun<VT> Rk{R.c, R.B, R.M, H, R.st, R.f0, R.final_out, R.nc, R.lkp, R.act};
+    MRun<VT> Rk{R.c, R.B, R.M, H, R.st, R.f0, R.final_out, R.nc, R.lkp, R.act, R.z64, R.ldz,
+                R.fused};
     cycle_m<VT>(Rk, 1);
 }
 
-/* precond_apply_t of host_amg.hip from x = 0 on interleaved blocks */
+/* precond_apply_t of host_amg.hip from x = 0 on interleaved blocks. With
+   z64, a multiplicative or symmetrized composite may deliver its result
+   widened into that column-major FP64 block instead of x; *fused tells. */
 template <typename VT>
-void precond_m(Ctx &c, Boot &B, MultiWork &M, const VT *r, VT *x, const int *act)
+void precond_m(Ctx &c, Boot &B, MultiWork &M, const VT *r, VT *x, const int *act,
+               double *z64 = nullptr, long ldz = 0, bool *fused = nullptr)
 {
+    if (fused) *fused = false;
     CycleBufs<VT> &W = mbufs<VT>(M);
     const int n = B.A0->n;
     const int nc = M.F.nc; /* columns in use */
@@ -373,9 +391,11 @@ void precond_m(Ctx &c, Boot &B, MultiWork &M, const VT *r, VT *x, const int *act
             for (int k = nh - 1; k >= 0; k--) order.push_back(k);
         for (size_t t = 0; t < order.size(); t++) {
             const bool last = (t + 1 == order.size());
-            MRun<VT> R{c, B, M, B.H[order[t]], st, r, last ? x : nullptr, nc, lkp, act};
+            MRun<VT> R{c,   B,   M,   B.H[order[t]], st,   r, last ? x : nullptr,
+                       nc,  lkp, act, last ? z64 : nullptr, ldz, fused};
             run_hierarchy_m<VT>(R, order[t]);
         }
+        if (fused && *fused) return;
         if (st[0].zero)
             launch_fill_il<VT>(c.stream, n, VT(0), x, lkp, nc, act);
         else if (st[0].x != x)
@@ -398,21 +418,24 @@ void precond_m(Ctx &c, Boot &B, MultiWork &M, const VT *r, VT *x, const int *act
 
 /* Z = B(R) for the masked slots: R is turned into the cycle's interleaved
    layout (rounded to FP32 for a reduced-precision hierarchy), the cycles run,
-   and the result comes back column-major in FP64 */
+   and the result comes back column-major in FP64: written there by the
+   cycle's last sweep where that sweep can, converted after otherwise */
 void precond_block(Ctx &c, Boot &B, MultiWork &M, const int *act)
 {
     const int n = B.A0->n;
     const long ld = M.F.ld;
     const int nc = M.F.nc, lkp = log2_cols(nc);
+    bool fused = false;
     if (B.prec == BCMG_PREC_FP64) {
         launch_to_il<double, double>(c.stream, n, M.F.R, ld, M.r64.get(), lkp, nc, act);
-        precond_m<double>(c, B, M, M.r64.get(), M.z64.get(), act);
-        launch_from_il<double, double>(c.stream, n, M.z64.get(), M.F.Z, ld, lkp, nc, act);
+        precond_m<double>(c, B, M, M.r64.get(), M.z64.get(), act, M.F.Z, ld, &fused);
+        if (!fused)
+            launch_from_il<double, double>(c.stream, n, M.z64.get(), M.F.Z, ld, lkp, nc, act);
         return;
     }
     launch_to_il<double, float>(c.stream, n, M.F.R, ld, M.r32.get(), lkp, nc, act);
-    precond_m<float>(c, B, M, M.r32.get(), M.z32.get(), act);
-    launch_from_il<float, double>(c.stream, n, M.z32.get(), M.F.Z, ld, lkp, nc, act);
+    precond_m<float>(c, B, M, M.r32.get(), M.z32.get(), act, M.F.Z, ld, &fused);
+    if (!fused) launch_from_il<float, double>(c.stream, n, M.z32.get(), M.F.Z, ld, lkp, nc, act);
 }
 
 void upload_flags(Ctx &c, MultiWork &M)

@@ -253,35 +253,23 @@ __device__ __forceinline__ void group_on(int jg, int W, int nc, const int *act, 
    right from 0. A partly active group computes all W columns (they share
    the gathered cache lines) and stores only the active ones; the columns of
    a group lie inside the block's 2^lkp stride, so the idle ones are
-   allocated memory. */
-/* PF entries of a row are gathered at once. 7 waves per SIMD (at most 72
-   VGPRs) hold without spills for every instance launched: PF 8 only with
-   FP32 values and vectors, PF 6 otherwise. */
+   allocated memory.
+   il_row: the sums of lane (row r, column group jg) and the operator's
+   epilogue. The row's own operands come first, so that their latency
+   overlaps the gathers; then PF entries at a time: their indices and values,
+   all their gathers in flight, the sums in entry order. */
 template <typename MT, typename VT, int OP, int W, int PF>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_rows_il(int n, const int *__restrict__ rp,
-                                                const int *__restrict__ ci,
-                                                const MT *__restrict__ av,
-                                                const VT *__restrict__ x,
-                                                const VT *__restrict__ f,
-                                                const VT *__restrict__ d, VT *__restrict__ y,
-                                                int lkp, int nc, const int *__restrict__ act)
+__device__ __forceinline__ vec_t<VT, W> il_row(int r, int jg, const int *__restrict__ rp,
+                                               const int *__restrict__ ci,
+                                               const MT *__restrict__ av,
+                                               const VT *__restrict__ x,
+                                               const VT *__restrict__ f,
+                                               const VT *__restrict__ d, int lkp)
 {
     using V = vec_t<VT, W>;
-    const int gpr = (nc + W - 1) / W; /* lanes per row: the columns in use */
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    const long t = (long)b * BS + threadIdx.x;
-    const int r = (int)(t / gpr);
-    const int jg = (int)(t % gpr);
-    if (r >= n) return;
-    bool any, all;
-    group_on(jg, W, nc, act, any, all);
-    if (!any) return;
     const int ra = rp[r], re = rp[r + 1];
     V acc = V(0);
     const VT *xg = x + jg * W;
-    /* the row's own operands first, so that their latency overlaps the
-       gathers; then PF entries at a time: their indices and values, all
-       their gathers in flight, the sums in entry order */
     const size_t o = ((size_t)r << lkp) + jg * W;
     V fo = V(0), xo = V(0);
     VT dr = VT(1);
@@ -308,18 +296,64 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_
         for (int u = 0; u < PF; u++)
             if (e0 + u < re) acc = acc + vv[u] * gg[u];
     }
-    V res;
-    if (OP == OP_SPMV)
-        res = acc;
-    else if (OP == OP_RESID)
-        res = fo - acc;
-    else
-        res = (fo - acc) / dr + xo;
+    if (OP == OP_SPMV) return acc;
+    if (OP == OP_RESID) return fo - acc;
+    return (fo - acc) / dr + xo;
+}
+
+/* PF entries of a row are gathered at once. 7 waves per SIMD (at most 72
+   VGPRs) hold without spills for every instance launched: PF 8 only with
+   FP32 values and vectors, PF 6 otherwise. */
+template <typename MT, typename VT, int OP, int W, int PF>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_rows_il(
+    int n, const int *__restrict__ rp, const int *__restrict__ ci, const MT *__restrict__ av,
+    const VT *__restrict__ x, const VT *__restrict__ f, const VT *__restrict__ d,
+    VT *__restrict__ y, int lkp, int nc, const int *__restrict__ act)
+{
+    using V = vec_t<VT, W>;
+    const int gpr = (nc + W - 1) / W; /* lanes per row: the columns in use */
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const long t = (long)b * BS + threadIdx.x;
+    const int r = (int)(t / gpr);
+    const int jg = (int)(t % gpr);
+    if (r >= n) return;
+    bool any, all;
+    group_on(jg, W, nc, act, any, all);
+    if (!any) return;
+    const V res = il_row<MT, VT, OP, W, PF>(r, jg, rp, ci, av, x, f, d, lkp);
+    const size_t o = ((size_t)r << lkp) + jg * W;
     if (all) {
         *reinterpret_cast<V *>(y + o) = res;
     } else {
         for (int u = 0; u < W; u++)
             if (col_on(jg * W + u, nc, act)) y[o + u] = vget<VT, W>(res, u);
+    }
+}
+
+/* A Jacobi/L1 sweep of k_rows_il whose result goes straight to a
+   column-major FP64 block (column j at z + j ldz), widened exactly: the
+   last sweep of a cycle hands its result to FCG this way instead of writing
+   the interleaved block and converting it after. */
+template <typename MT, typename VT, int W, int PF>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_jacobi_il_c64(
+    int n, const int *__restrict__ rp, const int *__restrict__ ci, const MT *__restrict__ av,
+    const VT *__restrict__ x, const VT *__restrict__ f, const VT *__restrict__ d,
+    double *__restrict__ z, long ldz, int lkp, int nc, const int *__restrict__ act)
+{
+    const int gpr = (nc + W - 1) / W;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const long t = (long)b * BS + threadIdx.x;
+    const int r = (int)(t / gpr);
+    const int jg = (int)(t % gpr);
+    if (r >= n) return;
+    bool any, all;
+    group_on(jg, W, nc, act, any, all);
+    if (!any) return;
+    const auto res = il_row<MT, VT, OP_JACOBI, W, PF>(r, jg, rp, ci, av, x, f, d, lkp);
+#pragma unroll
+    for (int u = 0; u < W; u++) {
+        const int j = jg * W + u;
+        if (col_on(j, nc, act)) z[(size_t)j * ldz + r] = (double)vget<VT, W>(res, u);
     }
 }
 
@@ -982,6 +1016,27 @@ void rows_il_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x, co
     }
 }
 
+/* the c64 sweep where k_rows_il's vector path runs (no group fallback);
+   false: not launched */
+template <typename MT, typename VT>
+bool jacobi_il_c64_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x, const VT *f,
+                         const VT *d, double *z, long ldz, int lkp, int nc, const int *act)
+{
+    constexpr int WM = 16 / sizeof(VT);
+    if (!stream_ok<VT>(A) || nc < WM) return false;
+    constexpr bool all32 = std::is_same<MT, float>::value && std::is_same<VT, float>::value;
+    const bool pf8 = all32 && A.maxrow > 6 && A.maxrow <= 8;
+    const int n = A.n;
+    const unsigned nb = (unsigned)blocks_for((long long)n * ((nc + WM - 1) / WM));
+    if (pf8)
+        k_jacobi_il_c64<MT, VT, WM, (all32 ? 8 : 6)><<<nb, BS, 0, s>>>(
+            n, A.row.get(), A.col.get(), av, x, f, d, z, ldz, lkp, nc, act);
+    else
+        k_jacobi_il_c64<MT, VT, WM, 6><<<nb, BS, 0, s>>>(n, A.row.get(), A.col.get(), av, x, f,
+                                                         d, z, ldz, lkp, nc, act);
+    return true;
+}
+
 template <typename MT, typename VT>
 void rows_il_op(hipStream_t s, const DevCsr &A, const MT *av, RowOp op, const VT *x, const VT *f,
                 const VT *d, VT *y, int lkp, int nc, const int *act)
@@ -1117,6 +1172,33 @@ void launch_rows_il(hipStream_t s, const DevCsr &A, MatType mt, RowOp op, const 
     BCMG_CHECK_LAUNCH();
 }
 
+template <typename VT>
+bool launch_jacobi_il_c64(hipStream_t s, const DevCsr &A, MatType mt, const VT *x, const VT *f,
+                          const VT *d, double *z, long ldz, int lkp, int nc, const int *act)
+{
+    if (A.n <= 0 || nc <= 0) return true;
+    if (nc > MULTI_MAX_COLS || nc > (1 << lkp))
+        throw Error(BCMG_EINVAL, "interleaved block narrower than its columns");
+    bool done;
+    if constexpr (std::is_same<VT, double>::value) {
+        if (mt != MT_F64) throw Error(BCMG_EINVAL, "FP64 vectors need FP64 matrix values");
+        done = jacobi_il_c64_typed<double, double>(s, A, A.val.get(), x, f, d, z, ldz, lkp, nc,
+                                                   act);
+    } else {
+        if (mt == MT_F32)
+            done = jacobi_il_c64_typed<float, float>(s, A, A.val32.get(), x, f, d, z, ldz, lkp,
+                                                     nc, act);
+        else if (mt == MT_BF16)
+            done = jacobi_il_c64_typed<unsigned short, float>(s, A, A.val16.get(), x, f, d, z,
+                                                              ldz, lkp, nc, act);
+        else
+            done = jacobi_il_c64_typed<double, float>(s, A, A.val.get(), x, f, d, z, ldz, lkp,
+                                                      nc, act);
+    }
+    BCMG_CHECK_LAUNCH();
+    return done;
+}
+
 /* lanes of W = 16 / sizeof(VT) columns when the block is that wide */
 #define IL_VEC_LAUNCH(KERNEL, VT, ...)                                                   \
     do {                                                                                 \
@@ -1205,6 +1287,9 @@ void launch_from_il(hipStream_t s, int n, const S *a, D *b, long ldb, int lkp, i
 }
 
 #define BCMG_INSTANTIATE(VT)                                                                 \
+    template bool launch_jacobi_il_c64<VT>(hipStream_t, const DevCsr &, MatType, const VT *, \
+                                           const VT *, const VT *, double *, long, int, int,  \
+                                           const int *);                                      \
     template void launch_rows_il<VT>(hipStream_t, const DevCsr &, MatType, RowOp, const VT *, \
                                      const VT *, const VT *, VT *, int, int, const int *);    \
     template void launch_jacobi_zero_il<VT>(hipStream_t, int, const VT *, const VT *, VT *,  \
