@@ -47,7 +47,7 @@ SETTINGS = dict(solver_type=0, max_hrc=1, conv_ratio=0.8, aggrsweeps=2, aggrtype
 # x 1.999 + WRITE_SIZE x 1.000, factors from a plain copy kernel of known
 # size; summary: profiles/jacobi_sweep_poisson160_traffic.md). Re-measured
 # whenever the sweep kernel changes; reported only for the 160^3 workload.
-SWEEP_TRAFFIC_BYTES_160 = 509.4e6
+SWEEP_TRAFFIC_BYTES_160 = 507.5e6
 
 
 C2_GRID = 128
@@ -331,7 +331,7 @@ def spmv_bytes(n, nnz):
 
 # Measured HBM bytes per level-0 SpMV launch at 160^3 (same PMC runs and
 # factors as the sweep: profiles/jacobi_sweep_poisson160_traffic.md).
-SPMV_TRAFFIC_BYTES_160 = 439.3e6
+SPMV_TRAFFIC_BYTES_160 = 438.1e6
 MALL_BYTES = 256 << 20
 
 
