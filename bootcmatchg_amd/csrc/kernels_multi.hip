@@ -333,9 +333,13 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_
 /* A Jacobi/L1 sweep of k_rows_il whose result goes straight to a
    column-major FP64 block (column j at z + j ldz), widened exactly: the
    last sweep of a cycle hands its result to FCG this way instead of writing
-   the interleaved block and converting it after. */
-template <typename MT, typename VT, int W, int PF>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_jacobi_il_c64(
+   the interleaved block and converting it after. When the lanes per row
+   divide the block (TILE), the block's rows are whole: the results pass
+   through LDS so that each column's rows leave as one contiguous run;
+   otherwise every lane stores its own columns. No thread leaves before the
+   barrier. */
+template <typename MT, typename VT, int W, int PF, bool TILE>
+__global__ __launch_bounds__(BS) void k_jacobi_il_c64(
     int n, const int *__restrict__ rp, const int *__restrict__ ci, const MT *__restrict__ av,
     const VT *__restrict__ x, const VT *__restrict__ f, const VT *__restrict__ d,
     double *__restrict__ z, long ldz, int lkp, int nc, const int *__restrict__ act)
@@ -345,15 +349,33 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_
     const long t = (long)b * BS + threadIdx.x;
     const int r = (int)(t / gpr);
     const int jg = (int)(t % gpr);
-    if (r >= n) return;
-    bool any, all;
-    group_on(jg, W, nc, act, any, all);
-    if (!any) return;
-    const auto res = il_row<MT, VT, OP_JACOBI, W, PF>(r, jg, rp, ci, av, x, f, d, lkp);
+    bool any = false, all = false;
+    if (r < n) group_on(jg, W, nc, act, any, all);
+    vec_t<VT, W> res{};
+    if (any) res = il_row<MT, VT, OP_JACOBI, W, PF>(r, jg, rp, ci, av, x, f, d, lkp);
+    if constexpr (!TILE) {
+        if (!any) return;
 #pragma unroll
-    for (int u = 0; u < W; u++) {
-        const int j = jg * W + u;
-        if (col_on(j, nc, act)) z[(size_t)j * ldz + r] = (double)vget<VT, W>(res, u);
+        for (int u = 0; u < W; u++) {
+            const int j = jg * W + u;
+            if (col_on(j, nc, act)) z[(size_t)j * ldz + r] = (double)vget<VT, W>(res, u);
+        }
+    } else {
+        /* rows [r0, r0 + rb) of the block; column j of local row q at
+           tile[j * (rb + 1) + q] (the pad spreads a row's columns over the
+           banks) */
+        __shared__ double tile[BS * W + MULTI_MAX_COLS];
+        const int rb = BS / gpr, pitch = rb + 1;
+        const int r0 = (int)((long)b * BS / gpr);
+        const int q = threadIdx.x / gpr;
+#pragma unroll
+        for (int u = 0; u < W; u++) tile[(jg * W + u) * pitch + q] = (double)vget<VT, W>(res, u);
+        __syncthreads();
+        for (int e = threadIdx.x; e < gpr * W * rb; e += BS) {
+            const int j = e / rb, qq = e - j * rb;
+            if (r0 + qq < n && col_on(j, nc, act))
+                z[(size_t)j * ldz + r0 + qq] = tile[j * pitch + qq];
+        }
     }
 }
 
@@ -1028,12 +1050,20 @@ bool jacobi_il_c64_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT 
     const bool pf8 = all32 && A.maxrow > 6 && A.maxrow <= 8;
     const int n = A.n;
     const unsigned nb = (unsigned)blocks_for((long long)n * ((nc + WM - 1) / WM));
-    if (pf8)
-        k_jacobi_il_c64<MT, VT, WM, (all32 ? 8 : 6)><<<nb, BS, 0, s>>>(
+    const int gpr = (nc + WM - 1) / WM;
+    const bool tile = (BS % gpr) == 0;
+    if (pf8 && tile)
+        k_jacobi_il_c64<MT, VT, WM, (all32 ? 8 : 6), true><<<nb, BS, 0, s>>>(
+            n, A.row.get(), A.col.get(), av, x, f, d, z, ldz, lkp, nc, act);
+    else if (pf8)
+        k_jacobi_il_c64<MT, VT, WM, (all32 ? 8 : 6), false><<<nb, BS, 0, s>>>(
+            n, A.row.get(), A.col.get(), av, x, f, d, z, ldz, lkp, nc, act);
+    else if (tile)
+        k_jacobi_il_c64<MT, VT, WM, 6, true><<<nb, BS, 0, s>>>(
             n, A.row.get(), A.col.get(), av, x, f, d, z, ldz, lkp, nc, act);
     else
-        k_jacobi_il_c64<MT, VT, WM, 6><<<nb, BS, 0, s>>>(n, A.row.get(), A.col.get(), av, x, f,
-                                                         d, z, ldz, lkp, nc, act);
+        k_jacobi_il_c64<MT, VT, WM, 6, false><<<nb, BS, 0, s>>>(
+            n, A.row.get(), A.col.get(), av, x, f, d, z, ldz, lkp, nc, act);
     return true;
 }
 
