@@ -75,6 +75,22 @@ def test_multi_equals_single(ctx, name, setting, prec):
     assert len(set(int(v) for v in it if v)) >= 1
 
 
+@pytest.mark.parametrize("prec", [bc.PREC_FP64, bc.PREC_FP32])
+@pytest.mark.parametrize("k", [5, 20, 32])
+def test_multi_column_counts(ctx, k, prec):
+    """An operator above the one-workgroup relaxation's 4096 rows, so that
+    the cycle's last sweep writes FCG's FP64 block itself: 5, 20 and 32
+    columns give column groups that divide a 256-lane block and ones that
+    do not (the results staged through LDS, or stored lane by lane), with
+    columns stopping at different iterations."""
+    A = OPERATORS["poisson3d_17"]()
+    n = len(A[0]) - 1
+    assert n > 4096
+    B = bc.Boot(ctx, bc.Csr(ctx, A), bc.make_params(**SETTINGS["V_L1_2sweeps"]))
+    B.set_precision(prec)
+    check_multi_vs_single(ctx, B, rhs_set(n, k))
+
+
 def test_multi_more_than_32_columns_and_graphs(ctx):
     A = OPERATORS["poisson3d_12"]()
     n = len(A[0]) - 1
