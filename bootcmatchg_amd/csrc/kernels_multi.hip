@@ -552,6 +552,65 @@ __global__ __launch_bounds__(BS) void k_transpose_il(int n, const S *__restrict_
     }
 }
 
+/* column-major FP64 -> interleaved D with 16-byte accesses on both sides
+   (W = 16 / sizeof(D) columns per store; needs 2^lkp >= W): 128 rows per
+   block; a wave reads 128 rows of one column with one 16-byte load per lane
+   (two rows), the tile holds the block's rows, and each lane then writes
+   the W columns of one row with one 16-byte store (per column when the
+   group is partly active or past nc). Inactive columns are neither read nor
+   written, as in k_transpose_il. */
+constexpr int TRV = 128;
+template <typename D>
+__global__ __launch_bounds__(BS) void k_to_il_v(int n, const double *__restrict__ a, long lda,
+                                                D *__restrict__ b, int lkp, int nc,
+                                                const int *__restrict__ act)
+{
+    constexpr int W = 16 / sizeof(D);
+    using V = vec_t<D, W>;
+    __shared__ D tile[TRV * (MULTI_MAX_COLS + 1)];
+    constexpr int pitch = MULTI_MAX_COLS + 1;
+    const int i0 = blockIdx.x * TRV;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j = wv; j < nc; j += BS / 64) {
+        if (!col_on(j, nc, act)) continue;
+        const int i = i0 + 2 * lane;
+        const double *col = a + (size_t)j * lda;
+        double v0 = 0.0, v1 = 0.0;
+        if (i + 1 < n) {
+            const double2 v = *reinterpret_cast<const double2 *>(col + i);
+            v0 = v.x;
+            v1 = v.y;
+        } else if (i < n) {
+            v0 = col[i];
+        }
+        tile[(2 * lane) * pitch + j] = (D)v0;
+        tile[(2 * lane + 1) * pitch + j] = (D)v1;
+    }
+    __syncthreads();
+    const int lg = lkp - (W == 4 ? 2 : 1); /* log2 of the groups per row */
+    for (int e = threadIdx.x; e < (TRV << lg); e += BS) {
+        const int ii = e >> lg, g = e & ((1 << lg) - 1);
+        const int i = i0 + ii;
+        const int j0 = g * W;
+        if (i >= n || j0 >= nc) continue;
+        bool all = j0 + W <= nc;
+        V v;
+#pragma unroll
+        for (int u = 0; u < W; u++) {
+            v[u] = tile[ii * pitch + j0 + u];
+            all = all && col_on(j0 + u, nc, act);
+        }
+        D *dst = b + ((size_t)i << lkp) + j0;
+        if (all) {
+            *reinterpret_cast<V *>(dst) = v;
+        } else {
+#pragma unroll
+            for (int u = 0; u < W; u++)
+                if (col_on(j0 + u, nc, act)) dst[u] = v[u];
+        }
+    }
+}
+
 /* --------------------------------------------------------- K-cycle
    The inner iteration of host_amg.hip's kcycle_inner on interleaved blocks:
    column j keeps its scalars at ks + j KS_STRIDE (the single-column layout)
@@ -1300,8 +1359,13 @@ void launch_to_il(hipStream_t s, int n, const S *a, long lda, D *b, int lkp, int
 {
     if (n <= 0 || nc <= 0) return;
     if (lkp > 5) throw Error(BCMG_EINVAL, "interleaved block wider than 32 columns");
-    k_transpose_il<S, D, true><<<(n + TR_ROWS - 1) / TR_ROWS, BS, 0, s>>>(n, a, b, lda, lkp, nc,
-                                                                          act);
+    constexpr int W = 16 / sizeof(D);
+    if (std::is_same<S, double>::value && (1 << lkp) >= W && lda % 2 == 0)
+        k_to_il_v<D><<<(n + TRV - 1) / TRV, BS, 0, s>>>(n, reinterpret_cast<const double *>(a),
+                                                         lda, b, lkp, nc, act);
+    else
+        k_transpose_il<S, D, true><<<(n + TR_ROWS - 1) / TR_ROWS, BS, 0, s>>>(n, a, b, lda, lkp,
+                                                                              nc, act);
     BCMG_CHECK_LAUNCH();
 }
 
