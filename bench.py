@@ -46,8 +46,9 @@ SETTINGS = dict(solver_type=0, max_hrc=1, conv_ratio=0.8, aggrsweeps=2, aggrtype
 # Measured HBM bytes per level-0 sweep launch at 160^3 (rocprofv3 FETCH_SIZE
 # x 1.999 + WRITE_SIZE x 1.000, factors from a plain copy kernel of known
 # size; summary: profiles/jacobi_sweep_poisson160_traffic.md). Re-measured
-# whenever the sweep kernel changes; reported only for the 160^3 workload.
-SWEEP_TRAFFIC_BYTES_160 = 507.5e6
+# whenever the sweep kernel changes; reported only for the 160^3 workload
+# streamed with 16-bit columns (the library's choice for it).
+SWEEP_TRAFFIC_BYTES_160 = 451.5e6
 
 
 C2_GRID = 128
@@ -318,20 +319,21 @@ def smoothed(bc, ctx, steps, grids=SA_GRIDS):
     return out
 
 
-def sweep_bytes(n, nnz):
+def sweep_bytes(n, nnz, cb=4):
     """Algorithmic HBM bytes of one Jacobi/L1 sweep u' = u + (f - A u) / d:
-    val + col (12 B/nnz), row pointer, u (gathered, counted once), f, d, u'."""
-    return 12 * nnz + 4 * (n + 1) + 4 * 8 * n
+    val + col (8 + cb B/nnz, cb = 4 for 32-bit columns, 2 for the 16-bit
+    column-minus-row copy), row pointer, u (gathered, counted once), f, d, u'."""
+    return (8 + cb) * nnz + 4 * (n + 1) + 4 * 8 * n
 
 
-def spmv_bytes(n, nnz):
+def spmv_bytes(n, nnz, cb=4):
     """y = A x: val + col, row pointer, x (counted once), y."""
-    return 12 * nnz + 4 * (n + 1) + 2 * 8 * n
+    return (8 + cb) * nnz + 4 * (n + 1) + 2 * 8 * n
 
 
 # Measured HBM bytes per level-0 SpMV launch at 160^3 (same PMC runs and
 # factors as the sweep: profiles/jacobi_sweep_poisson160_traffic.md).
-SPMV_TRAFFIC_BYTES_160 = 438.1e6
+SPMV_TRAFFIC_BYTES_160 = 384.6e6
 MALL_BYTES = 256 << 20
 
 
@@ -462,7 +464,8 @@ def main():
     # fine-level SpMV bandwidth (algorithmic bytes / event-timed launch)
     dy = bc.DeviceArray(ctx, n)
     spmv_ms = bc.time_spmv(ctx, dA, db, dy, 50)
-    spmv_gbs = spmv_bytes(n, nnz) / (spmv_ms * 1e-3) / 1e9
+    cb = dA.column_bytes
+    spmv_gbs = spmv_bytes(n, nnz, cb) / (spmv_ms * 1e-3) / 1e9
 
     # whole FCG solve against the HBM roofline (cached hierarchy: no setup)
     sb = solve_bytes([tuple(z) for z in rec[-1]["levels"]], rec[-1]["iterations"],
@@ -473,10 +476,12 @@ def main():
                 "achieved_solves_per_s": cached_rate,
                 "frac": cached_rate * sb / (HBM_PEAK_GBS * 1e9),
                 "note": "FCG with a cached hierarchy; bytes counted the reference "
-                        "algorithm's way (setup excluded, time only)"}
+                        "algorithm's way, 32-bit columns on every level (the library "
+                        "streams 16-bit ones where they fit, so it moves fewer bytes "
+                        "than this count); setup excluded, time only"}
 
     sweep_avg_ms = sm_ms / max(sm_launches, 1)
-    sweep_b = sweep_bytes(n, nnz)
+    sweep_b = sweep_bytes(n, nnz, cb)
     achieved = sweep_b / (sweep_avg_ms * 1e-3) / 1e9
 
     cpu = None
@@ -541,17 +546,17 @@ def main():
         "spmv_gbs": spmv_gbs,
         "spmv_frac_of_hbm": spmv_gbs / HBM_PEAK_GBS,
         "spmv": {"kernel": "level-0 y = A x (k_rows_staged<OP_SPMV>)",
-                 "bytes_per_launch": spmv_bytes(n, nnz), "avg_launch_ms": spmv_ms,
-                 "achieved_gbs": spmv_gbs,
-                 "traffic_bytes": SPMV_TRAFFIC_BYTES_160 if N == GRID else None,
-                 "working_set_bytes": spmv_bytes(n, nnz), "mall_bytes": MALL_BYTES},
+                 "bytes_per_launch": spmv_bytes(n, nnz, cb), "avg_launch_ms": spmv_ms,
+                 "achieved_gbs": spmv_gbs, "column_bytes": cb,
+                 "traffic_bytes": SPMV_TRAFFIC_BYTES_160 if (N == GRID and cb == 2) else None,
+                 "working_set_bytes": spmv_bytes(n, nnz, cb), "mall_bytes": MALL_BYTES},
         "solve_roofline": solve_rl,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": SWEEP_TRAFFIC_BYTES_160 if N == GRID else None,
+                     "traffic": SWEEP_TRAFFIC_BYTES_160 if (N == GRID and cb == 2) else None,
                      "kernel": "level-0 Jacobi/L1 sweep (k_rows_staged<OP_JACOBI, non-temporal>)",
                      "bytes_per_launch": sweep_b, "avg_launch_ms": sweep_avg_ms,
-                     "launches": sm_launches},
+                     "launches": sm_launches, "column_bytes": cb},
         "cpu_baseline": cpu,
         "precision_form": precision,
         "multi_rhs": multi,

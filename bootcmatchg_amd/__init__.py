@@ -232,6 +232,14 @@ class Csr:
         _check(lib().bcmg_csr_info(self.h, ctypes.byref(n), ctypes.byref(nnz)))
         self.n, self.nnz = n.value, nnz.value
 
+    @property
+    def column_bytes(self):
+        """Bytes per column index the streamed row kernels read (2 with the
+        16-bit column-minus-row copy, else 4)."""
+        b = ctypes.c_int()
+        _check(lib().bcmg_csr_column_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
     def download(self):
         row = np.empty(self.n + 1, np.int32)
         col = np.empty(self.nnz, np.int32)

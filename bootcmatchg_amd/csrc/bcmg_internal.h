@@ -126,6 +126,13 @@ struct DevCsr {
     DBuf<unsigned short> val16;  /* bfloat16 bit patterns */
     /* fill val32 (bf16 = false) or val16 (bf16 = true) from val */
     void make_lowp(hipStream_t s, bool bf16);
+    /* narrow copy of the columns, col - row in 16 bits, kept when every entry
+       of a square operator whose FP64 stream (12 B per entry) is read
+       non-temporally fits (banded and stencil orderings); the row kernels
+       read it instead of col whenever the int32-column stream of the value
+       type they use would be non-temporal */
+    DBuf<short> col16;
+    bool narrow = false;
     int group = 8;          /* lanes per row of the row-group kernels */
     int tile = 0;           /* max nnz of a 256-row block; 0 = not analysed, -1 pending */
     int maxrow = 0;         /* longest row (valid with tile) */
@@ -140,6 +147,10 @@ void analyze_rows(Ctx &c, DevCsr &A);
 /* queue the tile computation of A (A.tile = -1 until resolve_tiles) */
 void launch_block_nnz(Ctx &c, DevCsr &A);
 void resolve_tiles(Ctx &c, const std::vector<DevCsr *> &mats);
+/* build the narrow column copy of every square operator in mats above the
+   non-temporal size whose column offsets all fit 16 bits (one round trip
+   for the lot) */
+void make_narrow(Ctx &c, const std::vector<DevCsr *> &mats);
 
 /* ------------------------------------------------------------- context */
 

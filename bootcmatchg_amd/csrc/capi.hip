@@ -368,6 +368,13 @@ int bcmg_csr_info(bcmg_csr A, int *n, int *nnz)
     return BCMG_OK;
 }
 
+int bcmg_csr_column_bytes(bcmg_csr A, int *bytes)
+{
+    if (!A || !bytes) return BCMG_EINVAL;
+    *bytes = A->A.narrow ? 2 : 4;
+    return BCMG_OK;
+}
+
 int bcmg_csr_free(bcmg_csr A)
 {
     if (!A) return BCMG_OK;
@@ -806,6 +813,7 @@ int bcmg_spgemm(bcmg_ctx ctx, int nl, int ml, int mr, const int *lrow, const int
         h->ctx = ctx;
         spgemm(ctx->c, L, R, h->A, slots_per_pass, right_unique);
         resolve_tiles(ctx->c, {&h->A});
+        make_narrow(ctx->c, {&h->A});
     } catch (...) {
         delete h;
         throw;

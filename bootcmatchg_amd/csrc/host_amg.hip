@@ -180,6 +180,9 @@ void adaptive_coarsening(Ctx &c, const bcmg_params &p, BuildState &bs, DevCsr &A
         relax_prepare(c, H.L[level - 1], H.L[level - 1].mat(), p.coarse_solver);
     for (int i = 0; i + 1 < level; i++) pending.push_back(&H.L[i].R);
     resolve_tiles(c, pending);
+    pending.clear();
+    for (int i = 1; i < level; i++) pending.push_back(&H.L[i].Aown);
+    make_narrow(c, pending);
     double cx = 0, cw = 0;
     for (int i = 0; i < level; i++) {
         cx += H.L[i].mat().nnz;

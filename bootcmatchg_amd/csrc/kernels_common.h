@@ -38,58 +38,152 @@ __device__ __forceinline__ T ldm(const T *p)
     return *p;
 }
 
-/* Phase 1 of the staged row-block kernels: the block's entries
-   [e0, e0 + cnt) of col/val are streamed into LDS (lcol, lval) with
-   consecutive lanes on consecutive entries (coalesced, U loads in flight per
-   lane; non-temporal when NT). */
-template <typename MT, bool NT>
-__device__ __forceinline__ void stage_entries(const int *__restrict__ ci,
-                                              const MT *__restrict__ av, int e0, int cnt,
-                                              int *__restrict__ lcol, MT *__restrict__ lval)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+/* Staged LDS layout: element e of a block's source range [e0, e1) sits at
+   LDS index e - stage_base<T>(e0) (the range start rounded down to a 16-byte
+   chunk), so that every whole 16-byte chunk of the source lands on a 16-byte
+   LDS boundary. */
+template <typename T>
+__host__ __device__ __forceinline__ int stage_base(int e0)
 {
-    constexpr int U = 8;
-    for (int k0 = threadIdx.x; k0 < cnt; k0 += BS * U) {
-        int c[U];
-        MT v[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k = k0 + u * BS;
-            if (k < cnt) {
-                c[u] = ldm<NT>(ci + e0 + k);
-                v[u] = ldm<NT>(av + e0 + k);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int k = k0 + u * BS;
-            if (k < cnt) {
-                lcol[k] = c[u];
-                lval[k] = v[u];
-            }
-        }
-    }
+    return e0 & ~(int)(16 / sizeof(T) - 1);
 }
 
-/* Phase 2: one row per thread; the row's staged entries [ka, kb) gather x
-   (consecutive rows gather neighbouring addresses, so the gathers coalesce
-   across the wave), 8 entries in flight, and p = a_ij x_j (rounded) is added
-   left to right from 0 - the serial order. */
-template <typename MT, typename VT>
-__device__ __forceinline__ VT staged_row_sum(const int *__restrict__ lcol,
+/* The part of [e0, e0 + cnt) made of whole 16-byte chunks, [a, b); the
+   partial chunks [e0, a) and [b, e1) hold fewer than 16 / sizeof(T)
+   elements each (thread t < V moves element t of each). */
+template <typename T>
+struct Chunks {
+    static constexpr int V = 16 / sizeof(T);
+    int a0, a, b, e0, e1;
+    __device__ __forceinline__ Chunks(int s, int cnt) : e0(s), e1(s + cnt)
+    {
+        a0 = stage_base<T>(s);
+        a = (s + V - 1) & ~(V - 1);
+        b = e1 & ~(V - 1);
+        if (a > b) a = b = e1;  /* no whole chunk: all of it element-wise */
+    }
+    __device__ __forceinline__ int count() const { return (b - a) / V; }
+    __device__ __forceinline__ bool head() const { return (int)threadIdx.x < a - e0; }
+    __device__ __forceinline__ bool tail() const { return (int)threadIdx.x < e1 - b; }
+};
+
+/* whether the staged kernels copy in 16-byte chunks, else element-wise: the
+   chunks pay on streamed (non-temporal) operators and with 2-byte columns;
+   a cache-resident operator with int columns is latency-bound and the
+   element-wise copy issues less work there */
+template <bool NT, typename IT>
+constexpr bool stage_vec() { return NT || std::is_same<IT, short>::value; }
+
+/* Phase 1 of the staged row-block kernels: the block's entries
+   [e0, e0 + cnt) of col/val are copied into LDS (lcol, lval; layout above).
+   The whole 16-byte chunks move as 16-byte loads and LDS stores, the column
+   and value chunks of the same entries issued together (UV value chunks and
+   the matching column chunks in flight per lane; non-temporal when NT); the
+   partial chunks at either end go element by element, loaded before the
+   chunks and stored after them so that the block waits on one round trip.
+   Every access stays inside [e0, e0 + cnt). The sources must be 16-byte
+   aligned. IT is the stored column type: int (the column) or short (column
+   minus row, the narrow copy of a banded operator). */
+template <typename MT, bool NT, typename IT = int>
+__device__ __forceinline__ void stage_entries(const IT *__restrict__ ci,
+                                              const MT *__restrict__ av, int e0, int cnt,
+                                              IT *__restrict__ lcol, MT *__restrict__ lval)
+{
+    const Chunks<IT> sc(e0, cnt);
+    const Chunks<MT> sv(e0, cnt);
+    const int t = threadIdx.x;
+    if constexpr (!stage_vec<NT, IT>()) {
+        /* element-wise, consecutive lanes on consecutive entries, U in flight */
+        constexpr int U = 8;
+        IT *lc = lcol + (e0 - sc.a0);
+        MT *lv = lval + (e0 - sv.a0);
+        for (int k0 = t; k0 < cnt; k0 += BS * U) {
+            IT c[U];
+            MT v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = k0 + u * BS;
+                if (k < cnt) {
+                    c[u] = ldm<NT>(ci + e0 + k);
+                    v[u] = ldm<NT>(av + e0 + k);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = k0 + u * BS;
+                if (k < cnt) {
+                    lc[k] = c[u];
+                    lv[k] = v[u];
+                }
+            }
+        }
+        return;
+    }
+    constexpr int UV = 4;
+    constexpr int UC = UV * Chunks<MT>::V / Chunks<IT>::V;
+    static_assert(UC >= 1, "column chunks per value chunk");
+    IT ch = 0, ct = 0;
+    MT vh = 0, vt = 0;
+    if (sc.head()) ch = ldm<NT>(ci + e0 + t);
+    if (sc.tail()) ct = ldm<NT>(ci + sc.b + t);
+    if (sv.head()) vh = ldm<NT>(av + e0 + t);
+    if (sv.tail()) vt = ldm<NT>(av + sv.b + t);
+    const int nc = sc.count(), nv = sv.count();
+    const u32x4 *cs = reinterpret_cast<const u32x4 *>(ci + sc.a);
+    const u32x4 *vs = reinterpret_cast<const u32x4 *>(av + sv.a);
+    u32x4 *cd = reinterpret_cast<u32x4 *>(lcol + (sc.a - sc.a0));
+    u32x4 *vd = reinterpret_cast<u32x4 *>(lval + (sv.a - sv.a0));
+    for (int j = 0; j * (BS * UV) < nv || j * (BS * UC) < nc; j++) {
+        const int qv = j * (BS * UV) + t;
+        const int qc = j * (BS * UC) + t;
+        u32x4 c[UC], v[UV];
+#pragma unroll
+        for (int u = 0; u < UC; u++)
+            if (qc + u * BS < nc) c[u] = ldm<NT>(cs + qc + u * BS);
+#pragma unroll
+        for (int u = 0; u < UV; u++)
+            if (qv + u * BS < nv) v[u] = ldm<NT>(vs + qv + u * BS);
+#pragma unroll
+        for (int u = 0; u < UC; u++)
+            if (qc + u * BS < nc) cd[qc + u * BS] = c[u];
+#pragma unroll
+        for (int u = 0; u < UV; u++)
+            if (qv + u * BS < nv) vd[qv + u * BS] = v[u];
+    }
+    if (sc.head()) lcol[e0 - sc.a0 + t] = ch;
+    if (sc.tail()) lcol[sc.b - sc.a0 + t] = ct;
+    if (sv.head()) lval[e0 - sv.a0 + t] = vh;
+    if (sv.tail()) lval[sv.b - sv.a0 + t] = vt;
+}
+
+/* Phase 2: one row per thread; the row's len staged entries (columns from
+   LDS index kc, values from kv) gather x (consecutive rows gather
+   neighbouring addresses, so the gathers coalesce across the wave), 8
+   entries in flight, and p = a_ij x_j (rounded) is added left to right from
+   0 - the serial order. With short columns the stored value is the offset
+   from the thread's row r. */
+template <typename MT, typename VT, typename IT = int>
+__device__ __forceinline__ VT staged_row_sum(const IT *__restrict__ lcol,
                                              const MT *__restrict__ lval,
-                                             const VT *__restrict__ x, int ka, int kb)
+                                             const VT *__restrict__ x, int kc, int kv, int len,
+                                             int r = 0)
 {
     VT t = 0;
     constexpr int PF = 8;
-    for (int k0 = ka; k0 < kb; k0 += PF) {
-        const int kn = kb - k0;
+    for (int k0 = 0; k0 < len; k0 += PF) {
+        const int kn = len - k0;
         int c[PF];
         VT v[PF], g[PF];
 #pragma unroll
         for (int u = 0; u < PF; u++) {
             if (u < kn) {
-                c[u] = lcol[k0 + u];
-                v[u] = mval<VT>(lval[k0 + u]);
+                if constexpr (std::is_same<IT, short>::value)
+                    c[u] = r + (int)lcol[kc + k0 + u];
+                else
+                    c[u] = lcol[kc + k0 + u];
+                v[u] = mval<VT>(lval[kv + k0 + u]);
             }
         }
 #pragma unroll
@@ -102,19 +196,31 @@ __device__ __forceinline__ VT staged_row_sum(const int *__restrict__ lcol,
     return t;
 }
 
-/* LDS of the staged row kernels: the tile's columns (8-byte aligned), then
-   its values */
-inline size_t staged_lds_bytes(int tile, size_t mt_size)
+/* LDS of the staged row kernels: the tile's columns (room for the leading
+   partial chunk, rounded to 16 bytes), then its values */
+__host__ __device__ inline int staged_col_bytes(int tile, int it_size)
 {
-    return ((size_t)(tile + 1) / 2) * 8 + (size_t)tile * mt_size;
+    return ((tile + 16 / it_size) * it_size + 15) & ~15;
 }
 
-template <typename MT>
-__device__ __forceinline__ void staged_lds(int tile, int *&lcol, MT *&lval)
+inline size_t staged_lds_bytes(int tile, size_t mt_size, size_t it_size = 4)
 {
-    extern __shared__ double smem[];
-    lcol = reinterpret_cast<int *>(smem);
-    lval = reinterpret_cast<MT *>(smem + ((tile + 1) >> 1));
+    return (size_t)staged_col_bytes(tile, (int)it_size) +
+           ((size_t)tile + 16 / mt_size) * mt_size;
+}
+
+template <typename MT, typename IT = int>
+__device__ __forceinline__ void staged_lds(int tile, IT *&lcol, MT *&lval)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_staged[];
+    lcol = reinterpret_cast<IT *>(smem_staged);
+    lval = reinterpret_cast<MT *>(smem_staged + staged_col_bytes(tile, (int)sizeof(IT)));
+}
+
+/* the staged kernels read col/val in 16-byte chunks */
+inline bool stage_aligned(const void *a, const void *b)
+{
+    return ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
 }
 
 constexpr int RED_FINAL_BS = 1024; /* threads of the one-block final reductions */

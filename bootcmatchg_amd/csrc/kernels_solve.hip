@@ -32,19 +32,20 @@ constexpr size_t RED_SPMV = 4 * RED_MAX_BLOCKS + 32;  /* SpMV+dots partials */
    matrices), sums the products in column order and applies the epilogue. The
    row's own operands (row pointers, f, d, x) are loaded before phase 1 so
    that their latency overlaps the stream. The sum order (products rounded,
-   then added left to right) is the serial one. */
-template <typename MT, typename VT, int OP, bool NT>
+   then added left to right) is the serial one. IT = short reads the narrow
+   column copy (column minus row): same entries, 2 bytes fewer each. */
+template <typename MT, typename VT, int OP, bool NT, typename IT>
 __global__ __launch_bounds__(BS) void k_rows_staged(int n, const int *__restrict__ rp,
-                                                    const int *__restrict__ ci,
+                                                    const IT *__restrict__ ci,
                                                     const MT *__restrict__ av,
                                                     const VT *__restrict__ x,
                                                     const VT *__restrict__ f,
                                                     const VT *__restrict__ d,
                                                     VT *__restrict__ y, int tile)
 {
-    int *lcol;
+    IT *lcol;
     MT *lval;
-    staged_lds<MT>(tile, lcol, lval);
+    staged_lds<MT, IT>(tile, lcol, lval);
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int r0 = b * BS;
     const int r1 = min(r0 + BS, n);
@@ -63,10 +64,11 @@ __global__ __launch_bounds__(BS) void k_rows_staged(int n, const int *__restrict
     }
     const int e0 = rp[r0];
     const int cnt = rp[r1] - e0;
-    stage_entries<MT, NT>(ci, av, e0, cnt, lcol, lval);
+    stage_entries<MT, NT, IT>(ci, av, e0, cnt, lcol, lval);
     __syncthreads();
     if (!own) return;
-    const VT t = staged_row_sum<MT, VT>(lcol, lval, x, ra - e0, re - e0);
+    const VT t = staged_row_sum<MT, VT, IT>(lcol, lval, x, ra - stage_base<IT>(e0),
+                                            ra - stage_base<MT>(e0), re - ra, r);
     if (OP == OP_SPMV)
         y[r] = t;
     else if (OP == OP_RESID)
@@ -93,6 +95,26 @@ __global__ __launch_bounds__(BS) void k_block_nnz(int n, const int *__restrict__
         part[2 * blockIdx.x] = rp[min(r0 + BS, n)] - rp[r0];
         part[2 * blockIdx.x + 1] = m;
     }
+}
+
+/* narrow column copy: c16[k] = col[k] - row for every entry of row r (one
+   row per thread, setup only); wide[0] is raised when an offset needs more
+   than 16 bits */
+__global__ __launch_bounds__(BS) void k_narrow_cols(int n, const int *__restrict__ rp,
+                                                    const int *__restrict__ ci,
+                                                    short *__restrict__ c16,
+                                                    int *__restrict__ wide)
+{
+    const int r = blockIdx.x * BS + threadIdx.x;
+    if (r >= n) return;
+    int bad = 0;
+    const int k1 = rp[r + 1];
+    for (int k = rp[r]; k < k1; k++) {
+        const int o = ci[k] - r;
+        bad |= (o < -32768) | (o > 32767);
+        c16[k] = (short)o;
+    }
+    if (bad) atomicOr(wide, 1);
 }
 
 /* maxima of the per-block pairs into out[0] (tile), out[1] (longest row) */
@@ -281,9 +303,9 @@ __global__ __launch_bounds__(RED_FINAL_BS) void k_fcg_final(int nb, const double
    alpha = r.z, beta = t.z (K = 2, first step) and gamma = to.z (K = 3),
    finished by k_fcg_final. The products use the z, r, to values the
    epilogue loads anyway, so the separate triple-product pass is saved. */
-template <int K, bool NT>
+template <int K, bool NT, typename IT>
 __global__ __launch_bounds__(BS) void k_spmv_dots(int n, const int *__restrict__ rp,
-                                                  const int *__restrict__ ci,
+                                                  const IT *__restrict__ ci,
                                                   const double *__restrict__ av,
                                                   const double *__restrict__ z,
                                                   double *__restrict__ t,
@@ -291,9 +313,9 @@ __global__ __launch_bounds__(BS) void k_spmv_dots(int n, const int *__restrict__
                                                   const double *__restrict__ to,
                                                   double *__restrict__ part, int tile)
 {
-    int *lcol;
+    IT *lcol;
     double *lval;
-    staged_lds<double>(tile, lcol, lval);
+    staged_lds<double, IT>(tile, lcol, lval);
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int r0 = b * BS;
     const int r1 = min(r0 + BS, n);
@@ -310,13 +332,14 @@ __global__ __launch_bounds__(BS) void k_spmv_dots(int n, const int *__restrict__
     }
     const int e0 = rp[r0];
     const int cnt = rp[r1] - e0;
-    stage_entries<double, NT>(ci, av, e0, cnt, lcol, lval);
+    stage_entries<double, NT, IT>(ci, av, e0, cnt, lcol, lval);
     __syncthreads();
     double sdot[K];
 #pragma unroll
     for (int k = 0; k < K; k++) sdot[k] = 0.0;
     if (own) {
-        const double tr = staged_row_sum<double, double>(lcol, lval, z, ra - e0, re - e0);
+        const double tr = staged_row_sum<double, double, IT>(
+            lcol, lval, z, ra - stage_base<IT>(e0), ra - stage_base<double>(e0), re - ra, r);
         t[r] = tr;
         sdot[0] = rr * zr;
         sdot[1] = tr * zr;
@@ -457,13 +480,30 @@ void rows_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x, const
     const int *rp = A.row.get(), *ci = A.col.get();
     if (stream_ok<VT>(A)) {
         const int nb = (n + BS - 1) / BS;
+        /* the narrow columns where the int32-column stream is non-temporal */
+        const bool use16 = A.narrow && (size_t)A.nnz * (4 + sizeof(MT)) > NT_BYTES;
+        if (!stage_aligned(use16 ? (const void *)A.col16.get() : (const void *)ci, av))
+            throw Error(BCMG_EINVAL, "row kernels: matrix arrays not 16-byte aligned");
+        if (use16) {
+            const short *c16 = A.col16.get();
+            const size_t lds = staged_lds_bytes(A.tile, sizeof(MT), sizeof(short));
+            if (lds > 65536) throw Error(BCMG_EUNSUP, "row tile too large for this value type");
+            if ((size_t)A.nnz * (2 + sizeof(MT)) > NT_BYTES)
+                k_rows_staged<MT, VT, OP, true, short><<<nb, BS, lds, s>>>(n, rp, c16, av, x, f,
+                                                                           d, y, A.tile);
+            else
+                k_rows_staged<MT, VT, OP, false, short><<<nb, BS, lds, s>>>(n, rp, c16, av, x, f,
+                                                                            d, y, A.tile);
+            return;
+        }
         const size_t lds = staged_lds_bytes(A.tile, sizeof(MT));
         if (lds > 65536) throw Error(BCMG_EUNSUP, "row tile too large for this value type");
         if ((size_t)A.nnz * (4 + sizeof(MT)) > NT_BYTES)
-            k_rows_staged<MT, VT, OP, true><<<nb, BS, lds, s>>>(n, rp, ci, av, x, f, d, y, A.tile);
+            k_rows_staged<MT, VT, OP, true, int><<<nb, BS, lds, s>>>(n, rp, ci, av, x, f, d, y,
+                                                                     A.tile);
         else
-            k_rows_staged<MT, VT, OP, false><<<nb, BS, lds, s>>>(n, rp, ci, av, x, f, d, y,
-                                                                  A.tile);
+            k_rows_staged<MT, VT, OP, false, int><<<nb, BS, lds, s>>>(n, rp, ci, av, x, f, d, y,
+                                                                      A.tile);
         return;
     }
     const int G = A.group;
@@ -562,6 +602,34 @@ void analyze_rows(Ctx &c, DevCsr &A)
 {
     launch_block_nnz(c, A);
     resolve_tiles(c, {&A});
+    make_narrow(c, {&A});
+}
+
+
+void make_narrow(Ctx &c, const std::vector<DevCsr *> &mats)
+{
+    std::vector<DevCsr *> cand;
+    for (DevCsr *A : mats)
+        if (A->n == A->m && (size_t)A->nnz * 12 > NT_BYTES && !A->narrow) cand.push_back(A);
+    if (cand.empty()) return;
+    const int nc = (int)cand.size();
+    DBuf<int> wide(nc);
+    k_fill<int><<<(nc + BS - 1) / BS, BS, 0, c.stream>>>(nc, 0, wide.get());
+    for (int k = 0; k < nc; k++) {
+        DevCsr &A = *cand[k];
+        A.col16.alloc(A.nnz);
+        k_narrow_cols<<<(A.n + BS - 1) / BS, BS, 0, c.stream>>>(A.n, A.row.get(), A.col.get(),
+                                                                A.col16.get(), wide.get() + k);
+    }
+    BCMG_CHECK_LAUNCH();
+    std::vector<int> h(nc);
+    BCMG_HIP(hipMemcpyAsync(h.data(), wide.get(), nc * sizeof(int), hipMemcpyDeviceToHost,
+                            c.stream));
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < nc; k++) {
+        cand[k]->narrow = (h[k] == 0);
+        if (!cand[k]->narrow) cand[k]->col16.release();
+    }
 }
 
 void DevCsr::make_lowp(hipStream_t s, bool bf16)
@@ -828,6 +896,29 @@ BCMG_INSTANTIATE_K(double)
 BCMG_INSTANTIATE_K(float)
 #undef BCMG_INSTANTIATE_K
 
+/* the fused FCG SpMV + inner products over the stored column type IT */
+template <typename IT>
+static void spmv_dots_typed(hipStream_t s, int nb, int n, int k, const int *rp, const IT *ci,
+                            const double *av, int nnz, int tile, const double *z, double *t,
+                            const double *r, const double *to, double *part)
+{
+    const size_t lds = staged_lds_bytes(tile, sizeof(double), sizeof(IT));
+    if (!stage_aligned(ci, av))
+        throw Error(BCMG_EINVAL, "SpMV: matrix arrays not 16-byte aligned");
+    const bool nt = (size_t)nnz * (sizeof(IT) + sizeof(double)) > NT_BYTES;
+    if (k == 2) {
+        if (nt)
+            k_spmv_dots<2, true, IT><<<nb, BS, lds, s>>>(n, rp, ci, av, z, t, r, to, part, tile);
+        else
+            k_spmv_dots<2, false, IT><<<nb, BS, lds, s>>>(n, rp, ci, av, z, t, r, to, part, tile);
+    } else {
+        if (nt)
+            k_spmv_dots<3, true, IT><<<nb, BS, lds, s>>>(n, rp, ci, av, z, t, r, to, part, tile);
+        else
+            k_spmv_dots<3, false, IT><<<nb, BS, lds, s>>>(n, rp, ci, av, z, t, r, to, part, tile);
+    }
+}
+
 size_t red_size_for(int n) { return (size_t)RED_SPMV + 3 * (size_t)((n + BS - 1) / BS) + 64; }
 
 void launch_spmv_dots(Ctx &c, const DevCsr &A, int k, const double *z, double *t,
@@ -848,28 +939,19 @@ void launch_spmv_dots(Ctx &c, const DevCsr &A, int k, const double *z, double *t
     if (c.red.n < red_size_for(n))
         throw Error(BCMG_EINVAL, "reduction scratch not sized for this operator");
     const int nb = (n + BS - 1) / BS;
-    const size_t lds = staged_lds_bytes(A.tile, sizeof(double));
     double *part = c.red.get() + RED_SPMV;
-    const bool nt = (size_t)A.nnz * 12 > NT_BYTES;
-    const int *rp = A.row.get(), *ci = A.col.get();
+    const int *rp = A.row.get();
     const double *av = A.val.get();
-    if (k == 2) {
-        if (nt)
-            k_spmv_dots<2, true><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part,
-                                                             A.tile);
-        else
-            k_spmv_dots<2, false><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part,
-                                                             A.tile);
+    if (A.narrow)
+        spmv_dots_typed<short>(c.stream, nb, n, k, rp, A.col16.get(), av, A.nnz, A.tile, z, t, r,
+                               to, part);
+    else
+        spmv_dots_typed<int>(c.stream, nb, n, k, rp, A.col.get(), av, A.nnz, A.tile, z, t, r, to,
+                             part);
+    if (k == 2)
         k_fcg_final<2><<<1, RED_FINAL_BS, 0, c.stream>>>(nb, part, scal);
-    } else {
-        if (nt)
-            k_spmv_dots<3, true><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part,
-                                                             A.tile);
-        else
-            k_spmv_dots<3, false><<<nb, BS, lds, c.stream>>>(n, rp, ci, av, z, t, r, to, part,
-                                                             A.tile);
+    else
         k_fcg_final<3><<<1, RED_FINAL_BS, 0, c.stream>>>(nb, part, scal);
-    }
     BCMG_CHECK_LAUNCH();
 }
 

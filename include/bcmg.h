@@ -91,6 +91,13 @@ int bcmg_csr_upload(bcmg_ctx ctx, int n, int nnz, const int *row, const int *col
                     const double *val, bcmg_csr *out);
 int bcmg_csr_download(bcmg_csr A, int *row, int *col, double *val);
 int bcmg_csr_info(bcmg_csr A, int *n, int *nnz);
+/* Bytes per column index that the FP64 row kernels read for A: 2 when the
+   library keeps a 16-bit copy (column minus row) of a square operator of
+   more than 16.7 M entries (an FP64 stream above 192 MB) whose offsets all
+   fit, else 4; FP32/BF16 value copies use it when their 32-bit-column stream
+   also exceeds 192 MB. No counterpart in the reference (which always streams
+   32-bit columns); it lets a caller price a kernel's memory traffic. */
+int bcmg_csr_column_bytes(bcmg_csr A, int *bytes);
 /* Replaces CSRm::free (src/matrix/CSR.cu:160-181). */
 int bcmg_csr_free(bcmg_csr A);
 

@@ -78,6 +78,83 @@ def test_spmv_long_rows(ctx):
     assert np.all(np.abs(dy.numpy() - orc.spmv(A, x)) <= bound)
 
 
+def _with_far_zeros(A):
+    """A plus explicit zeros at (0, n-1) and (n-1, 0): the same sums up to
+    +-0 terms, but offsets no 16-bit column can hold."""
+    import scipy.sparse as sp
+    n = len(A[0]) - 1
+    r = np.concatenate([np.repeat(np.arange(n), np.diff(A[0])), [0, n - 1]])
+    c = np.concatenate([A[1], [n - 1, 0]])
+    v = np.concatenate([A[2], [0.0, 0.0]])
+    S = sp.csr_matrix((v, (r, c)), shape=(n, n))
+    S.sort_indices()
+    assert S.nnz == len(A[1]) + 2
+    return (S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.astype(np.float64))
+
+
+# entries above which an operator's FP64 stream (12 B each) is non-temporal
+NARROW_NNZ = (192 << 20) // 12
+
+
+def test_narrow_columns_bitexact(ctx):
+    """A square operator whose FP64 stream exceeds 192 MB and whose column
+    offsets fit 16 bits is streamed through its column-minus-row copy: SpMV,
+    residual and Jacobi sweeps give the bits of the 32-bit-column kernels
+    (the same operator with two far explicit zeros), and stay within the row
+    bound of the oracle."""
+    A = ops.poisson3d(140)
+    n = len(A[0]) - 1
+    assert len(A[1]) > NARROW_NNZ
+    Aw = _with_far_zeros(A)
+    dA, dW = bc.Csr(ctx, A), bc.Csr(ctx, Aw)
+    assert dA.column_bytes == 2 and dW.column_bytes == 4
+    rng = np.random.default_rng(11)
+    x = rng.uniform(-1, 1, n)
+    f = rng.uniform(-1, 1, n)
+    dx, df = dev(ctx, x), dev(ctx, f)
+    y1, y2 = bc.DeviceArray(ctx, n), bc.DeviceArray(ctx, n)
+    bc.spmv(ctx, dA, dx, y1)
+    bc.spmv(ctx, dW, dx, y2)
+    assert np.array_equal(bits(y1.numpy()), bits(y2.numpy()))
+    assert np.all(np.abs(y1.numpy() - orc.spmv(A, x)) <= RTOL_ROW * absmv(A, x) + 1e-300)
+    bc.residual(ctx, dA, dx, df, y1)
+    bc.residual(ctx, dW, dx, df, y2)
+    assert np.array_equal(bits(y1.numpy()), bits(y2.numpy()))
+    d = np.abs(ops.to_scipy(A)).sum(axis=1).A1
+    dd = dev(ctx, d)
+    u1, u2 = dev(ctx, x), dev(ctx, x)
+    s1, s2 = bc.DeviceArray(ctx, n), bc.DeviceArray(ctx, n)
+    bc.relax(ctx, dA, dd, df, u1, s1, 3)
+    bc.relax(ctx, dW, dd, df, u2, s2, 3)
+    assert np.array_equal(bits(u1.numpy()), bits(u2.numpy()))
+
+
+@pytest.mark.parametrize("off,narrow", [(32767, True), (-32768, True), (32768, False),
+                                        (-32769, False)])
+def test_narrow_columns_offset_limits(ctx, off, narrow):
+    """The 16-bit copy is kept exactly when every column - row offset lies
+    in [-32768, 32767]: a banded operator above the non-temporal size with
+    one extra diagonal at the given offset."""
+    import scipy.sparse as sp
+    n = 1 << 21
+    band = sp.diags([-np.ones(n - 1), 4.0 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1])
+    far = sp.diags([-0.5 * np.ones(n - abs(off))], [off])
+    wide = sp.diags([-0.25 * np.ones(n - 2000)] * 2, [2000, -2000])
+    S = band + far + wide + 2 * sp.eye(n)
+    for k in (3, 5, 7):
+        S = S + sp.diags([-0.01 * np.ones(n - k)] * 2, [k, -k])
+    S = S.tocsr()
+    S.sort_indices()
+    A = (S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.astype(np.float64))
+    assert S.nnz > NARROW_NNZ
+    dA = bc.Csr(ctx, A)
+    assert dA.column_bytes == (2 if narrow else 4)
+    x = np.random.default_rng(12).uniform(-1, 1, n)
+    y = bc.DeviceArray(ctx, n)
+    bc.spmv(ctx, dA, dev(ctx, x), y)
+    assert np.all(np.abs(y.numpy() - orc.spmv(A, x)) <= RTOL_ROW * absmv(A, x) + 1e-300)
+
+
 @pytest.mark.parametrize("name", ["poisson3d_12", "aniso2d_48", "lognormal3d_14"])
 def test_weights_bitexact(ctx, name):
     A = OPERATORS[name]()
