@@ -48,7 +48,7 @@ SETTINGS = dict(solver_type=0, max_hrc=1, conv_ratio=0.8, aggrsweeps=2, aggrtype
 # size; summary: profiles/jacobi_sweep_poisson160_traffic.md). Re-measured
 # whenever the sweep kernel changes; reported only for the 160^3 workload
 # streamed with 16-bit columns (the library's choice for it).
-SWEEP_TRAFFIC_BYTES_160 = 451.5e6
+SWEEP_TRAFFIC_BYTES_160 = 443.4e6
 
 
 C2_GRID = 128
@@ -333,7 +333,7 @@ def spmv_bytes(n, nnz, cb=4):
 
 # Measured HBM bytes per level-0 SpMV launch at 160^3 (same PMC runs and
 # factors as the sweep: profiles/jacobi_sweep_poisson160_traffic.md).
-SPMV_TRAFFIC_BYTES_160 = 384.6e6
+SPMV_TRAFFIC_BYTES_160 = 375.8e6
 MALL_BYTES = 256 << 20
 
 
