@@ -284,7 +284,10 @@ inline const VT *pvals(const Level &lv)
 /* solve-phase kernels (kernels_solve.hip). VT is the vector/compute type
    (double or float); MatType picks which stored values of a DevCsr a row
    kernel reads. */
-enum RowOp { OP_SPMV = 0, OP_RESID = 1, OP_JACOBI = 2 };
+/* OP_JACOBI_Z: the first two Jacobi sweeps from a zero iterate in one pass
+   (the first sweep's iterate f / d is formed at each gather and for the row
+   itself; x is not read) */
+enum RowOp { OP_SPMV = 0, OP_RESID = 1, OP_JACOBI = 2, OP_JACOBI_Z = 3 };
 template <typename VT>
 void launch_rows(hipStream_t s, const DevCsr &A, MatType mt, RowOp op, const VT *x, const VT *f,
                  const VT *d, VT *y);
@@ -346,6 +349,9 @@ size_t red_size_for(int n);
    column j at j * cs + i * rs), skipped when act[j] == 0 */
 template <typename VT>
 bool relax_block_ok(const DevCsr &A);
+/* whether the first two sweeps from zero run as one OP_JACOBI_Z pass on A */
+template <typename VT>
+bool fuse_zero_sweeps(const DevCsr &A, MatType mt);
 template <typename VT>
 void launch_relax_block(hipStream_t s, const DevCsr &A, MatType mt, const VT *d, const VT *f,
                         const VT *xin, VT *xout, int sweeps, bool zero, int ncols = 1,

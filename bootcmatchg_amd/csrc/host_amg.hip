@@ -220,8 +220,10 @@ struct CycleRun {
 
 /* `sweeps` Jacobi sweeps on level lv (src/AMG/relaxation.cu:211-285),
    ping-pong between st.x and st.x2. A sweep from a known-zero x is f / d
-   (the SpMV of a zero vector is skipped; same values). A small level runs
-   all its sweeps in one workgroup (same values, one launch). */
+   (the SpMV of a zero vector is skipped; same values); when another sweep
+   follows, the two run as one pass that forms f / d where the second sweep
+   reads it (except on a streamed FP64 operator). A small level runs all its
+   sweeps in one workgroup (same values, one launch). */
 template <typename VT>
 void relax_level(Ctx &c, const Level &lv, MatType mt, int type, int sweeps, const VT *f,
                  LevelState<VT> &s, VT *final_out, bool level0)
@@ -245,6 +247,17 @@ void relax_level(Ctx &c, const Level &lv, MatType mt, int type, int sweeps, cons
     int timed = 0;
     for (int k = 0; k < sweeps; k++) {
         VT *out = s.x2;
+        if (s.zero && k + 1 < sweeps && fuse_zero_sweeps<VT>(A, mt)) {
+            /* the sweep from zero and the next one in one pass (same values:
+               the first iterate f / d is formed where the second reads it) */
+            k++;
+            if (final_out && k == sweeps - 1 && s.x != final_out) out = final_out;
+            launch_rows<VT>(c.stream, A, mt, OP_JACOBI_Z, nullptr, f, d, out);
+            s.x2 = s.x;
+            s.x = out;
+            s.zero = false;
+            continue;
+        }
         if (final_out && k == sweeps - 1 && s.x != final_out) out = final_out;
         if (s.zero) {
             launch_jacobi_zero<VT>(c.stream, A.n, f, d, out);

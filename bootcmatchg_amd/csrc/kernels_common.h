@@ -163,12 +163,14 @@ __device__ __forceinline__ void stage_entries(const IT *__restrict__ ci,
    neighbouring addresses, so the gathers coalesce across the wave), 8
    entries in flight, and p = a_ij x_j (rounded) is added left to right from
    0 - the serial order. With short columns the stored value is the offset
-   from the thread's row r. */
-template <typename MT, typename VT, typename IT = int>
+   from the thread's row r. FZ gathers f_j / d_j in place of x_j (the
+   iterate of one sweep from zero, rounded as that sweep rounds it). */
+template <typename MT, typename VT, typename IT = int, bool FZ = false>
 __device__ __forceinline__ VT staged_row_sum(const IT *__restrict__ lcol,
                                              const MT *__restrict__ lval,
                                              const VT *__restrict__ x, int kc, int kv, int len,
-                                             int r = 0)
+                                             int r = 0, const VT *__restrict__ zf = nullptr,
+                                             const VT *__restrict__ zd = nullptr)
 {
     VT t = 0;
     constexpr int PF = 8;
@@ -186,9 +188,22 @@ __device__ __forceinline__ VT staged_row_sum(const IT *__restrict__ lcol,
                 v[u] = mval<VT>(lval[kv + k0 + u]);
             }
         }
+        if constexpr (FZ) {
+            VT gd[PF];
 #pragma unroll
-        for (int u = 0; u < PF; u++)
-            if (u < kn) g[u] = x[c[u]];
+            for (int u = 0; u < PF; u++)
+                if (u < kn) {
+                    g[u] = zf[c[u]];
+                    gd[u] = zd[c[u]];
+                }
+#pragma unroll
+            for (int u = 0; u < PF; u++)
+                if (u < kn) g[u] = g[u] / gd[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < PF; u++)
+                if (u < kn) g[u] = x[c[u]];
+        }
 #pragma unroll
         for (int u = 0; u < PF; u++)
             if (u < kn) t = t + v[u] * g[u];

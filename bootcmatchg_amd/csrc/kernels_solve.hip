@@ -60,6 +60,9 @@ __global__ __launch_bounds__(BS) void k_rows_staged(int n, const int *__restrict
         if (OP == OP_JACOBI) {
             dr = d[r];
             xr = x[r];
+        } else if (OP == OP_JACOBI_Z) {
+            dr = d[r];
+            xr = fr / dr;
         }
     }
     const int e0 = rp[r0];
@@ -67,8 +70,8 @@ __global__ __launch_bounds__(BS) void k_rows_staged(int n, const int *__restrict
     stage_entries<MT, NT, IT>(ci, av, e0, cnt, lcol, lval);
     __syncthreads();
     if (!own) return;
-    const VT t = staged_row_sum<MT, VT, IT>(lcol, lval, x, ra - stage_base<IT>(e0),
-                                            ra - stage_base<MT>(e0), re - ra, r);
+    const VT t = staged_row_sum<MT, VT, IT, OP == OP_JACOBI_Z>(
+        lcol, lval, x, ra - stage_base<IT>(e0), ra - stage_base<MT>(e0), re - ra, r, f, d);
     if (OP == OP_SPMV)
         y[r] = t;
     else if (OP == OP_RESID)
@@ -165,7 +168,11 @@ __global__ __launch_bounds__(BS) void k_rows(int n, const int *__restrict__ rp,
     if (row >= n) return;
     const int j0 = rp[row], j1 = rp[row + 1];
     VT t = 0;
-    for (int j = j0 + lane; j < j1; j += G) t = fma(mval<VT>(av[j]), x[ci[j]], t);
+    for (int j = j0 + lane; j < j1; j += G) {
+        const int c = ci[j];
+        const VT g = (OP == OP_JACOBI_Z) ? f[c] / d[c] : x[c];
+        t = fma(mval<VT>(av[j]), g, t);
+    }
 #pragma unroll
     for (int o = G >> 1; o > 0; o >>= 1) t += __shfl_xor(t, o, G);
     if (lane == 0) {
@@ -173,8 +180,10 @@ __global__ __launch_bounds__(BS) void k_rows(int n, const int *__restrict__ rp,
             y[row] = t;
         else if (OP == OP_RESID)
             y[row] = f[row] - t;
-        else
+        else if (OP == OP_JACOBI)
             y[row] = (f[row] - t) / d[row] + x[row];
+        else
+            y[row] = (f[row] - t) / d[row] + f[row] / d[row];
     }
 }
 
@@ -527,8 +536,10 @@ void rows_op(hipStream_t s, const DevCsr &A, const MT *av, RowOp op, const VT *x
         rows_typed<MT, VT, OP_SPMV>(s, A, av, x, f, d, y);
     else if (op == OP_RESID)
         rows_typed<MT, VT, OP_RESID>(s, A, av, x, f, d, y);
-    else
+    else if (op == OP_JACOBI)
         rows_typed<MT, VT, OP_JACOBI>(s, A, av, x, f, d, y);
+    else
+        rows_typed<MT, VT, OP_JACOBI_Z>(s, A, av, x, f, d, y);
 }
 
 template <typename MT, typename VT>
@@ -670,6 +681,17 @@ void launch_rows(hipStream_t s, const DevCsr &A, RowOp op, const double *x, cons
 {
     launch_rows<double>(s, A, MT_F64, op, x, f, d, y);
 }
+
+template <typename VT>
+bool fuse_zero_sweeps(const DevCsr &A, MatType mt)
+{
+    /* a streamed FP64 operator: the division at every gather costs more
+       than the pass it saves */
+    const size_t vb = mt == MT_F64 ? 8 : mt == MT_F32 ? 4 : 2;
+    return !(std::is_same<VT, double>::value && (size_t)A.nnz * (4 + vb) > NT_BYTES);
+}
+template bool fuse_zero_sweeps<double>(const DevCsr &, MatType);
+template bool fuse_zero_sweeps<float>(const DevCsr &, MatType);
 
 template <typename VT>
 bool relax_block_ok(const DevCsr &A)
