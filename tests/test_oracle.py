@@ -515,3 +515,231 @@ def test_kcycle_is_rejected_in_a_settings_file(tmp_path):
     f.write_text("\n".join(["0", "1", "0.8", "2", "0", "39", "3", "4", "4", "16", "4", "4",
                             "2000", "1e-8"]) + "\n")
     assert orc.params_from_file(str(f))[0] == -2          # ORC_ESETTINGS
+
+
+# ------------------------------------------- H/W cycles, composites, bootstrap
+#
+# An independent restatement (scipy.sparse, not the oracle's code) of the
+# recursion of src/SOLVER/GAMG_cycle.cu:25-140 with the grid-sweep counts of
+# src/AMG/AMG.cu:371-394, of the composites of src/SOLVER/FCG.cu:112-187 and of
+# the bootstrap inner iterations of src/BOOT/bootstrap.cu:5-44. Only the level
+# operators and prolongators are taken from the oracle (they are pinned above
+# against scipy's Galerkin product); smoother divisors are recomputed here.
+# Each restatement takes a named slip, and every test checks that the slipped
+# variant disagrees with the oracle, so a plausible misreading shared by the
+# oracle and the GPU library would fail it.
+
+class SparseHierarchy:
+    """Level operators A_l, prolongators P_l, diagonals and L1 row sums."""
+
+    def __init__(self, B, k):
+        nl = B.levels(k)
+        self.A = [ops.to_scipy(B.level_A(k, l)).tocsr() for l in range(nl)]
+        self.P = []
+        for l in range(nl - 1):
+            agg, p = B.level_P(k, l)
+            nf, nc = self.A[l].shape[0], self.A[l + 1].shape[0]
+            self.P.append(sp.csr_matrix((p, (np.arange(nf), agg)), shape=(nf, nc)))
+        self.D = [A.diagonal() for A in self.A]
+        self.M = [np.asarray(abs(A).sum(axis=1)).ravel() for A in self.A]
+
+    @property
+    def levels(self):
+        return len(self.A)
+
+
+def visits(cycle_type, l, L, slip=None):
+    """How often level l + 1 (0-based) is visited from level l of L levels.
+    V: once. W: twice, but the level above the coarsest visits it once (the
+    break of GAMG_cycle.cu:107-108). H: twice from even 0-based levels
+    (AMG.cu:381-386), with the same break."""
+    g = 1
+    if cycle_type == 2:
+        g = 2
+    elif cycle_type == 1:
+        g = 2 if l % 2 == (1 if slip == "h_odd_levels" else 0) else 1
+    if l + 1 == L - 1 and slip != "no_coarsest_break":
+        g = 1
+    return g
+
+
+def sparse_cycle(H, f, x, s, l=0, slip=None):
+    """One cycle from level l: pre-smoothing, restricted residual, recursion,
+    prolongation, post-smoothing; the coarsest level gets relaxnumber_coarse
+    sweeps of the coarse smoother. s: cycle_type, pre, post, coarse,
+    relax_type, coarse_solver."""
+    L = H.levels
+
+    def relax(l, f, x, k, t):
+        d = H.M[l] if t == 4 else H.D[l]
+        for _ in range(k):
+            x = x + (f - H.A[l] @ x) / d
+        return x
+
+    if l == L - 1:
+        return relax(l, f, x, s["coarse"], s["coarse_solver"])
+    x = relax(l, f, x, s["pre"], s["relax_type"])
+    fc = H.P[l].T @ (f - H.A[l] @ x)
+    xc = np.zeros(len(fc))
+    for _ in range(visits(s["cycle_type"], l, L, slip)):
+        xc = sparse_cycle(H, fc, xc, s, l + 1, slip)
+    x = x + H.P[l] @ xc
+    return relax(l, f, x, s["post"], s["relax_type"])
+
+
+def sparse_composite(Hs, r, x, s, solver_type, slip=None):
+    """Composite of hierarchies: 0 multiplicative, 1 symmetrized (forward,
+    then backward), 2 additive (mean of the cycles, each from the input x)."""
+    run = lambda k, x: sparse_cycle(Hs[k], r, x, s)
+    order = list(range(len(Hs)))
+    if solver_type == 0:
+        for k in (order[::-1] if slip == "reversed" else order):
+            x = run(k, x)
+        return x
+    if solver_type == 1:
+        for k in order + (order if slip == "forward_twice" else order[::-1]):
+            x = run(k, x)
+        return x
+    acc = np.zeros(len(r))
+    xin = x
+    for k in order:
+        xk = run(k, xin if slip != "chained" else x)
+        x = xk
+        acc = acc + xk
+    return acc / (1 if slip == "sum" else len(Hs))
+
+
+def cycle_settings(prm):
+    return dict(cycle_type=prm.cycle_type, pre=prm.prerelax_sweeps, post=prm.postrelax_sweeps,
+                coarse=prm.relaxnumber_coarse, relax_type=prm.relax_type,
+                coarse_solver=prm.coarse_solver)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+DEEP = lambda: ops.lognormal3d(16, 4.0, seed=11)        # 5 levels with one sweep
+
+
+FOUR = lambda: ops.lognormal3d(12, 4.0, seed=3)         # 4 levels with one sweep
+
+
+@pytest.mark.parametrize("cycle_type,relax_type,coarse_solver,slips,op", [
+    (1, 4, 4, ["h_odd_levels"], DEEP),
+    (1, 0, 4, ["h_odd_levels", "no_coarsest_break"], FOUR),
+    (1, 4, 0, ["h_odd_levels", "no_coarsest_break"], FOUR),
+    (2, 4, 0, ["no_coarsest_break"], DEEP),
+    (2, 0, 0, ["no_coarsest_break"], FOUR),
+    (0, 4, 4, [], DEEP),
+])
+def test_hw_cycles_match_restatement(cycle_type, relax_type, coarse_solver, slips, op):
+    A = op()
+    n = len(A[0]) - 1
+    prm = orc.params(aggrsweeps=1, cycle_type=cycle_type, relax_type=relax_type,
+                     coarse_solver=coarse_solver, prerelax_sweeps=2, postrelax_sweeps=1,
+                     relaxnumber_coarse=6)
+    B = orc.Boot(A, prm)
+    assert B.rc == 0 and B.levels(0) >= 4
+    H = SparseHierarchy(B, 0)
+    L = H.levels
+    for slip in slips:            # every named slip changes the visit schedule here
+        assert any(visits(cycle_type, l, L) != visits(cycle_type, l, L, slip)
+                   for l in range(L - 1)), slip
+    s = cycle_settings(prm)
+    rng = np.random.default_rng(cycle_type * 10 + relax_type)
+    r, x0 = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    for xin in (np.zeros(n), x0):
+        z = B.precond(r, xin)
+        assert rel(z, sparse_cycle(H, r, xin, s)) <= 1e-12
+        for slip in slips:
+            assert rel(z, sparse_cycle(H, r, xin, s, slip=slip)) > 1e-6, slip
+
+
+def test_w_cycle_visit_counts_on_five_levels():
+    """The restatement's schedule against the textbook counts: with 5 levels a
+    W-cycle visits level l (0-based) 2^l times, except the coarsest, visited
+    as often as the level above it: 1, 2, 4, 8, 8; the H-cycle doubles from
+    levels 0 and 2 only: 1, 2, 2, 4, 4."""
+    L = 5
+    counts = [1]
+    for l in range(L - 1):
+        counts.append(counts[-1] * visits(2, l, L))
+    assert counts == [1, 2, 4, 8, 8]
+    counts = [1]
+    for l in range(L - 1):
+        counts.append(counts[-1] * visits(1, l, L))
+    assert counts == [1, 2, 2, 4, 4]                   # H-cycle: even levels double
+
+
+@pytest.mark.parametrize("solver_type,slips", [
+    (0, ["reversed"]),
+    (1, ["forward_twice"]),
+    (2, ["chained", "sum"]),
+])
+@pytest.mark.parametrize("cycle_type", [0, 2])
+def test_three_hierarchy_composites_match_restatement(solver_type, slips, cycle_type):
+    A = DEEP()
+    n = len(A[0]) - 1
+    prm = orc.params(solver_type=solver_type, max_hrc=3, conv_ratio=1e-6, aggrsweeps=1,
+                     cycle_type=cycle_type, prerelax_sweeps=2, postrelax_sweeps=2,
+                     relaxnumber_coarse=8)
+    B = orc.Boot(A, prm)
+    assert B.rc == 0 and B.info()[0] == 3
+    Hs = [SparseHierarchy(B, k) for k in range(3)]
+    assert all(H.levels >= 4 for H in Hs)
+    # hierarchy 2 is built from the bootstrapped w, so it differs from 0
+    assert not np.array_equal(Hs[2].P[0].data, Hs[0].P[0].data)
+    s = cycle_settings(prm)
+    rng = np.random.default_rng(solver_type)
+    r, x0 = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    for xin in (np.zeros(n), x0):
+        z = B.precond(r, xin)
+        assert rel(z, sparse_composite(Hs, r, xin, s, solver_type)) <= 1e-12
+        for slip in slips:
+            assert rel(z, sparse_composite(Hs, r, xin, s, solver_type, slip)) > 1e-6, slip
+
+
+def inner_iterations(Hs, A, s, solver_type, slip=None):
+    """x = 1; 15 times x <- B(0; x); ratio ||x_15||_A / ||x_14||_A; w = x / ||x||_A."""
+    n = A.shape[0]
+    norm = (lambda v: np.linalg.norm(v)) if slip == "two_norm" else (lambda v: np.sqrt(v @ (A @ v)))
+    x = np.ones(n)
+    old = norm(x)
+    ratio = 1.0
+    for _ in range(14 if slip == "fourteen" else 15):
+        x = sparse_composite(Hs, np.zeros(n), x, s, solver_type)
+        new = norm(x)
+        ratio, old = new / old, new
+    scale = np.linalg.norm(x) if slip == "two_norm_scale" else np.sqrt(x @ (A @ x))
+    return x / scale, ratio
+
+
+@pytest.mark.parametrize("solver_type", [0, 1, 2])
+def test_bootstrap_inner_iterations_match_restatement(solver_type):
+    """w(2) (the vector hierarchy 2 is built from) comes from the composite of
+    hierarchies 0 and 1; the final estimated ratio from that of all three."""
+    A = DEEP()
+    S = ops.to_scipy(A)
+    prm = orc.params(solver_type=solver_type, max_hrc=3, conv_ratio=1e-6, aggrsweeps=1,
+                     prerelax_sweeps=2, postrelax_sweeps=2, relaxnumber_coarse=8)
+    B = orc.Boot(A, prm)
+    assert B.rc == 0 and B.info()[0] == 3
+    Hs = [SparseHierarchy(B, k) for k in range(3)]
+    s = cycle_settings(prm)
+    n = S.shape[0]
+    assert np.array_equal(B.w(0), np.ones(n)) and np.array_equal(B.w(1), np.ones(n))
+    w2, _ = inner_iterations(Hs[:2], S, s, solver_type)
+    assert rel(B.w(2), w2) <= 1e-13
+    assert abs(np.sqrt(w2 @ (S @ w2)) - 1.0) <= 1e-12
+    _, ratio = inner_iterations(Hs, S, s, solver_type)
+    est = B.info()[1]
+    assert abs(est - ratio) <= 1e-13 * ratio
+    # a slip must miss by ten times the tolerance the restatement meets (one
+    # application fewer moves the power iteration's ratio by 7e-12 to 4e-5)
+    for slip in ("two_norm", "fourteen"):
+        _, rs = inner_iterations(Hs, S, s, solver_type, slip)
+        assert abs(est - rs) > 1e-12 * ratio, slip
+    for slip in ("two_norm_scale", "fourteen"):
+        ws, _ = inner_iterations(Hs[:2], S, s, solver_type, slip)
+        assert rel(B.w(2), ws) > 1e-12, slip
