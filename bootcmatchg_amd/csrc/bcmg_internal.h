@@ -57,6 +57,9 @@ struct Pool {
     std::multimap<size_t, void *> free_;
     std::unordered_map<void *, size_t> cls_;
     size_t cached = 0, live = 0;
+    /* while set, released blocks are freed at once instead of cached (the
+       serving loop gives a closed lane's memory back to the device) */
+    bool free_released = false;
     static size_t size_class(size_t bytes);
     void *alloc(size_t bytes);
     void release(void *p);
@@ -560,9 +563,11 @@ void launch_abs_rowsum(hipStream_t s, const DevCsr &A, double *M);
 void match_weights(Ctx &c, const DevCsr &A, const double *w, double *W);
 /* suitor + symmetrisation: Msym (and M raw); returns number of non-mutual v */
 int match_suitor(Ctx &c, const DevCsr &A, const double *W, int *M, int *Msym, bool want_count);
-/* canonical pairwise aggregation; returns n_c. members[2*I], [2*I+1] (or -1) */
+/* canonical pairwise aggregation; returns n_c. members[2*I], [2*I+1] (or -1).
+   validate: check that Msym is a symmetric matching first (BCMG_EINVAL if
+   not) - for matchings that do not come from match_suitor */
 int aggregate_pairs(Ctx &c, int n, const int *Msym, const double *w, int *agg, double *p,
-                    int *members);
+                    int *members, bool validate = false);
 /* A_c = R (A P) in the fixed order, pairwise P given by (agg, p, members) */
 void galerkin_pairwise(Ctx &c, const DevCsr &A, const int *agg, const double *p,
                        const int *members, int nc, DevCsr &Ac);

@@ -745,7 +745,7 @@ int bcmg_aggregate(bcmg_ctx ctx, int n, const int *M_sym, const double *w, int *
     API_BEGIN
     CtxScope scope_(ctx);
     DBuf<int> members(2 * (size_t)n);
-    *n_coarse = aggregate_pairs(ctx->c, n, M_sym, w, agg, p, members.get());
+    *n_coarse = aggregate_pairs(ctx->c, n, M_sym, w, agg, p, members.get(), true);
     API_END
 }
 

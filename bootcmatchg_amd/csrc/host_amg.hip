@@ -242,12 +242,14 @@ void relax_level(Ctx &c, const Level &lv, MatType mt, int type, int sweeps, cons
         s.zero = false;
         return;
     }
-    /* timed: one event pair around the run of consecutive sweep launches */
+    /* timed: one event pair around the run of consecutive sweep launches
+       (the first two sweeps from zero are then not fused, so that every
+       sweep of the level falls inside the pair or is the zero sweep) */
     hipEvent_t e1 = nullptr;
     int timed = 0;
     for (int k = 0; k < sweeps; k++) {
         VT *out = s.x2;
-        if (s.zero && k + 1 < sweeps && fuse_zero_sweeps<VT>(A, mt)) {
+        if (s.zero && k + 1 < sweeps && !timing && fuse_zero_sweeps<VT>(A, mt)) {
             /* the sweep from zero and the next one in one pass (same values:
                the first iterate f / d is formed where the second reads it) */
             k++;

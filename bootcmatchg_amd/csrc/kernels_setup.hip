@@ -420,13 +420,22 @@ __device__ __forceinline__ bool good_pair(int v, int u, const double *__restrict
     return *nrm > DBL_EPSILON;
 }
 
+/* err (optional, for matchings from outside the library): raised when v's
+   partner is out of range, v itself, or not matched back to v; such a v
+   leads nothing and w[u] is not read */
 __global__ __launch_bounds__(BS) void k_agg_leader(int n, const int *__restrict__ Ms,
                                                    const double *__restrict__ w,
-                                                   int *__restrict__ flag)
+                                                   int *__restrict__ flag,
+                                                   int *__restrict__ err)
 {
     const int v = blockIdx.x * BS + threadIdx.x;
     if (v >= n) return;
     const int u = Ms[v];
+    if (err && (u >= n || u == v || (u >= 0 && Ms[u] != v))) {
+        atomicOr(err, 1);
+        flag[v] = 0;
+        return;
+    }
     double nrm;
     const bool good = good_pair(v, u, w, &nrm);
     flag[v] = (!good || v < u) ? 1 : 0;
@@ -823,11 +832,23 @@ int match_suitor(Ctx &c, const DevCsr &A, const double *W, int *M, int *Msym, bo
 }
 
 int aggregate_pairs(Ctx &c, int n, const int *Msym, const double *w, int *agg, double *p,
-                    int *members)
+                    int *members, bool validate)
 {
-    DBuf<int> flag(n), ids(n + 1);
-    k_agg_leader<<<blocks_for(n), BS, 0, c.stream>>>(n, Msym, w, flag.get());
+    DBuf<int> flag(n), ids(n + 1), err;
+    if (validate) {
+        err.alloc(1);
+        BCMG_HIP(hipMemsetAsync(err.get(), 0, sizeof(int), c.stream));
+    }
+    k_agg_leader<<<blocks_for(n), BS, 0, c.stream>>>(n, Msym, w, flag.get(), err.get());
     BCMG_CHECK_LAUNCH();
+    if (validate) {
+        BCMG_HIP(hipMemcpyAsync(c.h_ints, err.get(), sizeof(int), hipMemcpyDeviceToHost,
+                                c.stream));
+        BCMG_HIP(hipStreamSynchronize(c.stream));
+        if (c.h_ints[0] != 0)
+            throw Error(BCMG_EINVAL, "aggregation: M_sym is not a symmetric matching (a "
+                                     "partner out of range, the row itself, or not matched back)");
+    }
     const long long nc = exclusive_scan(c, flag.get(), ids.get(), n);
     k_agg_assign<<<blocks_for(n), BS, 0, c.stream>>>(n, Msym, w, flag.get(), ids.get(), agg, p,
                                                       members);

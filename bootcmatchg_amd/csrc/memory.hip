@@ -58,6 +58,10 @@ void Pool::release(void *p)
     const size_t cls = it->second;
     cls_.erase(it);
     live -= cls;
+    if (free_released) {
+        (void)hipFree(p);
+        return;
+    }
     free_.emplace(cls, p);
     cached += cls;
 }

@@ -12,12 +12,17 @@
  *   3. admits queued requests into free slots - requests that arrived while
  *      others were on the device join those lanes at an iteration boundary,
  *      and a lane is opened for an operator only while the device memory it
- *      needs fits the budget (idle lanes are closed to make room),
+ *      needs fits the budget (idle lanes are closed, and their memory freed,
+ *      to make room; requests of an operator whose one-slot lane exceeds the
+ *      whole budget fail with BCMG_ENOMEM),
  *   4. queues its next iteration (a captured graph) at once; the lanes'
  *      streams run concurrently on the device while the host turns to the
  *      next lane.
  * A slot computes exactly what a single solve of its right-hand side
  * computes, so a result does not depend on what else is being served.
+ * A lane reads a request's b only after the work queued on the context's
+ * stream before the step (an event the lane streams wait on), so a caller
+ * may write b asynchronously on that stream right before submitting.
  */
 #include "bcmg_internal.h"
 
@@ -74,14 +79,31 @@ struct Server {
     Ctx &main;
     size_t budget;
     int max_slots;
+    hipEvent_t ev_main = nullptr;   /* end of the context stream's work at a step */
     std::vector<Boot *> ops;
     std::vector<std::deque<Request>> queue;
     std::vector<std::vector<std::unique_ptr<Lane>>> lanes; /* per operator */
     std::unordered_map<long long, ServerResult> results;
     std::vector<long long> completed_ids; /* finished since the last drain */
-    size_t in_use = 0;
+    size_t in_use = 0;              /* device bytes held by open lanes (pool classes) */
     long long next_id = 1;
-    Server(Ctx &c, size_t b, int k) : main(c), budget(b), max_slots(k) {}
+    Server(Ctx &c, size_t b, int k) : main(c), budget(b), max_slots(k)
+    {
+        BCMG_HIP(hipEventCreateWithFlags(&ev_main, hipEventDisableTiming));
+    }
+    ~Server()
+    {
+        lanes.clear();
+        if (ev_main) (void)hipEventDestroy(ev_main);
+    }
+
+    /* an upper bound of the device bytes a k-slot lane of B takes: its
+       buffers' sizes, each rounded up to the pool's size class (less than a
+       quarter more, and at most 255 bytes more for blocks up to 4 KiB) */
+    static size_t lane_bound(const Boot &B, int k)
+    {
+        return multi_work_bytes(B, k) / 4 * 5 + ((size_t)64 << 10);
+    }
 
     int add(Boot *B)
     {
@@ -140,8 +162,27 @@ struct Server {
         }
         if (vo < 0) return false;
         in_use -= lanes[vo][vl]->bytes;
-        lanes[vo].erase(lanes[vo].begin() + vl);
+        /* the lane's blocks go back to the device, not to the pool's cache
+           (the pool reuses a block only for its own size class, so cached
+           blocks of closed lanes would add up past the budget) */
+        Pool *pool = current_pool();
+        const bool prev = pool ? pool->free_released : false;
+        if (pool) pool->free_released = true;
+        try {
+            lanes[vo].erase(lanes[vo].begin() + vl);
+        } catch (...) {
+            if (pool) pool->free_released = prev;
+            throw;
+        }
+        if (pool) pool->free_released = prev;
         return true;
+    }
+
+    /* device bytes the calling thread's pool holds in live blocks */
+    static size_t pool_live()
+    {
+        const Pool *pool = current_pool();
+        return pool ? pool->live : 0;
     }
 
     /* open a lane for op (up to its slot limit, within the memory budget,
@@ -152,7 +193,7 @@ struct Server {
         Boot &B = *ops[op];
         const int want = max_slots - slots_of(op);
         if (want < 1) return nullptr;
-        auto fits = [&](int k) { return in_use + multi_work_bytes(B, k) <= budget; };
+        auto fits = [&](int k) { return in_use + lane_bound(B, k) <= budget; };
         while (!fits(1))
             if (!evict_one(op)) return nullptr;
         int k = want;
@@ -165,10 +206,11 @@ struct Server {
         L->c.use_graphs = main.use_graphs;
         BCMG_HIP(hipStreamCreateWithFlags(&L->c.stream, hipStreamNonBlocking));
         L->B = &B;
+        const size_t live0 = pool_live();
         multi_work(L->c, B, L->M, k);
         L->E = std::make_unique<SlotEngine>(L->c, B, *L->M);
         L->slot_req.assign(k, Request());
-        L->bytes = multi_work_bytes(B, k);
+        L->bytes = pool_live() - live0;
         in_use += L->bytes;
         lanes[op].push_back(std::move(L));
         return lanes[op].back().get();
@@ -182,21 +224,49 @@ struct Server {
     int step()
     {
         int completed = 0;
+        /* b of a request may come from work the caller queued on the
+           context's stream before this step: lanes admit after it */
+        BCMG_HIP(hipEventRecord(ev_main, main.stream));
         for (size_t op = 0; op < ops.size(); op++) {
             auto &q = queue[op];
             for (auto &L : lanes[op]) {
                 completed += finish(*L);
-                if (!q.empty()) admit(*L, q);
+                if (!q.empty()) {
+                    BCMG_HIP(hipStreamWaitEvent(L->c.stream, ev_main, 0));
+                    admit(*L, q);
+                }
                 L->E->launch();
             }
             while (!q.empty()) {
                 Lane *L = open_lane((int)op);
-                if (!L) break;
+                if (!L) {
+                    /* a one-slot lane larger than the whole budget never fits */
+                    if (lane_bound(*ops[op], 1) > budget) completed += fail_queued((int)op);
+                    break;
+                }
+                BCMG_HIP(hipStreamWaitEvent(L->c.stream, ev_main, 0));
                 admit(*L, q);
                 L->E->launch();
             }
         }
         return completed;
+    }
+
+    /* every queued request of op ends with BCMG_ENOMEM */
+    int fail_queued(int op)
+    {
+        const double t = now_ms();
+        int k = 0;
+        for (const Request &r : queue[op]) {
+            ServerResult &res = results[r.id];
+            res.done = true;
+            res.status = BCMG_ENOMEM;
+            res.latency_ms = t - r.t_submit;
+            completed_ids.push_back(r.id);
+            k++;
+        }
+        queue[op].clear();
+        return k;
     }
 
     /* wait, collect, retire (results recorded after the copies of x) */
@@ -245,7 +315,11 @@ struct Server {
         if (L.B->prec != BCMG_PREC_FP64 && L.M->prec != L.B->prec) {
             /* reduced-precision buffers are allocated now (see open_lane) */
             BCMG_HIP(hipStreamSynchronize(main.stream));
+            const size_t live0 = pool_live();
             ensure_multi_into(L.c, *L.B, *L.M, L.M->kmax);
+            const size_t grown = pool_live() - live0;
+            L.bytes += grown;
+            in_use += grown;
         }
         std::vector<int> js;
         std::vector<const double *> bs;

@@ -43,3 +43,27 @@ SETTINGS = {
                             coarse_solver=4, relax_type=4, relaxnumber_coarse=16,
                             prerelax_sweeps=2, postrelax_sweeps=2, itnlim=2000, rtol=1e-8),
 }
+
+
+# FCG on the GPU against the oracle's FCG (both FP64). The kernels sum rows in
+# another order than the oracle (fused multiply-adds, lane-strided partial
+# sums), so the two runs' iterates differ by rounding only: with equal
+# iteration counts the solutions agree to SOL_RTOL (observed: 1e-16 to 1e-15).
+# The counts may differ by up to 2 (the reference's own bar, enforced against
+# the oracle's count); then both runs stopped at rtol, and the GPU solution's
+# true residual is checked against rtol instead.
+SOL_RTOL = 1e-12
+
+
+def check_fcg_vs_oracle(x, it, xo, it_o, A, b, rtol, label=""):
+    rel = float(np.linalg.norm(x - xo) / np.linalg.norm(xo))
+    print("%s: iterations %d (oracle %d), relative difference to the oracle %.2e"
+          % (label, it, it_o, rel), flush=True)
+    assert abs(it - it_o) <= 2, (label, it, it_o)
+    if it == it_o:
+        assert rel <= SOL_RTOL, (label, rel)
+    else:
+        S = ops.to_scipy(A)
+        tr = np.linalg.norm(b - S @ x) / np.linalg.norm(b)
+        assert tr <= rtol * (1 + 1e-6), (label, tr)
+    return rel

@@ -3,7 +3,8 @@
 - 2x2, a diagonal-only operator (no edge to match: every aggregate is a
   singleton), two disconnected blocks, and an operator already below the
   coarse-size limit: hierarchies bit-exact, FCG iterations within +-2,
-  solutions within 1e-5 of the oracle's, also through the multi-column path;
+  solutions within 1e-12 of the oracle's when the counts agree, also through
+  the multi-column path;
 - 1x1: the first FCG step is exact, so the loop pass FCG v3 always makes
   (src/SOLVER/FCG.cu:498-565) divides 0 by 0 (the reference returns NaN);
   both sides report the breakdown instead.
@@ -16,7 +17,7 @@ import bootcmatchg_amd as bc
 from bootcmatchg_amd import operators as ops
 from oracle import oracle as orc
 
-from common import SETTINGS, bits
+from common import SETTINGS, bits, check_fcg_vs_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -73,8 +74,7 @@ def test_degenerate_operators(ctx, name, setting):
     dx = bc.DeviceArray(ctx, n)
     db = bc.DeviceArray.from_numpy(ctx, b)
     it, _ = B.fcg(db, dx)
-    assert abs(it - res_o.iterations) <= 2, (it, res_o.iterations)
-    assert np.linalg.norm(dx.numpy() - xo) <= 1e-5 * np.linalg.norm(xo)
+    check_fcg_vs_oracle(dx.numpy(), it, xo, res_o.iterations, A, b, prm["rtol"], name)
     # the same operator through the multi-column path: bit-identical columns
     xs = [bc.DeviceArray(ctx, n) for _ in range(2)]
     its, _, st = B.fcg_multi([db, db], xs)

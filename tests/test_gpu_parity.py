@@ -13,7 +13,7 @@ import bootcmatchg_amd as bc
 from bootcmatchg_amd import operators as ops
 from oracle import oracle as orc
 
-from common import OPERATORS, SETTINGS, bits
+from common import OPERATORS, SETTINGS, bits, check_fcg_vs_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -216,6 +216,28 @@ def _compare_hierarchy(B, O, k):
             assert np.array_equal(bits(pg), bits(po)), (k, l)
 
 
+def test_aggregate_rejects_an_invalid_matching(ctx):
+    """bcmg_aggregate takes a caller's matching: a partner out of range, a
+    row matched to itself or a partner not matched back is BCMG_EINVAL,
+    raised before any aggregate is written."""
+    n = 8
+    w = dev(ctx, np.ones(n))
+    good = np.array([1, 0, 3, 2, -1, 6, 5, -1], dtype=np.int32)
+    bad = {"out_of_range": [1, 0, 3, 2, -1, 6, 5, 8], "self": [1, 0, 2, -1, -1, 6, 5, -1],
+           "not_mutual": [1, 0, 3, 2, 3, 6, 5, -1]}
+    agg = bc.DeviceArray(ctx, n, np.int32)
+    p = bc.DeviceArray(ctx, n)
+    assert bc.aggregate(ctx, n, dev(ctx, good), w, agg, p) == 5
+    assert list(agg.numpy()) == [0, 0, 1, 1, 2, 3, 3, 4]
+    for name, M in bad.items():
+        sentinel = np.full(n, -7, dtype=np.int32)
+        agg.upload(sentinel)
+        with pytest.raises(bc.BcmgError) as e:
+            bc.aggregate(ctx, n, dev(ctx, np.array(M, dtype=np.int32)), w, agg, p)
+        assert e.value.code == bc.EINVAL, name
+        assert np.array_equal(agg.numpy(), sentinel), name
+
+
 @pytest.mark.parametrize("name,setting", [
     ("poisson3d_17", "V_L1_2sweeps"),
     ("aniso2d_48", "W_L1_3sweeps"),
@@ -239,11 +261,7 @@ def test_hierarchy_and_fcg(ctx, name, setting):
     db, dx = dev(ctx, b), bc.DeviceArray(ctx, n)
     it, resid = B.fcg(db, dx)
     x = dx.numpy()
-    assert abs(it - res_o.iterations) <= 2, (it, res_o.iterations)
-    S = ops.to_scipy(A)
-    rtol = prm.rtol
-    assert np.linalg.norm(b - S @ x) <= 10 * rtol * np.linalg.norm(b)
-    assert np.linalg.norm(x - xo) <= 1e-5 * np.linalg.norm(xo)
+    check_fcg_vs_oracle(x, it, xo, res_o.iterations, A, b, prm.rtol, name + " " + setting)
 
 
 @pytest.mark.parametrize("name,setting", [
@@ -264,15 +282,17 @@ def test_bootstrap_composites(ctx, name, setting):
     nh_o, er_o = O.info()
     nh, er = B.info()
     assert nh == nh_o
-    assert abs(er - er_o) <= 1e-6 * abs(er_o)
+    print("%s %s: estimated ratio %.17g (oracle %.17g)" % (name, setting, er, er_o))
+    assert abs(er - er_o) <= 1e-12 * abs(er_o), (er, er_o)
     for k in range(min(nh, 2)):
         _compare_hierarchy(B, O, k)
     b = ops.uniform_pm1(1, n)
     rc, xo, res_o, _ = O.fcg(b)
+    assert rc == 0
     dx = bc.DeviceArray(ctx, n)
     it, _ = B.fcg(dev(ctx, b), dx)
-    assert abs(it - res_o.iterations) <= 2, (it, res_o.iterations)
-    assert np.linalg.norm(dx.numpy() - xo) <= 1e-5 * np.linalg.norm(xo)
+    check_fcg_vs_oracle(dx.numpy(), it, xo, res_o.iterations, A, b, prm.rtol,
+                        name + " " + setting)
 
 
 def test_precond_apply(ctx):
@@ -298,8 +318,8 @@ def test_solve_host_api_and_errors(ctx):
     x, info = bc.solve(A, prm, np.ones(n))
     rc, xo, res = orc.solve(A, np.ones(n), orc.params(rtol=1e-6))
     assert rc == 0
-    assert abs(info["iterations"] - res.iterations) <= 2
-    assert np.linalg.norm(x - xo) <= 1e-5 * np.linalg.norm(xo)
+    check_fcg_vs_oracle(x, info["iterations"], xo, res.iterations, A, np.ones(n), 1e-6,
+                        "solve() poisson 10^3")
     with pytest.raises(bc.BcmgError) as e:
         bc.solve(A, prm, np.zeros(n))
     assert e.value.code == bc.EZERORHS
@@ -381,8 +401,7 @@ def test_randomly_ordered_operator(ctx):
     assert rc == 0
     dxs = bc.DeviceArray(ctx, n)
     it, _ = B.fcg(dev(ctx, b), dxs)
-    assert abs(it - res_o.iterations) <= 2, (it, res_o.iterations)
-    assert np.linalg.norm(dxs.numpy() - xo) <= 1e-5 * np.linalg.norm(xo)
+    check_fcg_vs_oracle(dxs.numpy(), it, xo, res_o.iterations, A, b, prm["rtol"], "permuted")
 
 
 def test_context_on_a_caller_stream(ctx):

@@ -112,3 +112,128 @@ def test_budget_closes_idle_lanes(ctx, boots):
     for (rf, xf), (rt, xt) in zip(full, tight):
         assert rf["iterations"] == rt["iterations"] and rf["residual"] == rt["residual"]
         assert np.array_equal(bits(xf), bits(xt))
+
+
+def _hip():
+    import ctypes
+    return ctypes.CDLL("libamdhip64.so")
+
+
+def device_bytes_in_use():
+    import ctypes
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert _hip().hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return total.value - free.value
+
+
+# four operators of about 90 000 rows each, all of different sizes (so that
+# no two lanes share the pool's size classes)
+SIMILAR = [lambda: ops.poisson3d(44), lambda: ops.poisson3d(46), lambda: ops.poisson2d(300),
+           lambda: ops.aniso2d(310, 0.01)]
+
+
+def test_closed_lanes_give_their_memory_back(ctx):
+    """A budget that holds one full lane at a time: requests of another
+    operator close idle lanes, and a closed lane's device memory leaves the
+    process, so the memory in use grows by at most the budget (not by the
+    sum of every lane ever opened)."""
+    mats = [g() for g in SIMILAR]
+    prm = bc.make_params(**SETTINGS["V_L1_2sweeps"])
+    # lane sizes, measured in the module's context
+    sizes = []
+    for A in mats:
+        B = bc.Boot(ctx, bc.Csr(ctx, A), prm)
+        n = len(A[0]) - 1
+        _, info = serve(ctx, [(B, n)], [(0, ops.uniform_pm1(1, n))], [0], slots=8)
+        sizes.append(info["bytes"])
+        B.free()
+    assert max(sizes) < 1.1 * min(sizes), sizes
+    budget = int(1.3 * max(sizes))
+    c2 = bc.Context(0)
+    boots = []
+    for A in mats:
+        B = bc.Boot(c2, bc.Csr(c2, A), prm)
+        boots.append((B, len(A[0]) - 1))
+    c2.sync()
+    used0 = device_bytes_in_use()
+    reqs = [(op % 4, ops.uniform_pm1(50 + op, boots[op % 4][1])) for op in range(12)]
+    arrivals = [40 * i for i in range(12)]      # one operator after the other
+    out, info = serve(c2, boots, reqs, arrivals, slots=8, budget=budget)
+    used1 = device_bytes_in_use()
+    print("lane bytes %s, budget %d, device memory grown by %d" % (sizes, budget, used1 - used0))
+    # lanes narrower than 8 slots may share the budget; what they hold, and
+    # what the device gave the process, stays within it
+    assert info["bytes"] <= budget
+    assert used1 - used0 <= budget + (32 << 20), (used1 - used0, budget)
+    for (op, b), (r, x) in zip(reqs, out):
+        assert r["status"] == bc.OK
+        B, n = boots[op]
+        x1 = bc.DeviceArray(c2, n)
+        it1, res1 = B.fcg(bc.DeviceArray.from_numpy(c2, b), x1)
+        assert (it1, res1) == (r["iterations"], r["residual"])
+        assert np.array_equal(bits(x1.numpy()), bits(x))
+    c2.close()
+
+
+def test_b_written_on_the_callers_stream_right_before_submit():
+    """A context on the caller's stream: b is copied into place on that
+    stream (behind a queue of large copies) just before submit and step; the
+    served result equals the single solve of b bit for bit."""
+    import ctypes
+    hip = _hip()
+    s = ctypes.c_void_p()
+    assert hip.hipStreamCreateWithFlags(ctypes.byref(s), ctypes.c_uint(1)) == 0
+    try:
+        c2 = bc.Context(0, stream=s.value)
+        A = OPERATORS["poisson3d_17"]()
+        n = len(A[0]) - 1
+        B = bc.Boot(c2, bc.Csr(c2, A), bc.make_params(**SETTINGS["V_L1_2sweeps"]))
+        b = ops.uniform_pm1(77, n)
+        src = bc.DeviceArray.from_numpy(c2, b)
+        dst = bc.DeviceArray.from_numpy(c2, np.zeros(n))
+        big = 64 << 20
+        bsrc = bc.DeviceArray(c2, big)
+        bdst = bc.DeviceArray(c2, big)
+        c2.sync()
+        S = bc.Server(c2, 64 << 30, 4)
+        S.add(B)
+        x = bc.DeviceArray(c2, n)
+        for _ in range(4):                    # keep the stream busy
+            assert hip.hipMemcpyAsync(bdst.ptr, bsrc.ptr, ctypes.c_size_t(8 * big),
+                                      ctypes.c_int(3), s) == 0
+        assert hip.hipMemcpyAsync(dst.ptr, src.ptr, ctypes.c_size_t(8 * n), ctypes.c_int(3),
+                                  s) == 0
+        rid = S.submit(0, dst, x)
+        r = None
+        for _ in range(20000):
+            S.step()
+            r = S.result(rid)
+            if r is not None:
+                break
+        assert r is not None and r["status"] == bc.OK
+        x1 = bc.DeviceArray(c2, n)
+        it1, res1 = B.fcg(src, x1)
+        assert (it1, res1) == (r["iterations"], r["residual"])
+        assert np.array_equal(bits(x1.numpy()), bits(x.numpy()))
+        S.close()
+        c2.close()
+    finally:
+        hip.hipStreamDestroy(s)
+
+
+def test_budget_below_any_lane_fails_the_requests(ctx, boots):
+    """No lane of an operator fits a tiny budget even with every other lane
+    closed: its requests end with BCMG_ENOMEM instead of staying queued."""
+    S = bc.Server(ctx, 1024, 4)
+    S.add(boots[0][0])
+    n = boots[0][1]
+    x = bc.DeviceArray(ctx, n)
+    rids = [S.submit(0, bc.DeviceArray.from_numpy(ctx, ops.uniform_pm1(q, n)), x)
+            for q in range(3)]
+    done = S.step()
+    assert done == 3
+    for rid in rids:
+        r = S.result(rid)
+        assert r is not None and r["status"] == bc.ENOMEM
+    assert S.info()["queued"] == 0 and S.info()["lanes"] == 0
+    S.close()
