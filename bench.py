@@ -174,6 +174,18 @@ def multi_rhs(bc, ctx, steps):
     per = (time.perf_counter() - t0) / steps
     rel = [float(np.linalg.norm(bh[j] - S @ xs[j].numpy()) / np.linalg.norm(bh[j]))
            for j in range(C3_RHS)]
+    levels = [B.level_size(0, l) for l in range(B.levels(0))]
+    mb = multi_solve_bytes(levels, [int(v) for v in it], C3_SETTINGS["prerelax_sweeps"],
+                           C3_SETTINGS["postrelax_sweeps"], C3_SETTINGS["relaxnumber_coarse"],
+                           cycle_visits(len(levels), C3_SETTINGS["cycle_type"]))
+    out["solve_roofline"] = {
+        "bytes_per_batch": mb, "bytes_per_rhs": mb / C3_RHS,
+        "bound_rhs_solves_per_s": HBM_PEAK_GBS * 1e9 / (mb / C3_RHS),
+        "achieved_rhs_solves_per_s": C3_RHS / per,
+        "frac": (mb / per) / (HBM_PEAK_GBS * 1e9),
+        "note": "matrix data counted once per launch for all columns over the longest "
+                "column's iterations, vectors per column over its own iterations; FP32 "
+                "hierarchy and cycle vectors, FP64 FCG vectors, 32-bit indices"}
     out.update({"multi_fp32_ms_per_batch": per * 1e3,
                 "multi_fp32_ms_per_rhs": per * 1e3 / C3_RHS,
                 "rhs_solves_per_s": C3_RHS / per,
@@ -185,45 +197,66 @@ def multi_rhs(bc, ctx, steps):
     return out
 
 
+SERVE_RATES = (10, 50, 200, 500, 1000, 2000)
+
+
 def serving(bc, ctx, quick=False):
     """The serving workload: 8 cached operators, Poisson arrivals, uniform
-    operator choice; the served loop (continuous batching) against one solve
-    at a time in FP64 on the same trace. quick: only the served FP32 loop on
-    a 200-request trace (for profiling)."""
+    operator choice, a distinct seeded right-hand side per request; the
+    served loop (continuous batching) in FP32 and FP64 against one FP64 solve
+    at a time on the same traces. Latency includes the upload of b and the
+    download of x. A saturation trace (400 requests offered at 4000/s) gives
+    the throughput; the rate sweep gives p50/p99 from light load to overload
+    (at most 200 requests or 4 s of arrivals per rate). quick: only the
+    served FP32 loop on a 200-request trace (for profiling)."""
     from bootcmatchg_amd import serving as sv
     pool = sv.Pool(bc, ctx, SETTINGS, 0)
     out = {"operators": [{"name": o["name"], "rows": o["n"], "nnz": o["nnz"]} for o in pool.ops],
            "settings": "0,1,0.8,2,0,39,0,4,4,16,4,4,2000,1e-8",
-           "slots_per_operator": 16}
+           "slots_per_operator": 16,
+           "latency": "scheduled arrival to solution in host memory (queueing, upload of b, "
+                      "solve, download of x); b of request k is uniform[-1,1] stream k"}
     if quick:
         pool.set_precision(1)
         S = bc.Server(ctx, 96 << 30, 16)
         for o in pool.ops:
             S.add(o["B"])
-        arr, opsel, rhsel = sv.trace(4000, len(pool.ops), 200, 3.0, seed=1)
-        out["served_fp32"] = sv.run_served(pool, S, arr, opsel, rhsel)
+        arr, opsel = sv.trace(4000, len(pool.ops), 200, 3.0, seed=1)
+        out["served_fp32"] = sv.run_served(pool, S, arr, opsel, pool.rhs(opsel, 0), 8)
         S.close()
         return out
-    arr, opsel, rhsel = sv.trace(4000, len(pool.ops), 400, 3.0, seed=1)
+    arr, opsel = sv.trace(4000, len(pool.ops), 400, 3.0, seed=1)
+    bh = pool.rhs(opsel, 0)
+    sweep = {}
+    traces = {}
+    for rate in SERVE_RATES:
+        a, o_ = sv.trace(rate, len(pool.ops), 200, 4.0, seed=rate)
+        traces[rate] = (a, o_, pool.rhs(o_, 100000 * rate))
+        sweep[str(rate)] = {}
     out["saturation_trace"] = "%d requests offered at 4000/s" % len(arr)
-    out["plain_fp64_one_at_a_time"] = sv.run_plain(pool, arr, opsel, rhsel)
-    for prec in (0, 1):
+    out["plain_fp64_one_at_a_time"] = sv.run_plain(pool, arr, opsel, bh)
+    for rate in SERVE_RATES:
+        a, o_, b_ = traces[rate]
+        sweep[str(rate)]["plain_fp64"] = sv.run_plain(pool, a, o_, b_, 4)
+    for prec in (1, 0):
         pool.set_precision(prec)
         S = bc.Server(ctx, 96 << 30, 16)
         for o in pool.ops:
             S.add(o["B"])
-        a2, o2, r2 = sv.trace(4000, len(pool.ops), 64, 1.0, seed=9)
-        sv.run_served(pool, S, a2, o2, r2)          # lanes opened, graphs captured
-        out["served_%s" % PREC_NAMES[prec]] = sv.run_served(pool, S, arr, opsel, rhsel)
-        if prec == 1:
-            for rate in (50, 200):
-                a, o_, r_ = sv.trace(rate, len(pool.ops), 300, 3.0, seed=2)
-                out["served_fp32_at_%d_per_s" % rate] = sv.run_served(pool, S, a, o_, r_)
+        a2, o2 = sv.trace(4000, len(pool.ops), 64, 1.0, seed=9)
+        sv.run_served(pool, S, a2, o2, pool.rhs(o2, 7000000), 8)  # lanes opened, graphs captured
+        out["served_%s" % PREC_NAMES[prec]] = sv.run_served(pool, S, arr, opsel, bh)
+        for rate in SERVE_RATES:
+            a, o_, b_ = traces[rate]
+            sweep[str(rate)]["served_%s" % PREC_NAMES[prec]] = sv.run_served(pool, S, a, o_, b_, 4)
         out["device_bytes_%s" % PREC_NAMES[prec]] = S.info()["bytes"]
         S.close()
+    out["rate_sweep"] = sweep
     plain = out["plain_fp64_one_at_a_time"]["solves_per_s"]
     out["served_fp32_vs_plain_fp64"] = out["served_fp32"]["solves_per_s"] / plain
     out["served_fp64_vs_plain_fp64"] = out["served_fp64"]["solves_per_s"] / plain
+    out["served_fp32_vs_served_fp64"] = (out["served_fp32"]["solves_per_s"]
+                                         / out["served_fp64"]["solves_per_s"])
     for o in pool.ops:
         o["B"].free()
     return out
@@ -361,6 +394,52 @@ def solve_bytes(levels, iterations, pre, post, coarse):
     return iterations * (cyc + fcg)
 
 
+def cycle_visits(L, cycle_type, max_levels=39):
+    """Visits of each level in one cycle (src/SOLVER/GAMG_cycle.cu:103-111
+    with the grid sweeps of src/AMG/AMG.cu:371-394): level j + 1 is entered
+    num_grid_sweeps[j] times per visit of level j, once when it is the
+    coarsest."""
+    ml = max_levels - 1
+    g = [1] * ml
+    if cycle_type == 1:
+        g = [2 if i % 2 == 0 else 1 for i in range(ml)]
+    elif cycle_type == 2:
+        g = [2 if i < ml - 1 else 1 for i in range(ml)]
+    v = [1]
+    for j in range(L - 1):
+        v.append(v[j] * (1 if j + 1 == L - 1 else g[j]))
+    return v
+
+
+def multi_solve_bytes(levels, its, pre, post, coarse, visits, sv=4):
+    """Algorithmic HBM bytes of a batched solve of len(its) right-hand sides
+    on one hierarchy whose operators, divisors and prolongators hold sv-byte
+    values (4: FP32) with 32-bit indices, cycle vectors sv bytes per column
+    entry, FCG vectors FP64: every launch reads the matrix data once for all
+    columns (counted for max(its) iterations) and each column's vectors
+    (counted for its own iterations). Per level visit as solve_bytes; per
+    FCG iteration the fine SpMV, the triple product, two merged axpys, the
+    norm, one zeroing and the conversions of r into and z out of the cycle's
+    precision."""
+    L = len(levels)
+    cm = cv = 0
+    for j, (n, nnz) in enumerate(levels):
+        v = visits[j]
+        mat = nnz * (sv + 4) + 4 * (n + 1)
+        if j == L - 1:
+            cm += v * coarse * (mat + sv * n)
+            cv += v * coarse * 3 * sv * n
+            continue
+        nc = levels[j + 1][0]
+        cm += v * ((pre + post) * (mat + sv * n) + mat + 2 * n * (sv + 4) + 4 * (nc + 1))
+        cv += v * ((pre + post) * 3 * sv * n + 3 * sv * n + sv * (n + nc) + sv * nc
+                   + sv * (nc + 2 * n))
+    n0, nnz0 = levels[0]
+    fm = 12 * nnz0 + 4 * (n0 + 1)
+    fv = 16 * n0 + (32 + 80 + 8 + 8) * n0 + 2 * (8 + sv) * n0
+    return max(its) * (cm + fm) + sum(its) * (cv + fv)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -447,6 +526,24 @@ def main():
     elapsed = t1 - t0
     lat = np.array([r["total_ms"] for r in rec])
 
+    # setup phases, event-timed on the context stream (separate setups: the
+    # events stay out of the timed loop)
+    bc.time_setup(ctx, True)
+    phases = []
+    for _ in range(args.steps):
+        Bp = bc.Boot(ctx, dA, prm)
+        phases.append(Bp.setup_timing(-1))
+        Bp.free()
+    bc.time_setup(ctx, False)
+    setup_phases = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+    named = ("weights_ms", "matching_ms", "aggregation_ms", "galerkin_ms", "smoother_ms",
+             "inner_ms")
+    setup_phases["other_ms"] = setup_phases["total_ms"] - sum(setup_phases[k] for k in named)
+    setup_phases["note"] = ("means over %d setups with HIP events around each phase; "
+                            "total_ms is the host wall time of bcmg_setup, other_ms the "
+                            "rest (A-norm, sizes read back, tile analysis, buffers)"
+                            % args.steps)
+
     # cached hierarchy: FCG only (the serving form's unit of work)
     B = bc.Boot(ctx, dA, prm)
     B.fcg(db, dx)
@@ -508,8 +605,15 @@ def main():
         from bootcmatchg_amd import operators as ops2
         A2 = ops2.lognormal3d(C2_GRID, 4.0, seed=0)
         b2 = ops2.uniform_pm1(0, len(A2[0]) - 1)
+        xo2 = None
+        if cpu is not None:
+            rc2, xo2, ores2 = orc.solve(A2, b2, orc.params(**C2_SETTINGS))
+            assert rc2 == 0
         precision["c2_lognormal128_boot3"] = precision_forms(
-            bc, ctx, A2, b2, C2_SETTINGS, max(2, args.steps // 2), 1)
+            bc, ctx, A2, b2, C2_SETTINGS, max(2, args.steps // 2), 1, x_ref=xo2,
+            ref_name="fp64_oracle" if xo2 is not None else None)
+        if xo2 is not None:
+            precision["c2_lognormal128_boot3"]["oracle_iterations"] = ores2.iterations
         precision["c2_lognormal128_boot3"]["workload"] = (
             "cell-coefficient diffusion 128^3, k=exp(g), g~N(0,4) seed 0, harmonic faces; "
             "b uniform[-1,1] seed 0; settings 0,3,0.5,2,0,39,0,4,4,16,4,4,2000,1e-8")
@@ -538,6 +642,7 @@ def main():
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
         "setup_ms": float(np.mean([r["setup_ms"] for r in rec])),
+        "setup_phases_ms": setup_phases,
         "solve_ms": float(np.mean([r["solve_ms"] for r in rec])),
         "iterations": rec[-1]["iterations"],
         "levels": rec[-1]["levels"],

@@ -102,6 +102,13 @@ def lib():
 PREC_FP64, PREC_FP32, PREC_BF16 = 0, 1, 2
 
 
+class SetupTimes(ctypes.Structure):
+    """bcmg_setup_times: the setup's phases in ms."""
+    _fields_ = [(f, ctypes.c_double) for f in (
+        "weights_ms", "matching_ms", "aggregation_ms", "galerkin_ms", "smoother_ms",
+        "inner_ms", "hierarchy_ms", "total_ms")]
+
+
 def _check(rc):
     if rc != OK:
         raise BcmgError(rc, lib().bcmg_last_error().decode(errors="replace"))
@@ -297,6 +304,24 @@ class Boot:
         er = ctypes.c_double()
         _check(lib().bcmg_boot_info(self.h, ctypes.byref(nh), ctypes.byref(er)))
         return nh.value, er.value
+
+    def hier_info(self, k=0):
+        """Hierarchy k: levels, operator complexity, its W-cycle form, the
+        average coarsening ratio and the estimated ratio once k was added
+        (AMG::Hierarchy::printInfo, src/AMG/AMG.cu:241-251)."""
+        lv = ctypes.c_int()
+        d = [ctypes.c_double() for _ in range(4)]
+        _check(lib().bcmg_hier_info(self.h, k, ctypes.byref(lv), *[ctypes.byref(x) for x in d]))
+        return dict(levels=lv.value, op_cmplx=d[0].value, op_wcmplx=d[1].value,
+                    avg_cratio=d[2].value, ratio_after=d[3].value)
+
+    def setup_timing(self, k=-1):
+        """Event-timed setup phases in ms (recorded while time_setup(ctx, True)
+        was in force when the composite was built); k = -1 sums the
+        composite, k >= 0 gives hierarchy k."""
+        t = SetupTimes()
+        _check(lib().bcmg_setup_timing(self.h, k, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in SetupTimes._fields_}
 
     def levels(self, k=0):
         r = lib().bcmg_hier_levels(self.h, k)
@@ -532,6 +557,11 @@ def time_smoother(ctx, enable):
     """Bracket every level-0 Jacobi sweep (from a non-zero iterate) with
     HIP events until disabled; read with smoother_timing()."""
     _check(lib().bcmg_time_smoother(ctx.h, 1 if enable else 0))
+
+
+def time_setup(ctx, enable):
+    """Time the phases of the next setups with HIP events (Boot.setup_timing)."""
+    _check(lib().bcmg_time_setup(ctx.h, 1 if enable else 0))
 
 
 def smoother_timing(ctx):

@@ -12,6 +12,7 @@
 
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <unordered_map>
 #include <stdexcept>
 #include <type_traits>
@@ -170,6 +171,8 @@ struct Ctx {
     DBuf<int> iscratch;
     /* timing of the dominant kernel (optional) */
     bool time_smoother = false;
+    /* event timing of the setup phases (optional) */
+    bool time_setup = false;
     bool use_graphs = true;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -210,6 +213,19 @@ struct Hierarchy {
     int num_levels() const { return (int)L.size(); }
 };
 
+/* phases of the setup, timed with events on the context stream when
+   Ctx::time_setup is set (the field order of bcmg_setup_times) */
+enum SetupPhase {
+    PH_WEIGHTS = 0,     /* matching weights AH and their logarithms */
+    PH_MATCHING,        /* suitor matching and its symmetrisation */
+    PH_AGGREGATION,     /* aggregates, P, R = P^T, P composition, w restriction */
+    PH_GALERKIN,        /* coarse operators R (A P) */
+    PH_SMOOTHER,        /* smoother divisors (diagonal, L1 row sums) */
+    PH_INNER,           /* bootstrap inner iterations */
+    PH_HIER,            /* the whole build of one hierarchy with its inner iterations */
+    PH_COUNT
+};
+
 /* work vectors of the cycles in one compute type, per level, sized to the
    largest hierarchy */
 template <typename VT>
@@ -229,8 +245,15 @@ struct Boot {
     bcmg_params prm{};
     int coarse_solver = 4;
     std::vector<int> num_grid_sweeps;
-    std::vector<Hierarchy> H;
+    /* the composite's hierarchies; slot 1 shares slot 0's storage when it is
+       built from the same smooth vector (w = 1, src/BOOT/bootstrap.cu:68-75) */
+    std::vector<std::shared_ptr<Hierarchy>> H;
     double estimated_ratio = 1.0;
+    /* estimated ratio after each hierarchy was added (printed per hierarchy
+       by the reference, src/BOOT/bootstrap.cu:76) */
+    std::vector<double> ratio_after;
+    /* per hierarchy: event-timed phases (ms), zero unless timed */
+    std::vector<std::vector<double>> t_phase;
     DevCsr *A0 = nullptr;       /* fine operator (owned by the caller's handle) */
     int prec = BCMG_PREC_FP64;  /* storage/compute precision of the cycles */
     int restarts = 0;           /* true-residual restarts of the last FCG */

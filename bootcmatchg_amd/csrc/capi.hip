@@ -108,8 +108,8 @@ const Level &get_level(bcmg_boot B, int k, int l)
 {
     REQUIRE(B, "null composite");
     REQUIRE(k >= 0 && k < (int)B->B.H.size(), "hierarchy index out of range");
-    REQUIRE(l >= 0 && l < B->B.H[k].num_levels(), "level index out of range");
-    return B->B.H[k].L[l];
+    REQUIRE(l >= 0 && l < B->B.H[k]->num_levels(), "level index out of range");
+    return B->B.H[k]->L[l];
 }
 
 void d2h(bcmg_ctx ctx, void *dst, const void *src, size_t bytes)
@@ -589,6 +589,47 @@ int bcmg_boot_info(bcmg_boot B, int *n_hrc, double *estimated_ratio)
     return BCMG_OK;
 }
 
+int bcmg_hier_info(bcmg_boot B, int k, int *levels, double *op_cmplx, double *op_wcmplx,
+                   double *avg_cratio, double *ratio_after)
+{
+    if (!B || k < 0 || k >= (int)B->B.H.size()) return BCMG_EINVAL;
+    const Hierarchy &H = *B->B.H[k];
+    if (levels) *levels = H.num_levels();
+    if (op_cmplx) *op_cmplx = H.op_cmplx;
+    if (op_wcmplx) *op_wcmplx = H.op_wcmplx;
+    if (avg_cratio) *avg_cratio = H.avg_cratio;
+    if (ratio_after)
+        *ratio_after = k < (int)B->B.ratio_after.size() ? B->B.ratio_after[k] : B->B.estimated_ratio;
+    return BCMG_OK;
+}
+
+int bcmg_time_setup(bcmg_ctx ctx, int enable)
+{
+    if (!ctx) return BCMG_EINVAL;
+    ctx->c.time_setup = enable != 0;
+    return BCMG_OK;
+}
+
+int bcmg_setup_timing(bcmg_boot B, int k, bcmg_setup_times *out)
+{
+    if (!B || !out || k < -1 || k >= (int)B->B.H.size()) return BCMG_EINVAL;
+    double t[PH_COUNT] = {0};
+    for (int h = 0; h < (int)B->B.t_phase.size(); h++) {
+        if (k >= 0 && h != k) continue;
+        const std::vector<double> &v = B->B.t_phase[h];
+        for (int i = 0; i < PH_COUNT && i < (int)v.size(); i++) t[i] += v[i];
+    }
+    out->weights_ms = t[PH_WEIGHTS];
+    out->matching_ms = t[PH_MATCHING];
+    out->aggregation_ms = t[PH_AGGREGATION];
+    out->galerkin_ms = t[PH_GALERKIN];
+    out->smoother_ms = t[PH_SMOOTHER];
+    out->inner_ms = t[PH_INNER];
+    out->hierarchy_ms = t[PH_HIER];
+    out->total_ms = k < 0 ? B->B.t_setup_ms : t[PH_HIER];
+    return BCMG_OK;
+}
+
 int bcmg_fcg_restarts(bcmg_boot B, int *restarts)
 {
     if (!B || !restarts) return BCMG_EINVAL;
@@ -599,7 +640,7 @@ int bcmg_fcg_restarts(bcmg_boot B, int *restarts)
 int bcmg_hier_levels(bcmg_boot B, int k)
 {
     if (!B || k < 0 || k >= (int)B->B.H.size()) return BCMG_EINVAL;
-    return B->B.H[k].num_levels();
+    return B->B.H[k]->num_levels();
 }
 
 int bcmg_level_size(bcmg_boot B, int k, int l, int *n, int *nnz)
@@ -627,7 +668,7 @@ int bcmg_level_P(bcmg_boot B, int k, int l, int *agg, double *p)
 {
     API_BEGIN
     const Level &lv = get_level(B, k, l);
-    REQUIRE(l + 1 < B->B.H[k].num_levels(), "the coarsest level has no prolongator");
+    REQUIRE(l + 1 < B->B.H[k]->num_levels(), "the coarsest level has no prolongator");
     CtxScope scope_(B->ctx);
     d2h(B->ctx, agg, lv.agg.get(), sizeof(int) * lv.mat().n);
     d2h(B->ctx, p, lv.p.get(), sizeof(double) * lv.mat().n);
@@ -638,7 +679,7 @@ int bcmg_level_Pinfo(bcmg_boot B, int k, int l, int *nnz, int *smoothed, double 
 {
     API_BEGIN
     const Level &lv = get_level(B, k, l);
-    REQUIRE(l + 1 < B->B.H[k].num_levels(), "the coarsest level has no prolongator");
+    REQUIRE(l + 1 < B->B.H[k]->num_levels(), "the coarsest level has no prolongator");
     if (nnz) *nnz = lv.smoothed ? lv.Pneg.nnz : lv.mat().n;
     if (smoothed) *smoothed = lv.smoothed ? 1 : 0;
     if (omega) *omega = lv.omega;
@@ -649,7 +690,7 @@ int bcmg_level_Pcsr(bcmg_boot B, int k, int l, int *row, int *col, double *val)
 {
     API_BEGIN
     const Level &lv = get_level(B, k, l);
-    REQUIRE(l + 1 < B->B.H[k].num_levels(), "the coarsest level has no prolongator");
+    REQUIRE(l + 1 < B->B.H[k]->num_levels(), "the coarsest level has no prolongator");
     CtxScope scope_(B->ctx);
     const int n = lv.mat().n;
     if (lv.smoothed) {

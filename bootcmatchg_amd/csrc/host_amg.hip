@@ -9,6 +9,7 @@
  */
 #include "bcmg_internal.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -31,9 +32,81 @@ std::vector<int> grid_sweeps(int cycle_type, int max_levels)
     return g;
 }
 
+/* Event marks of the setup phases on the context stream, recorded only when
+   Ctx::time_setup is set and read after the setup's last synchronisation.
+   A phase is bracketed by a Scope; hierarchy h's marks add up in
+   Boot::t_phase[h]. */
+class SetupClock {
+  public:
+    explicit SetupClock(Ctx &c) : c_(c), on_(c.time_setup) {}
+    ~SetupClock()
+    {
+        for (auto &m : marks_) {
+            if (m.a) (void)hipEventDestroy(m.a);
+            if (m.b) (void)hipEventDestroy(m.b);
+        }
+    }
+    SetupClock(const SetupClock &) = delete;
+    SetupClock &operator=(const SetupClock &) = delete;
+
+    class Scope {
+      public:
+        Scope(SetupClock *k, int phase) : k_(k && k->on_ ? k : nullptr)
+        {
+            if (k_) idx_ = k_->open(phase);
+        }
+        ~Scope()
+        {
+            if (k_) k_->close(idx_);
+        }
+        Scope(const Scope &) = delete;
+        Scope &operator=(const Scope &) = delete;
+
+      private:
+        SetupClock *k_;
+        size_t idx_ = 0;
+    };
+
+    void set_hierarchy(int h) { hier_ = h; }
+    void collect(Boot &B)
+    {
+        for (auto &t : B.t_phase) std::fill(t.begin(), t.end(), 0.0);
+        for (const auto &m : marks_) {
+            float ms = 0.f;
+            if (!m.a || !m.b || hipEventElapsedTime(&ms, m.a, m.b) != hipSuccess) continue;
+            if (m.hier >= (int)B.t_phase.size()) B.t_phase.resize(m.hier + 1);
+            B.t_phase[m.hier].resize(PH_COUNT, 0.0);
+            B.t_phase[m.hier][m.phase] += ms;
+        }
+    }
+
+  private:
+    struct Mark {
+        int hier, phase;
+        hipEvent_t a, b;
+    };
+    size_t open(int phase)
+    {
+        Mark m{hier_, phase, nullptr, nullptr};
+        if (hipEventCreate(&m.a) == hipSuccess) (void)hipEventRecord(m.a, c_.stream);
+        marks_.push_back(m);
+        return marks_.size() - 1;
+    }
+    void close(size_t i)
+    {
+        Mark &m = marks_[i];
+        if (hipEventCreate(&m.b) == hipSuccess) (void)hipEventRecord(m.b, c_.stream);
+    }
+    Ctx &c_;
+    bool on_;
+    int hier_ = 0;
+    std::vector<Mark> marks_;
+};
+
 struct BuildState {
     int maxcoarsesize = 0;
     int ftcoarse = 1;
+    SetupClock *clk = nullptr;
 };
 
 /* src/AMG/matchingAggregation.cu:18-60 */
@@ -64,22 +137,38 @@ void matching_aggregation(Ctx &c, const bcmg_params &p, BuildState &bs, const De
     for (int s = 0; s < p.aggrsweeps; s++) {
         const int n = Ai->n;
         DBuf<double> W(Ai->nnz > 0 ? Ai->nnz : 1);
-        match_weights(c, *Ai, wptr, W.get());
+        {
+            SetupClock::Scope ph(bs.clk, PH_WEIGHTS);
+            match_weights(c, *Ai, wptr, W.get());
+        }
         DBuf<int> M(n), Ms(n);
-        match_suitor(c, *Ai, W.get(), M.get(), Ms.get(), false);
+        {
+            SetupClock::Scope ph(bs.clk, PH_MATCHING);
+            match_suitor(c, *Ai, W.get(), M.get(), Ms.get(), false);
+        }
         W.release();
         DBuf<int> agg(n), members(2 * (size_t)n);
         DBuf<double> pv(n);
-        const int nc = aggregate_pairs(c, n, Ms.get(), wptr, agg.get(), pv.get(), members.get());
+        int nc = 0;
+        {
+            SetupClock::Scope ph(bs.clk, PH_AGGREGATION);
+            nc = aggregate_pairs(c, n, Ms.get(), wptr, agg.get(), pv.get(), members.get());
+        }
         DevCsr Ac;
-        galerkin_pairwise(c, *Ai, agg.get(), pv.get(), members.get(), nc, Ac);
+        {
+            SetupClock::Scope ph(bs.clk, PH_GALERKIN);
+            galerkin_pairwise(c, *Ai, agg.get(), pv.get(), members.get(), nc, Ac);
+        }
         DBuf<double> wc(nc);
-        restrict_w(c.stream, nc, members.get(), pv.get(), wptr, wc.get());
-        if (s == 0) {
-            aggP = std::move(agg);
-            pP = std::move(pv);
-        } else {
-            compose_P(c.stream, n0, aggP.get(), pP.get(), agg.get(), pv.get());
+        {
+            SetupClock::Scope ph(bs.clk, PH_AGGREGATION);
+            restrict_w(c.stream, nc, members.get(), pv.get(), wptr, wc.get());
+            if (s == 0) {
+                aggP = std::move(agg);
+                pP = std::move(pv);
+            } else {
+                compose_P(c.stream, n0, aggP.get(), pP.get(), agg.get(), pv.get());
+            }
         }
         const double coarse_ratio = (double)n / (double)nc;
         cur = std::move(Ac);
@@ -100,7 +189,7 @@ void matching_aggregation(Ctx &c, const bcmg_params &p, BuildState &bs, const De
    smoother data from the new A_{j+1} before level j + 1 is smoothed. The
    reference's infinity norm asserts (src/matrix/CSR.cu:402-404); this takes
    the body it left commented out there (largest sum of |s_ij| over a row). */
-void smooth_prolongators(Ctx &c, const bcmg_params &p, Hierarchy &H)
+void smooth_prolongators(Ctx &c, const bcmg_params &p, Hierarchy &H, SetupClock *clk)
 {
     const int levels = H.num_levels();
     for (int j = 0; j + 1 < levels; j++) {
@@ -108,26 +197,36 @@ void smooth_prolongators(Ctx &c, const bcmg_params &p, Hierarchy &H)
         Level &next = H.L[j + 1];
         const DevCsr &A = lv.mat();
         const int n = A.n, nc = next.mat().n;
-        DBuf<double> S;
-        const double omega = sa_smoother(c, A, lv.D.get(), S);
-        DBuf<int> iota((size_t)n + 1);
-        launch_iota(c.stream, n + 1, iota.get());
-        const CsrRef Sr{n, n, A.nnz, A.row.get(), A.col.get(), S.get()};
-        const CsrRef Pt{n, nc, n, iota.get(), lv.agg.get(), lv.p.get()};
-        spgemm(c, Sr, Pt, lv.Pneg);
-        S.release();
-        iota.release();
-        transpose_csr(c, ref_of(lv.Pneg), lv.R);
-        DevCsr T, Ac;
-        spgemm(c, ref_of(lv.R), ref_of(A), T);
-        spgemm(c, ref_of(T), ref_of(lv.Pneg), Ac);
-        T = DevCsr();
+        double omega = 0.0;
+        {
+            SetupClock::Scope ph(clk, PH_AGGREGATION);
+            DBuf<double> S;
+            omega = sa_smoother(c, A, lv.D.get(), S);
+            DBuf<int> iota((size_t)n + 1);
+            launch_iota(c.stream, n + 1, iota.get());
+            const CsrRef Sr{n, n, A.nnz, A.row.get(), A.col.get(), S.get()};
+            const CsrRef Pt{n, nc, n, iota.get(), lv.agg.get(), lv.p.get()};
+            spgemm(c, Sr, Pt, lv.Pneg);
+            S.release();
+            iota.release();
+            transpose_csr(c, ref_of(lv.Pneg), lv.R);
+        }
+        DevCsr Ac;
+        {
+            SetupClock::Scope ph(clk, PH_GALERKIN);
+            DevCsr T;
+            spgemm(c, ref_of(lv.R), ref_of(A), T);
+            spgemm(c, ref_of(T), ref_of(lv.Pneg), Ac);
+        }
         launch_negate(c.stream, lv.Pneg.nnz, lv.Pneg.val.get());
         lv.smoothed = true;
         lv.omega = omega;
         next.Aown = std::move(Ac);
         next.hasD = next.hasM = false;
-        relax_prepare(c, next, next.Aown, p.relax_type);
+        {
+            SetupClock::Scope ph(clk, PH_SMOOTHER);
+            relax_prepare(c, next, next.Aown, p.relax_type);
+        }
     }
 }
 
@@ -144,7 +243,10 @@ void adaptive_coarsening(Ctx &c, const bcmg_params &p, BuildState &bs, DevCsr &A
     DBuf<double> tmp(n);
     const double normw = a_norm(c, A, w.get(), tmp.get());
     tmp.release();
-    relax_prepare(c, H.L[0], A, p.relax_type);
+    {
+        SetupClock::Scope ph(bs.clk, PH_SMOOTHER);
+        relax_prepare(c, H.L[0], A, p.relax_type);
+    }
     bs.ftcoarse = 1;
     double avcoarseratio = 0.0;
     int level = 0;
@@ -157,8 +259,14 @@ void adaptive_coarsening(Ctx &c, const bcmg_params &p, BuildState &bs, DevCsr &A
         Level &next = H.L[level];
         const DevCsr &Aprev = prev.mat();
         matching_aggregation(c, p, bs, Aprev, w, prev.agg, prev.p, next.Aown);
-        transpose_P(c, Aprev.n, next.Aown.n, prev.agg.get(), prev.p.get(), prev.R);
-        relax_prepare(c, next, next.Aown, p.relax_type);
+        {
+            SetupClock::Scope ph(bs.clk, PH_AGGREGATION);
+            transpose_P(c, Aprev.n, next.Aown.n, prev.agg.get(), prev.p.get(), prev.R);
+        }
+        {
+            SetupClock::Scope ph(bs.clk, PH_SMOOTHER);
+            relax_prepare(c, next, next.Aown, p.relax_type);
+        }
         const double size_coarse = next.Aown.n;
         avcoarseratio += Aprev.n / size_coarse;
         level++;
@@ -170,14 +278,16 @@ void adaptive_coarsening(Ctx &c, const bcmg_params &p, BuildState &bs, DevCsr &A
         for (int i = 0; i + 1 < level; i++) pending.push_back(&H.L[i].R);
         resolve_tiles(c, pending);
         pending.clear();
-        smooth_prolongators(c, p, H);
+        smooth_prolongators(c, p, H, bs.clk);
         for (int i = 0; i + 1 < level; i++) {
             pending.push_back(&H.L[i].Pneg);
             pending.push_back(&H.L[i + 1].Aown);
         }
     }
-    if (p.coarse_solver != p.relax_type)
+    if (p.coarse_solver != p.relax_type) {
+        SetupClock::Scope ph(bs.clk, PH_SMOOTHER);
         relax_prepare(c, H.L[level - 1], H.L[level - 1].mat(), p.coarse_solver);
+    }
     for (int i = 0; i + 1 < level; i++) pending.push_back(&H.L[i].R);
     resolve_tiles(c, pending);
     pending.clear();
@@ -396,7 +506,7 @@ void run_hierarchy(Ctx &c, Boot &B, int k, std::vector<LevelState<VT>> &st, cons
                    VT *final_out)
 {
     CycleBufs<VT> &W = bufs<VT>(B);
-    const Hierarchy &H = B.H[k];
+    const Hierarchy &H = *B.H[k];
     for (int l = 1; l < H.num_levels(); l++) {
         st[l].x = W.xa[l].get();
         st[l].x2 = W.xb[l].get();
@@ -412,7 +522,7 @@ template <typename VT>
 void precond_apply_t(Ctx &c, Boot &B, const VT *r, VT *x, bool x_is_zero)
 {
     CycleBufs<VT> &W = bufs<VT>(B);
-    const int n = B.H[0].L[0].mat().n;
+    const int n = B.H[0]->L[0].mat().n;
     const int nh = (int)B.H.size();
     const size_t maxL = W.xa.size();
     std::vector<LevelState<VT>> st(maxL);
@@ -457,7 +567,7 @@ template <typename VT>
 void ensure_bufs(Boot &B, CycleBufs<VT> &W)
 {
     size_t maxL = 0;
-    for (auto &h : B.H) maxL = std::max(maxL, h.L.size());
+    for (auto &h : B.H) maxL = std::max(maxL, h->L.size());
     if (W.xa.size() < maxL) {
         W.xa.resize(maxL);
         W.xb.resize(maxL);
@@ -466,19 +576,19 @@ void ensure_bufs(Boot &B, CycleBufs<VT> &W)
     for (size_t l = 0; l < maxL; l++) {
         size_t nl = 0;
         for (auto &h : B.H)
-            if (l < h.L.size()) nl = std::max(nl, (size_t)h.L[l].mat().n);
+            if (l < h->L.size()) nl = std::max(nl, (size_t)h->L[l].mat().n);
         W.xa[l].ensure(nl);
         W.xb[l].ensure(nl);
         W.f[l].ensure(nl);
     }
-    W.res.ensure(B.H[0].L[0].mat().n);
+    W.res.ensure(B.H[0]->L[0].mat().n);
     if (B.prm.cycle_type == 3) {
         for (auto *v : {&W.kw, &W.kd0, &W.kd1, &W.kv, &W.kv1}) {
             if (v->size() < maxL) v->resize(maxL);
             for (size_t l = 0; l < maxL; l++) {
                 size_t nl = 0;
                 for (auto &h : B.H)
-                    if (l < h.L.size()) nl = std::max(nl, (size_t)h.L[l].mat().n);
+                    if (l < h->L.size()) nl = std::max(nl, (size_t)h->L[l].mat().n);
                 (*v)[l].ensure(nl);
             }
         }
@@ -506,12 +616,12 @@ Boot::~Boot()
 void Boot::ensure_work()
 {
     size_t maxL = 0;
-    for (auto &h : H) maxL = std::max(maxL, h.L.size());
+    for (auto &h : H) maxL = std::max(maxL, h->L.size());
     kscal.ensure((maxL + 1) * KS_STRIDE);
     ensure_bufs<double>(*this, b64);
     if (prec != BCMG_PREC_FP64) {
         ensure_bufs<float>(*this, b32);
-        const size_t n0 = H[0].L[0].mat().n;
+        const size_t n0 = H[0]->L[0].mat().n;
         r32.ensure(n0);
         z32.ensure(n0);
     }
@@ -537,7 +647,7 @@ void precond_apply(Ctx &c, Boot &B, const double *r, double *x, bool x_is_zero)
         precond_apply_t<double>(c, B, r, x, x_is_zero);
         return;
     }
-    const int n = B.H[0].L[0].mat().n;
+    const int n = B.H[0]->L[0].mat().n;
     launch_d2f(c.stream, n, r, B.r32.get());
     if (!x_is_zero) launch_d2f(c.stream, n, x, B.z32.get());
     precond_apply_t<float>(c, B, B.r32.get(), B.z32.get(), x_is_zero);
@@ -552,14 +662,18 @@ void set_precision(Ctx &c, Boot &B, int prec)
     B.prec = prec;
     B.drop_graphs();
     if (prec == BCMG_PREC_FP64) return;
-    for (auto &H : B.H) {
+    /* level 0 (the caller's operator) and a hierarchy that several slots
+       of the composite share are converted once */
+    B.A0->make_lowp(c.stream, false);
+    std::vector<const Hierarchy *> done;
+    for (auto &Hp : B.H) {
+        if (std::find(done.begin(), done.end(), Hp.get()) != done.end()) continue;
+        done.push_back(Hp.get());
+        Hierarchy &H = *Hp;
         for (int l = 0; l < H.num_levels(); l++) {
             Level &lv = H.L[l];
             DevCsr &A = l == 0 ? *B.A0 : lv.Aown;
-            if (level_mt(B, l) == MT_BF16)
-                A.make_lowp(c.stream, true);
-            else
-                A.make_lowp(c.stream, false);
+            if (l > 0) A.make_lowp(c.stream, level_mt(B, l) == MT_BF16);
             const size_t n = A.n;
             if (lv.hasD) {
                 lv.D32.ensure(n);
@@ -650,17 +764,36 @@ void build_boot(Ctx &c, DevCsr &A, const bcmg_params &p, Boot &B)
     B.A0 = &A;
     B.coarse_solver = p.coarse_solver;
     B.num_grid_sweeps = grid_sweeps(p.cycle_type, p.max_levels);
+    SetupClock clk(c);
     BuildState bs;
+    bs.clk = &clk;
     bs.maxcoarsesize = (int)(40 * std::pow((double)A.n, (double)1. / 3.));
     DBuf<double> w(A.n);
     launch_fill(c.stream, A.n, 1.0, w.get());
     B.estimated_ratio = 1.0;
     B.H.clear();
+    B.ratio_after.clear();
+    B.t_phase.clear();
     while (B.estimated_ratio > p.conv_ratio && (int)B.H.size() < p.max_hrc) {
-        B.H.emplace_back();
-        adaptive_coarsening(c, p, bs, A, w.get(), B.H.back());
+        const int k = (int)B.H.size();
+        clk.set_hierarchy(k);
+        B.t_phase.emplace_back(PH_COUNT, 0.0);
+        SetupClock::Scope ph(&clk, PH_HIER);
+        if (k == 1) {
+            /* no inner iterations follow the first hierarchy, so the second
+               is built from the same w = 1 (src/BOOT/bootstrap.cu:68-75)
+               and equals the first bit for bit: it shares its storage */
+            B.H.push_back(B.H[0]);
+        } else {
+            B.H.push_back(std::make_shared<Hierarchy>());
+            adaptive_coarsening(c, p, bs, A, w.get(), *B.H.back());
+        }
         B.ensure_work();
-        if (B.H.size() > 1) inner_iterations(c, B, w);
+        if (k > 0) {
+            SetupClock::Scope in(&clk, PH_INNER);
+            inner_iterations(c, B, w);
+        }
+        B.ratio_after.push_back(B.estimated_ratio);
     }
     B.ensure_work();
     const int n = A.n;
@@ -672,6 +805,7 @@ void build_boot(Ctx &c, DevCsr &A, const bcmg_params &p, Boot &B)
     B.scal.ensure(16);
     c.ensure_reductions(n);
     BCMG_HIP(hipStreamSynchronize(c.stream));
+    clk.collect(B);
     B.t_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 

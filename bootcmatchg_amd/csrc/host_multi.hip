@@ -82,13 +82,13 @@ MultiWork &ensure_multi_into(Ctx &c, Boot &B, MultiWork &M, int kmax)
         M.drop_graph();
         M.kmax = kmax;
         size_t maxL = 0;
-        for (auto &h : B.H) maxL = std::max(maxL, h.L.size());
+        for (auto &h : B.H) maxL = std::max(maxL, h->L.size());
         M.ld.assign(maxL, 0);
         M.nl.assign(maxL, 0);
         for (size_t l = 0; l < maxL; l++) {
             size_t nl = 0;
             for (auto &h : B.H)
-                if (l < h.L.size()) nl = std::max(nl, (size_t)h.L[l].mat().n);
+                if (l < h->L.size()) nl = std::max(nl, (size_t)h->L[l].mat().n);
             M.ld[l] = round_ld(nl);
             M.nl[l] = (long)nl;
         }
@@ -353,7 +353,7 @@ template <typename VT>
 void run_hierarchy_m(MRun<VT> &R, int k)
 {
     CycleBufs<VT> &W = mbufs<VT>(R.M);
-    const Hierarchy &H = R.B.H[k];
+    const Hierarchy &H = *R.B.H[k];
     for (int l = 1; l < H.num_levels(); l++) {
         R.st[l].x = W.xa[l].get();
         R.st[l].x2 = W.xb[l].get();
@@ -391,7 +391,7 @@ void precond_m(Ctx &c, Boot &B, MultiWork &M, const VT *r, VT *x, const int *act
             for (int k = nh - 1; k >= 0; k--) order.push_back(k);
         for (size_t t = 0; t < order.size(); t++) {
             const bool last = (t + 1 == order.size());
-            MRun<VT> R{c,   B,   M,   B.H[order[t]], st,   r, last ? x : nullptr,
+            MRun<VT> R{c,   B,   M,   *B.H[order[t]], st,   r, last ? x : nullptr,
                        nc,  lkp, act, last ? z64 : nullptr, ldz, fused};
             run_hierarchy_m<VT>(R, order[t]);
         }
@@ -406,7 +406,7 @@ void precond_m(Ctx &c, Boot &B, MultiWork &M, const VT *r, VT *x, const int *act
             st[0].x = W.xb[0].get();
             st[0].x2 = W.xa[0].get();
             st[0].zero = true;
-            MRun<VT> R{c, B, M, B.H[k], st, r, nullptr, nc, lkp, act};
+            MRun<VT> R{c, B, M, *B.H[k], st, r, nullptr, nc, lkp, act};
             run_hierarchy_m<VT>(R, k);
             if (st[0].zero) launch_fill_il<VT>(c.stream, n, VT(0), st[0].x, lkp, nc, act);
             launch_axpy_il<VT>(c.stream, n, VT(1), st[0].x, W.xadd.get(), lkp, nc, act);
@@ -702,7 +702,7 @@ MultiWork &multi_work(Ctx &c, Boot &B, MultiWork *&holder, int kmax)
 size_t multi_work_bytes(const Boot &B, int kmax)
 {
     size_t maxL = 0;
-    for (auto &h : B.H) maxL = std::max(maxL, h.L.size());
+    for (auto &h : B.H) maxL = std::max(maxL, h->L.size());
     const size_t kp = (size_t)1 << log2_cols(kmax);
     /* per level: x, x2, f (and, for the K-cycle, its five inner vectors) */
     const size_t per_level = B.prm.cycle_type == 3 ? 8 : 3;
@@ -710,7 +710,7 @@ size_t multi_work_bytes(const Boot &B, int kmax)
     for (size_t l = 0; l < maxL; l++) {
         size_t nl = 0;
         for (auto &h : B.H)
-            if (l < h.L.size()) nl = std::max(nl, (size_t)h.L[l].mat().n);
+            if (l < h->L.size()) nl = std::max(nl, (size_t)h->L[l].mat().n);
         if (l == 0) n0 = nl;
         cyc += per_level * nl * kp;
     }

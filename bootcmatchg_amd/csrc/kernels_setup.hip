@@ -765,15 +765,17 @@ int group_for(double avg)
 
 /* ================================================================ host */
 
-long long exclusive_scan(Ctx &c, const int *in, int *out, int n, bool read_total)
+namespace {
+
+/* exclusive scan of n ints with the block offsets in boff (at least
+   ceil(n / SCAN_CHUNK) + 1 ints); out[n] = total */
+long long exclusive_scan_into(Ctx &c, const int *in, int *out, int n, int *boff, bool read_total)
 {
     if (n <= 0) {
         BCMG_HIP(hipMemsetAsync(out, 0, sizeof(int), c.stream));
         return 0;
     }
     const int nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
-    c.iscratch.ensure((size_t)nb + 1);
-    int *boff = c.iscratch.get();
     k_scan_blocks<<<nb, BS, 0, c.stream>>>(n, in, boff);
     k_scan_top<<<1, BS, 0, c.stream>>>(nb, boff);
     k_scan_apply<<<nb, BS, 0, c.stream>>>(n, in, boff, out);
@@ -782,6 +784,15 @@ long long exclusive_scan(Ctx &c, const int *in, int *out, int n, bool read_total
     BCMG_HIP(hipMemcpyAsync(c.h_ints, out + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     BCMG_HIP(hipStreamSynchronize(c.stream));
     return c.h_ints[0];
+}
+
+} // namespace
+
+long long exclusive_scan(Ctx &c, const int *in, int *out, int n, bool read_total)
+{
+    const int nb = n > 0 ? (n + SCAN_CHUNK - 1) / SCAN_CHUNK : 0;
+    c.iscratch.ensure((size_t)nb + 1);
+    return exclusive_scan_into(c, in, out, n, c.iscratch.get(), read_total);
 }
 
 void launch_diag(hipStream_t s, const DevCsr &A, double *D)
@@ -811,24 +822,46 @@ void match_weights(Ctx &c, const DevCsr &A, const double *w, double *W)
     BCMG_CHECK_LAUNCH();
 }
 
+namespace {
+
+/* device memory of one suitor matching beyond its inputs and outputs: the
+   offer word of every vertex, the non-mutual flags, their prefix sums, the
+   list of non-mutual candidates and the scan's block offsets. Every word is
+   written before it is read, so nothing needs clearing. */
+struct SuitorWork {
+    unsigned long long *S;
+    int *flag, *pos, *list, *boff;
+};
+
+int match_suitor_with(Ctx &c, const DevCsr &A, const double *W, int *M, int *Msym,
+                      const SuitorWork &w, bool want_count)
+{
+    const int n = A.n;
+    k_fill_u64<<<blocks_for(n), BS, 0, c.stream>>>(n, ~0ull, w.S);
+    k_suitor<<<blocks_for(n), BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), W, w.S);
+    k_suitor_extract<<<blocks_for(n), BS, 0, c.stream>>>(n, w.S, M);
+    k_sym_pairs<<<blocks_for(n), BS, 0, c.stream>>>(n, M, Msym, w.flag);
+    BCMG_CHECK_LAUNCH();
+    exclusive_scan_into(c, w.flag, w.pos, n, w.boff, false);
+    k_compact<<<blocks_for(n), BS, 0, c.stream>>>(n, w.flag, w.pos, w.list);
+    k_unsym_serial<<<1, 64, 0, c.stream>>>(w.pos + n, w.list, M, Msym);
+    BCMG_CHECK_LAUNCH();
+    if (!want_count) return -1;
+    BCMG_HIP(hipMemcpyAsync(c.h_ints, w.pos + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    BCMG_HIP(hipStreamSynchronize(c.stream));
+    return c.h_ints[0];
+}
+
+} // namespace
+
 int match_suitor(Ctx &c, const DevCsr &A, const double *W, int *M, int *Msym, bool want_count)
 {
     const int n = A.n;
     DBuf<unsigned long long> S(n);
-    DBuf<int> flag(n), pos(n + 1), list(n);
-    k_fill_u64<<<blocks_for(n), BS, 0, c.stream>>>(n, ~0ull, S.get());
-    k_suitor<<<blocks_for(n), BS, 0, c.stream>>>(n, A.row.get(), A.col.get(), W, S.get());
-    k_suitor_extract<<<blocks_for(n), BS, 0, c.stream>>>(n, S.get(), M);
-    k_sym_pairs<<<blocks_for(n), BS, 0, c.stream>>>(n, M, Msym, flag.get());
-    BCMG_CHECK_LAUNCH();
-    exclusive_scan(c, flag.get(), pos.get(), n, false);
-    k_compact<<<blocks_for(n), BS, 0, c.stream>>>(n, flag.get(), pos.get(), list.get());
-    k_unsym_serial<<<1, 64, 0, c.stream>>>(pos.get() + n, list.get(), M, Msym);
-    BCMG_CHECK_LAUNCH();
-    if (!want_count) return -1;
-    BCMG_HIP(hipMemcpyAsync(c.h_ints, pos.get() + n, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    BCMG_HIP(hipStreamSynchronize(c.stream));
-    return c.h_ints[0];
+    DBuf<int> flag(n), pos(n + 1), list(n), boff((n + SCAN_CHUNK - 1) / SCAN_CHUNK + 1);
+    return match_suitor_with(c, A, W, M, Msym,
+                             SuitorWork{S.get(), flag.get(), pos.get(), list.get(), boff.get()},
+                             want_count);
 }
 
 int aggregate_pairs(Ctx &c, int n, const int *Msym, const double *w, int *agg, double *p,

@@ -273,16 +273,79 @@ inline CSR *copyToHost(CSR *A_d)
 
 } // namespace CSRm
 
-/* MatrixMarket "coordinate real|integer general|symmetric" to a host CSR with
-   sorted columns (src/matrix/matrixIO.cu:4-192; m_type 0). Duplicates are
-   summed; a symmetric file is expanded to both triangles. */
+namespace bcmg_facade {
+
+/* one coordinate entry of a matrix file (0-based) */
+struct Coo {
+    int r, c;
+    double v;
+};
+
+/* COO entries to a host CSR of n rows and m columns: columns ascending within
+   a row, duplicates summed (the reference's readers keep file order and
+   duplicates, src/matrix/matrixIO.cu:145-170, 237-261, which its kernels then
+   read as unsorted rows; the library requires sorted, unique columns) */
+inline CSR *csr_from_coo(std::vector<Coo> &e, long long n, long long m, bool symmetric);
+
+inline void read_fail(FILE *fp, const char *what)
+{
+    if (fp) std::fclose(fp);
+    fail(-1, what);
+}
+
+/* MTX2 (src/matrix/matrixIO.cu:194-289): "rows nnz", then the nnz 1-based row
+   indices, the nnz 1-based column indices and the nnz values, whitespace
+   separated; square with n = rows */
+inline CSR *read_mtx2(FILE *fp)
+{
+    long long n = 0, L = 0;
+    if (std::fscanf(fp, "%lld %lld", &n, &L) != 2 || n <= 0 || L < 0 || n > 0x7fffffff ||
+        L > 0x7fffffff)
+        read_fail(fp, "readMatrixFromFile: bad MTX2 header");
+    std::vector<Coo> e((size_t)L);
+    for (auto &x : e) {
+        long long i = 0;
+        if (std::fscanf(fp, "%lld", &i) != 1 || i < 1 || i > n)
+            read_fail(fp, "readMatrixFromFile: bad MTX2 row index");
+        x.r = (int)(i - 1);
+    }
+    for (auto &x : e) {
+        long long j = 0;
+        if (std::fscanf(fp, "%lld", &j) != 1 || j < 1 || j > n)
+            read_fail(fp, "readMatrixFromFile: bad MTX2 column index");
+        x.c = (int)(j - 1);
+    }
+    for (auto &x : e)
+        if (std::fscanf(fp, "%lf", &x.v) != 1) read_fail(fp, "readMatrixFromFile: short MTX2 file");
+    std::fclose(fp);
+    return csr_from_coo(e, n, n, false);
+}
+
+} // namespace bcmg_facade
+
+/* Read a matrix file into a host CSR with sorted columns
+   (src/matrix/matrixIO.cu:4-30). m_type 0: MatrixMarket "coordinate
+   real|integer general|symmetric" (:33-192; a symmetric file is expanded to
+   both triangles); m_type 1: MTX2 (:194-289). Duplicates are summed. */
 inline CSR *readMatrixFromFile(const char *path, int m_type, bool loadOnDevice = true)
 {
-    if (m_type != 0) bcmg_facade::fail(-1, "readMatrixFromFile: only MatrixMarket (m_type 0)");
+    if (m_type != 0 && m_type != 1) {
+        std::cout << "You need to specify an input matrix type with the argument -F/--inputype\n";
+        std::exit(1);
+    }
     FILE *fp = std::fopen(path, "r");
     if (!fp) {
         std::cout << "file not found: " << path << "\n";
         std::exit(-1);
+    }
+    if (m_type == 1) {
+        CSR *A = bcmg_facade::read_mtx2(fp);
+        if (loadOnDevice) {
+            CSR *d = CSRm::copyToDevice(A);
+            CSRm::free(A);
+            return d;
+        }
+        return A;
     }
     char line[1025];
     if (!std::fgets(line, 1024, fp)) bcmg_facade::fail(-1, "readMatrixFromFile: empty file");
@@ -297,39 +360,22 @@ inline CSR *readMatrixFromFile(const char *path, int m_type, bool loadOnDevice =
     long long M = 0, N = 0, L = 0;
     if (std::sscanf(line, "%lld %lld %lld", &M, &N, &L) != 3)
         bcmg_facade::fail(-1, "readMatrixFromFile: bad size line");
-    struct E {
-        int r, c;
-        double v;
-    };
-    std::vector<E> e;
+    if (M <= 0 || N <= 0 || L < 0 || M > 0x7fffffff || N > 0x7fffffff || L > 0x3fffffff)
+        bcmg_facade::read_fail(fp, "readMatrixFromFile: bad size line");
+    std::vector<bcmg_facade::Coo> e;
     e.reserve((size_t)(symmetric ? 2 * L : L));
     for (long long k = 0; k < L; k++) {
         long long i, j;
         double v = 1.0;
         if (!std::fgets(line, 1024, fp) || std::sscanf(line, "%lld %lld %lf", &i, &j, &v) < 2)
-            bcmg_facade::fail(-1, "readMatrixFromFile: short file");
+            bcmg_facade::read_fail(fp, "readMatrixFromFile: short file");
+        if (i < 1 || i > M || j < 1 || j > N)
+            bcmg_facade::read_fail(fp, "readMatrixFromFile: index out of range");
         e.push_back({(int)(i - 1), (int)(j - 1), v});
         if (symmetric && i != j) e.push_back({(int)(j - 1), (int)(i - 1), v});
     }
     std::fclose(fp);
-    std::sort(e.begin(), e.end(), [](const E &a, const E &b) {
-        return a.r != b.r ? a.r < b.r : a.c < b.c;
-    });
-    std::vector<E> u;
-    for (const E &x : e) {
-        if (!u.empty() && u.back().r == x.r && u.back().c == x.c)
-            u.back().v += x.v;
-        else
-            u.push_back(x);
-    }
-    CSR *A = CSRm::init((stype)M, (stype)N, (stype)u.size(), true, false, symmetric);
-    std::fill(A->row, A->row + M + 1, 0);
-    for (size_t k = 0; k < u.size(); k++) {
-        A->row[u[k].r + 1]++;
-        A->col[k] = u[k].c;
-        A->val[k] = u[k].v;
-    }
-    for (long long i = 0; i < M; i++) A->row[i + 1] += A->row[i];
+    CSR *A = bcmg_facade::csr_from_coo(e, M, N, symmetric);
     if (loadOnDevice) {
         CSR *d = CSRm::copyToDevice(A);
         CSRm::free(A);
@@ -337,6 +383,93 @@ inline CSR *readMatrixFromFile(const char *path, int m_type, bool loadOnDevice =
     }
     return A;
 }
+
+inline CSR *bcmg_facade::csr_from_coo(std::vector<Coo> &e, long long n, long long m,
+                                      bool symmetric)
+{
+    std::stable_sort(e.begin(), e.end(), [](const Coo &a, const Coo &b) {
+        return a.r != b.r ? a.r < b.r : a.c < b.c;
+    });
+    size_t u = 0;
+    for (size_t k = 0; k < e.size(); k++) {
+        if (u > 0 && e[u - 1].r == e[k].r && e[u - 1].c == e[k].c)
+            e[u - 1].v += e[k].v;
+        else
+            e[u++] = e[k];
+    }
+    e.resize(u);
+    CSR *A = CSRm::init((stype)n, (stype)m, (stype)u, true, false, symmetric);
+    std::fill(A->row, A->row + n + 1, 0);
+    for (size_t k = 0; k < u; k++) {
+        A->row[e[k].r + 1]++;
+        A->col[k] = e[k].c;
+        A->val[k] = e[k].v;
+    }
+    for (long long i = 0; i < n; i++) A->row[i + 1] += A->row[i];
+    return A;
+}
+
+namespace bcmg_facade {
+
+/* What the reference prints while it builds the composite, per hierarchy
+   (AMG::Hierarchy::printInfo, src/AMG/AMG.cu:241-251; the #META lines and
+   TOTAL_MUL_TIME of src/AMG/matchingAggregation.cu:408-421; the estimated
+   ratio, src/BOOT/bootstrap.cu:76) and after it (the hierarchy count and
+   each coarsest size, src/BOOT/bootstrap.cu:81-86). TOTAL_MUL_TIME is the
+   event-timed Galerkin phase (0 unless setup timing was on); a hierarchy
+   that shares the first one's storage was not rebuilt and shows 0. */
+inline void print_setup(bcmg_boot B)
+{
+    int nh = 0;
+    double ratio = 0;
+    check(bcmg_boot_info(B, &nh, &ratio), "composite info");
+    std::vector<int> coarsest(nh, 0);
+    for (int k = 0; k < nh; k++) {
+        int levels = 0;
+        double cx = 0, cw = 0, cr = 0, ra = 0;
+        check(bcmg_hier_info(B, k, &levels, &cx, &cw, &cr, &ra), "hierarchy info");
+        for (int l = 0; l < levels; l++) {
+            int n = 0, nnz = 0;
+            check(bcmg_level_size(B, k, l, &n, &nnz), "level size");
+            const float avg_nnz = (float)nnz / (float)n;
+            std::cout << "A" << l << " n: " << n << " nnz: " << nnz << " avg_nnz: " << avg_nnz
+                      << "\n";
+            coarsest[k] = n;
+        }
+        std::cout << "\nCurrent cmplx for V-cycle: " << cx;
+        std::cout << "\nCurrent cmplx for W-cycle: " << cw;
+        std::cout << "\nAverage Coarsening Ratio: " << cr << "\n";
+        meta("agg;level_number", levels - 1, 0);
+        meta("agg;avg_coarse_ratio", cr, 1);
+        meta("agg;OpCmplx", cx, 1);
+        bcmg_setup_times t;
+        check(bcmg_setup_timing(B, k, &t), "setup timing");
+        std::cout << "TOTAL_MUL_TIME: " << (float)t.galerkin_ms << "\n\n";
+        std::cout << "estimated_ratio " << ra << "\n";
+    }
+    std::cout << "Number of hierarchy: " << nh << "\n";
+    for (int k = 0; k < nh; k++) std::printf("%d] %d\n", k, coarsest[k]);
+}
+
+/* the #META lines and timings of src/SOLVER/solver.cu:106-118 */
+inline void print_solve(bcmg_boot B, int num_iter, double residual, float boot_time,
+                        float solve_time)
+{
+    int n_hrc = 0;
+    double ratio = 0.0;
+    check(bcmg_boot_info(B, &n_hrc, &ratio), "composite info");
+    meta("agg;hierarchy_levels_num", n_hrc, 0);
+    meta("agg;final_estimated_ratio", ratio, 1);
+    meta("sol;num_iteration", num_iter, 0);
+    meta("sol;residual", residual, 1);
+    meta("time;bootstrap_time", boot_time, 1);
+    meta("time;solve_time", solve_time, 1);
+    std::cout << "bootstrap_time: " << (boot_time / 1000) << " sec \n";
+    std::cout << "solve_time: " << (solve_time / 1000) << " sec \n";
+    std::cout << "time_for_iteration: " << (solve_time / num_iter / 1000) << " sec \n";
+}
+
+} // namespace bcmg_facade
 
 /* src/SOLVER/solver.cu:20-129: A_host is a host CSR, rhs a device vector;
    returns a new device vector x (FCG from x0 = 0), caller frees it. */
@@ -350,9 +483,11 @@ inline vector<vtype> *solve(CSR *A_host, const params p, vector<vtype> *rhs)
     check(bcmg_csr_upload(c, A_host->n, A_host->nnz, A_host->row, A_host->col, A_host->val, &A),
           "copy matrix to device");
     std::cout << "BUILDING....:\n\n";
+    check(bcmg_time_setup(c, 1), "setup timing");
     auto t0 = std::chrono::steady_clock::now();
     check(bcmg_setup(c, A, &p, &B), "AMG setup");
     auto t1 = std::chrono::steady_clock::now();
+    print_setup(B);
     vector<vtype> *Sol = Vector::init<vtype>(A_host->n, true, true);
     std::cout << "SOLVING....:\n\n";
     int num_iter = 0;
@@ -364,20 +499,9 @@ inline vector<vtype> *solve(CSR *A_host, const params p, vector<vtype> *rhs)
         bcmg_csr_free(A);
         check(rc, "FCG");
     }
-    int n_hrc = 0;
-    double ratio = 0.0;
-    bcmg_boot_info(B, &n_hrc, &ratio);
     const float boot_time = std::chrono::duration<float, std::milli>(t1 - t0).count();
     const float solve_time = std::chrono::duration<float, std::milli>(t2 - t1).count();
-    meta("agg;hierarchy_levels_num", n_hrc, 0);
-    meta("agg;final_estimated_ratio", ratio, 1);
-    meta("sol;num_iteration", num_iter, 0);
-    meta("sol;residual", residual, 1);
-    meta("time;bootstrap_time", boot_time, 1);
-    meta("time;solve_time", solve_time, 1);
-    std::cout << "bootstrap_time: " << (boot_time / 1000) << " sec \n";
-    std::cout << "solve_time: " << (solve_time / 1000) << " sec \n";
-    std::cout << "time_for_iteration: " << (solve_time / num_iter / 1000) << " sec \n";
+    print_solve(B, num_iter, residual, boot_time, solve_time);
     bcmg_boot_free(B);
     bcmg_csr_free(A);
     return Sol;

@@ -189,6 +189,16 @@ int bcmg_boot_info(bcmg_boot B, int *n_hrc, double *estimated_ratio);
 /* number of levels of hierarchy k (AMG hierarchy->num_levels, AMG.cu:51) */
 int bcmg_hier_levels(bcmg_boot B, int k);
 int bcmg_level_size(bcmg_boot B, int k, int l, int *n, int *nnz);
+/* hierarchy k's summary (AMG::Hierarchy::finalize_cmplx / finalize_wcmplx
+   and the average coarsening ratio, src/AMG/AMG.cu:218-237,
+   src/AMG/matchingAggregation.cu:330-333,406; printed by
+   AMG::Hierarchy::printInfo, src/AMG/AMG.cu:241-251): levels, operator
+   complexity sum_l nnz_l / nnz_0, its W-cycle form sum_l 2^l nnz_l / nnz_0,
+   the mean of n_{l-1} / n_l over the coarsenings, and the bootstrap's
+   estimated ratio once hierarchy k was added (src/BOOT/bootstrap.cu:76).
+   Any output may be NULL. */
+int bcmg_hier_info(bcmg_boot B, int k, int *levels, double *op_cmplx, double *op_wcmplx,
+                   double *avg_cratio, double *ratio_after);
 int bcmg_level_A(bcmg_boot B, int k, int l, int *row, int *col, double *val);
 /* prolongator from level l to l+1: coarse index and value per fine row */
 int bcmg_level_P(bcmg_boot B, int k, int l, int *agg, double *p);
@@ -256,6 +266,26 @@ int bcmg_use_graphs(bcmg_ctx ctx, int enable);
    resets) the summed time and the number of sweeps it covers. */
 int bcmg_time_smoother(bcmg_ctx ctx, int enable);
 int bcmg_smoother_timing(bcmg_ctx ctx, double *total_ms, int *launches);
+/* Phases of bcmg_setup timed with HIP events on the context stream while
+   bcmg_time_setup(ctx, 1) is in force (0 otherwise). The reference times
+   its setup as a whole (bootstrap_time, src/SOLVER/solver.cu:76-87) and the
+   products of the coarsening (TOTAL_MUL_TIME, src/AMG/matchingAggregation.cu:
+   229-248,262-267,421); this splits it by the kernels' roles. */
+typedef struct {
+    double weights_ms;      /* matching weights AH and log (Matching::makeAH, toMaximumProductMatrix) */
+    double matching_ms;     /* suitor matching and symmetrisation (approx_match_gpu_suitor, unsymFix) */
+    double aggregation_ms;  /* aggregates, P, R = P^T, P composition, w restriction (and smoothing of P) */
+    double galerkin_ms;     /* coarse operators R (A P): the reference's TOTAL_MUL_TIME */
+    double smoother_ms;     /* relaxPrepare: diagonal and L1 row sums */
+    double inner_ms;        /* bootstrap inner iterations (Bootstrap::innerIterations) */
+    double hierarchy_ms;    /* each hierarchy's whole build including its inner iterations */
+    double total_ms;        /* k = -1: host wall time of bcmg_setup; k >= 0: hierarchy_ms */
+} bcmg_setup_times;
+int bcmg_time_setup(bcmg_ctx ctx, int enable);
+/* k = -1: sums over the composite; k >= 0: hierarchy k (a hierarchy that
+   shares the first one's storage shows only its inner iterations) */
+int bcmg_setup_timing(bcmg_boot B, int k, bcmg_setup_times *out);
+
 /* `reps` back-to-back launches of y = A x on the context stream between two
    HIP events; avg_ms is the mean time per launch. */
 int bcmg_time_spmv(bcmg_ctx ctx, bcmg_csr A, const double *x, double *y, int reps,

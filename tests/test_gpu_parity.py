@@ -295,6 +295,45 @@ def test_bootstrap_composites(ctx, name, setting):
                         name + " " + setting)
 
 
+def test_setup_phase_timing_and_the_shared_second_hierarchy(ctx):
+    """With setup timing on, every phase of a three-hierarchy bootstrap is
+    timed and the phases fit in the setup's wall time; hierarchy 1 (built
+    from w = 1 like hierarchy 0) is not rebuilt: its summary equals hierarchy
+    0's and only its inner iterations take time. Without timing the phases
+    read 0."""
+    A = OPERATORS["lognormal3d_14"]()
+    dA = bc.Csr(ctx, A)
+    prm = bc.make_params(**SETTINGS["boot3_mult"])
+    bc.time_setup(ctx, True)
+    try:
+        B = bc.Boot(ctx, dA, prm)
+    finally:
+        bc.time_setup(ctx, False)
+    nh, ratio = B.info()
+    assert nh == 3
+    t = B.setup_timing(-1)
+    print("setup phases (ms):", {k: round(v, 3) for k, v in t.items()})
+    named = ("weights_ms", "matching_ms", "aggregation_ms", "galerkin_ms", "smoother_ms",
+             "inner_ms")
+    assert all(t[k] > 0 for k in named), t
+    assert sum(t[k] for k in named) <= t["hierarchy_ms"] * 1.01 + 0.05
+    assert t["hierarchy_ms"] <= t["total_ms"] * 1.01 + 0.05
+    t0, t1, t2 = (B.setup_timing(k) for k in range(3))
+    assert t0["inner_ms"] == 0 and t1["inner_ms"] > 0 and t2["inner_ms"] > 0
+    assert all(t1[k] == 0 for k in named[:-1]), t1
+    assert all(t2[k] > 0 for k in named), t2
+    h0, h1 = B.hier_info(0), B.hier_info(1)
+    for key in ("levels", "op_cmplx", "op_wcmplx", "avg_cratio"):
+        assert h0[key] == h1[key], key
+    assert h0["ratio_after"] == 1.0 and B.hier_info(2)["ratio_after"] == ratio
+    lv = [B.level_size(0, l) for l in range(h0["levels"])]
+    assert abs(h0["op_cmplx"] - sum(z for _, z in lv) / lv[0][1]) <= 1e-15 * h0["op_cmplx"]
+    B.free()
+    B = bc.Boot(ctx, dA, prm)
+    assert all(v == 0 for k, v in B.setup_timing(-1).items() if k != "total_ms")
+    B.free()
+
+
 def test_precond_apply(ctx):
     A = OPERATORS["lognormal3d_10"]()
     n = len(A[0]) - 1

@@ -237,3 +237,63 @@ def test_budget_below_any_lane_fails_the_requests(ctx, boots):
         assert r is not None and r["status"] == bc.ENOMEM
     assert S.info()["queued"] == 0 and S.info()["lanes"] == 0
     S.close()
+
+
+POOL_RTOL = 1e-8
+POOL_SETTINGS = dict(solver_type=0, max_hrc=1, conv_ratio=0.8, aggrsweeps=2, aggrtype=0,
+                     max_levels=39, cycle_type=0, coarse_solver=4, relax_type=4,
+                     relaxnumber_coarse=16, prerelax_sweeps=4, postrelax_sweeps=4, itnlim=2000,
+                     rtol=POOL_RTOL)
+# members of the pool the oracle solves in seconds
+POOL_ORACLE = ("poisson3d_32", "poisson3d_64", "poisson2d_256")
+
+
+@pytest.mark.parametrize("prec", [bc.PREC_FP64, bc.PREC_FP32])
+def test_serving_pool_of_eight_operators(ctx, prec):
+    """The serving workload's pool (2D 5-point 256^2 and 1024^2 with eps 1
+    and 0.01, 3D 7-point 32^3 to 128^3, log-normal 64^3) at the drop-in
+    settings with rtol 1e-8: 40 requests with distinct seeded right-hand
+    sides, submitted a few at a time while earlier ones run. Every result
+    meets rtol on its true residual and equals the single solve of its
+    right-hand side bit for bit; on the 32^3, 64^3 and 256^2 members the
+    FP64 results also agree with the oracle."""
+    from bootcmatchg_amd import serving as sv
+    from oracle import oracle as orc
+    from common import check_fcg_vs_oracle
+    pool = sv.Pool(bc, ctx, POOL_SETTINGS, prec)
+    names = [o["name"] for o in pool.ops]
+    assert len(names) == 8
+    rng = np.random.default_rng(4)
+    opsel = rng.integers(0, 8, 40)
+    opsel[:8] = np.arange(8)                       # every operator served
+    reqs = [(int(op), ops.uniform_pm1(5000 + i, pool.ops[int(op)]["n"]))
+            for i, op in enumerate(opsel)]
+    arrivals = np.sort(rng.integers(0, 60, len(reqs)))
+    boots = [(o["B"], o["n"]) for o in pool.ops]
+    print("pool", names, "precision", prec, flush=True)
+    out, _ = serve(ctx, boots, reqs, arrivals, slots=16)
+    oracle_done = set()
+    mats = {}
+    for i, ((op, b), (r, x)) in enumerate(zip(reqs, out)):
+        o = pool.ops[op]
+        assert r["status"] == 0, (i, o["name"], r)
+        if op not in mats:
+            A = o["A"].download()
+            mats[op] = (A, ops.to_scipy(A))
+        A, S = mats[op]
+        tr = np.linalg.norm(b - S @ x) / np.linalg.norm(b)
+        assert tr <= POOL_RTOL * (1 + 1e-3), (i, o["name"], tr)
+        x1 = bc.DeviceArray(ctx, o["n"])
+        it1, res1 = o["B"].fcg(bc.DeviceArray.from_numpy(ctx, b), x1)
+        assert (it1, res1) == (r["iterations"], r["residual"]), (i, o["name"])
+        assert np.array_equal(bits(x1.numpy()), bits(x)), (i, o["name"])
+        if prec == bc.PREC_FP64 and o["name"] in POOL_ORACLE and o["name"] not in oracle_done:
+            oracle_done.add(o["name"])
+            rc, xo, ro = orc.solve(A, b, orc.params(**POOL_SETTINGS))
+            assert rc == 0
+            check_fcg_vs_oracle(x, r["iterations"], xo, ro.iterations, A, b, POOL_RTOL,
+                                "served %s" % o["name"])
+    if prec == bc.PREC_FP64:
+        assert oracle_done == set(POOL_ORACLE)
+    for o in pool.ops:
+        o["B"].free()

@@ -104,13 +104,33 @@ def test_no_device_is_reported_not_faked():
     assert e.value.code == bc.ENODEV
 
 
-def test_facade_header_compiles_against_the_reference_usage(tmp_path):
-    """The reference's solve_example body compiles against BootCMatchG.h."""
-    src = os.path.join(ROOT, "bootcmatchg_amd", "csrc", "solve_example.cpp")
-    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-I",
-                    os.path.join(ROOT, "include"), src], check=True)
-    subprocess.run(["gcc", "-std=c99", "-fsyntax-only", "-x", "c",
+REF_EXAMPLE = "/root/reference/src/solve_example.cu"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_EXAMPLE),
+                    reason="the reference tree is not present on this machine")
+def test_reference_driver_compiles_and_links_against_the_facade(tmp_path):
+    """The reference's own command-line driver, read where it lies in the
+    reference tree, compiles unchanged against include/BootCMatchG.h and links
+    against libbcmg.so (built in the temporary directory, not run: that needs
+    a GPU)."""
+    exe = str(tmp_path / "solve_example")
+    # fed on stdin, so that its #include "BootCMatchG.h" finds this header and
+    # not the CUDA one beside it
+    with open(REF_EXAMPLE) as src:
+        subprocess.run(["g++", "-std=c++17", "-x", "c++", "-I", os.path.join(ROOT, "include"),
+                        "-o", exe, "-", "-x", "none", "-L", os.path.dirname(bc.LIB_PATH),
+                        "-lbcmg"], stdin=src, check=True)
+    assert os.path.exists(exe)
+
+
+def test_headers_and_cli_compile():
+    """bcmg.h is plain C; the command-line solver compiles with warnings as errors."""
+    subprocess.run(["gcc", "-std=c99", "-fsyntax-only", "-Wall", "-x", "c",
                     os.path.join(ROOT, "include", "bcmg.h")], check=True)
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I",
+                    os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "bootcmatchg_amd", "csrc", "bcmg_cli.cpp")], check=True)
 
 
 def test_operator_generators_are_deterministic():
@@ -149,3 +169,89 @@ def test_extension_entry_points_validate_arguments():
                          ctypes.c_longlong(0), ctypes.byref(vp())) == bc.EINVAL
     assert L.bcmg_level_Pinfo(vp(), 0, 0, None, None, None) == bc.EINVAL
     assert L.bcmg_level_Pcsr(vp(), 0, 0, None, None, None) == bc.EINVAL
+
+
+READER_SRC = r'''
+#include "BootCMatchG.h"
+int main(int argc, char **argv)
+{
+    CSR *A = readMatrixFromFile(argv[1], std::atoi(argv[2]), false);
+    std::printf("%d %d %d\n", A->n, A->m, A->nnz);
+    for (int i = 0; i <= A->n; i++) std::printf("%d\n", A->row[i]);
+    for (int k = 0; k < A->nnz; k++) std::printf("%d %.17g\n", A->col[k], A->val[k]);
+    CSRm::free(A);
+    return 0;
+}
+'''
+
+
+@pytest.fixture(scope="module")
+def reader_exe(tmp_path_factory):
+    d = tmp_path_factory.mktemp("reader")
+    (d / "r.cpp").write_text(READER_SRC)
+    exe = str(d / "r")
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(d / "r.cpp"),
+                    "-o", exe, "-L", os.path.dirname(bc.LIB_PATH), "-lbcmg",
+                    "-Wl,-rpath," + os.path.dirname(bc.LIB_PATH)], check=True)
+    return exe
+
+
+def read_with(exe, path, m_type):
+    out = subprocess.run([exe, path, str(m_type)], capture_output=True, text=True, check=True,
+                         timeout=60).stdout.split("\n")
+    n, m, nnz = map(int, out[0].split())
+    row = np.array([int(v) for v in out[1:n + 2]], dtype=np.int32)
+    ent = [l.split() for l in out[n + 2:n + 2 + nnz]]
+    col = np.array([int(c) for c, _ in ent], dtype=np.int32)
+    val = np.array([float(v) for _, v in ent])
+    return n, m, row, col, val
+
+
+def test_matrix_readers_give_the_sorted_csr(reader_exe, tmp_path):
+    """readMatrixFromFile on the host (no device): a symmetric MatrixMarket
+    file (lower triangle), a general one in shuffled order and an MTX2 file in
+    shuffled order all give the operator's CSR (columns ascending)."""
+    from bootcmatchg_amd import operators as ops
+    A = ops.lognormal3d(5, 4.0, seed=3)
+    row, col, val = A
+    n = len(row) - 1
+    r = np.repeat(np.arange(n), np.diff(row))
+    low = col <= r
+    p1 = str(tmp_path / "sym.mtx")
+    with open(p1, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real symmetric\n%c\n%c\n")
+        f.write("%d %d %d\n" % (n, n, int(low.sum())))
+        for i, j, v in zip(r[low], col[low], val[low]):
+            f.write("%d %d %.17g\n" % (i + 1, j + 1, v))
+    perm = np.random.default_rng(0).permutation(len(col))
+    p2 = str(tmp_path / "gen.mtx")
+    with open(p2, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real general\n")
+        f.write("%d %d %d\n" % (n, n, len(col)))
+        for k in perm:
+            f.write("%d %d %.17g\n" % (r[k] + 1, col[k] + 1, val[k]))
+    p3 = str(tmp_path / "a.mtx2")
+    with open(p3, "w") as f:
+        f.write("%d %d\n" % (n, len(col)))
+        f.write(" ".join(str(v) for v in r[perm] + 1) + "\n")
+        f.write(" ".join(str(v) for v in col[perm] + 1) + "\n")
+        f.write(" ".join("%.17g" % v for v in val[perm]) + "\n")
+    for path, m_type in ((p1, 0), (p2, 0), (p3, 1)):
+        nn, m, rr, cc, vv = read_with(reader_exe, path, m_type)
+        assert (nn, m) == (n, n), path
+        assert np.array_equal(rr, row) and np.array_equal(cc, col), path
+        assert np.array_equal(vv, val), path
+
+
+def test_matrix_reader_errors(reader_exe, tmp_path):
+    """A matrix type other than 0/1 exits with the reference's message and
+    status 1; an MTX2 index out of range and a short file exit with 255."""
+    p = str(tmp_path / "x")
+    open(p, "w").write("2 3\n1 2 3\n1 2 2\n1 1 1\n")
+    q = subprocess.run([reader_exe, p, "2"], capture_output=True, text=True, timeout=60)
+    assert q.returncode == 1 and "-F/--inputype" in q.stdout
+    q = subprocess.run([reader_exe, p, "1"], capture_output=True, text=True, timeout=60)
+    assert q.returncode == 255
+    open(p, "w").write("2 3\n1 2 2\n1 2 2\n1 1\n")
+    q = subprocess.run([reader_exe, p, "1"], capture_output=True, text=True, timeout=60)
+    assert q.returncode == 255
