@@ -529,6 +529,16 @@ def relax(ctx, A, d_dev, f_dev, u_dev, u2_dev, sweeps):
     _check(lib().bcmg_relax(ctx.h, A.h, d_dev.ptr, f_dev.ptr, u_dev.ptr, u2_dev.ptr, sweeps))
 
 
+# rows of the static device array that holds bcmg_relax_noalloc's second iterate
+RELAX_NOALLOC_MAX_ROWS = 4194304
+
+
+def relax_noalloc(ctx, A, d_dev, f_dev, u_dev, sweeps):
+    """relax() without a workspace and without device allocation; operators
+    of at most RELAX_NOALLOC_MAX_ROWS rows."""
+    _check(lib().bcmg_relax_noalloc(ctx.h, A.h, d_dev.ptr, f_dev.ptr, u_dev.ptr, sweeps))
+
+
 def weights(ctx, A, w_dev, W_dev):
     _check(lib().bcmg_weights(ctx.h, A.h, w_dev.ptr, W_dev.ptr))
 

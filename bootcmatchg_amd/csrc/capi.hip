@@ -9,10 +9,18 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
+
+/* The second iterate of bcmg_relax_noalloc: one array per GPU, alive until
+   the process ends. Host code never uses this name as an address: each
+   device's copy is reached through the pointer hipGetSymbolAddress returns
+   for it (scratch_address below). */
+static __device__ __align__(256) double scratch[BCMG_RELAX_NOALLOC_MAX_ROWS];
 
 struct bcmg_ctx_s {
     bcmg::Ctx c;
+    double *scratch = nullptr; /* this device's copy of `scratch` */
 };
 
 struct bcmg_csr_s {
@@ -112,6 +120,24 @@ const Level &get_level(bcmg_boot B, int k, int l)
     return B->B.H[k]->L[l];
 }
 
+/* Device address of `scratch` on the current device, resolved once per
+   device and kept: every GPU holds its own copy at its own address. */
+double *scratch_address(int device)
+{
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex mu;
+    static double *addr[MAX_DEVICES] = {};
+    REQUIRE(device >= 0 && device < MAX_DEVICES, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void *p = nullptr;
+        BCMG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        REQUIRE(p != nullptr, "no device address for the relaxation scratch");
+        addr[device] = static_cast<double *>(p);
+    }
+    return addr[device];
+}
+
 void d2h(bcmg_ctx ctx, void *dst, const void *src, size_t bytes)
 {
     if (!bytes || !dst) return;
@@ -209,6 +235,7 @@ int ctx_create(int device, hipStream_t ext, bcmg_ctx *out)
     try {
         ctx->c.device = device;
         BCMG_HIP(hipSetDevice(device));
+        ctx->scratch = scratch_address(device);
         if (ext) {
             int sdev = -1;
             BCMG_HIP(hipStreamGetDevice(ext, &sdev));
@@ -749,6 +776,29 @@ int bcmg_relax(bcmg_ctx ctx, bcmg_csr A, const double *d, const double *f, doubl
     API_BEGIN
     CtxScope scope_(ctx);
     double *x = u, *y = u2;
+    for (int k = 0; k < sweeps; k++) {
+        launch_rows(ctx->c.stream, A->A, OP_JACOBI, x, f, d, y);
+        std::swap(x, y);
+    }
+    if (x != u) launch_copy(ctx->c.stream, A->A.n, x, u);
+    BCMG_HIP(hipStreamSynchronize(ctx->c.stream));
+    API_END
+}
+
+int bcmg_relax_noalloc(bcmg_ctx ctx, bcmg_csr A, const double *d, const double *f, double *u,
+                       int sweeps)
+{
+    if (!ctx || !A || !d || !f || !u || sweeps < 0) return BCMG_EINVAL;
+    if (A->A.n > BCMG_RELAX_NOALLOC_MAX_ROWS) {
+        set_last_error("bcmg_relax_noalloc: " + std::to_string(A->A.n) +
+                       " rows exceed the static scratch of " +
+                       std::to_string(BCMG_RELAX_NOALLOC_MAX_ROWS));
+        return BCMG_EINVAL;
+    }
+    API_BEGIN
+    /* no pool scope: nothing here allocates; the kernels are bcmg_relax's */
+    BCMG_HIP(hipSetDevice(ctx->c.device));
+    double *x = u, *y = ctx->scratch;
     for (int k = 0; k < sweeps; k++) {
         launch_rows(ctx->c.stream, A->A, OP_JACOBI, x, f, d, y);
         std::swap(x, y);

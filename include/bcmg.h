@@ -221,6 +221,18 @@ int bcmg_residual(bcmg_ctx ctx, bcmg_csr A, const double *x, const double *f, do
    Replaces jacobi_adaptive_miniwarp (src/AMG/relaxation.cu:211-285). */
 int bcmg_relax(bcmg_ctx ctx, bcmg_csr A, const double *d, const double *f, double *u,
                double *u2, int sweeps);
+/* bcmg_relax without the u2 argument and without any device allocation:
+   the second iterate lives in a static device array of
+   BCMG_RELAX_NOALLOC_MAX_ROWS doubles (32 MiB) that each GPU the library
+   runs on holds until the process ends. No counterpart in the reference:
+   it is for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Same kernels and bits as bcmg_relax. An operator with more rows returns
+   BCMG_EINVAL. All calls on one GPU share that GPU's array: calls that
+   overlap on one GPU (two threads or two contexts) race. */
+#define BCMG_RELAX_NOALLOC_MAX_ROWS 4194304
+int bcmg_relax_noalloc(bcmg_ctx ctx, bcmg_csr A, const double *d, const double *f, double *u,
+                       int sweeps);
 /* matching weights W aligned with A's nnz (diagonal slots 0). Replaces
    Matching::makeAH + toMaximumProductMatrix (src/AMG/matching.cu:293-472). */
 int bcmg_weights(bcmg_ctx ctx, bcmg_csr A, const double *w, double *W);
