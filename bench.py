@@ -44,11 +44,11 @@ SETTINGS = dict(solver_type=0, max_hrc=1, conv_ratio=0.8, aggrsweeps=2, aggrtype
 
 
 # Measured HBM bytes per level-0 sweep launch at 160^3 (rocprofv3 FETCH_SIZE
-# x 1.999 + WRITE_SIZE x 1.000, factors from a plain copy kernel of known
+# x 1.9987 + WRITE_SIZE x 1.000, factors from a plain copy kernel of known
 # size; summary: profiles/jacobi_sweep_poisson160_traffic.md). Re-measured
 # whenever the sweep kernel changes; reported only for the 160^3 workload
-# streamed with 16-bit columns (the library's choice for it).
-SWEEP_TRAFFIC_BYTES_160 = 443.4e6
+# streamed in the stencil-pattern column form (the library's choice for it).
+SWEEP_TRAFFIC_BYTES_160 = 372.3e6
 
 
 C2_GRID = 128
@@ -364,9 +364,19 @@ def spmv_bytes(n, nnz, cb=4):
     return (8 + cb) * nnz + 4 * (n + 1) + 2 * 8 * n
 
 
+def streamed_bytes(n, nnz, cb, vectors):
+    """Bytes a level-0 row kernel streams in the column form it reads: 8-byte
+    values, then cb-byte columns and the row pointers, or (cb = 0, the
+    stencil pattern) one mask byte per row and two row pointers per 256-row
+    block; plus `vectors` 8-byte vectors of n."""
+    if cb == 0:
+        return 8 * nnz + n + 8 * ((n + 255) // 256) + vectors * 8 * n
+    return (8 + cb) * nnz + 4 * (n + 1) + vectors * 8 * n
+
+
 # Measured HBM bytes per level-0 SpMV launch at 160^3 (same PMC runs and
 # factors as the sweep: profiles/jacobi_sweep_poisson160_traffic.md).
-SPMV_TRAFFIC_BYTES_160 = 375.8e6
+SPMV_TRAFFIC_BYTES_160 = 305.9e6
 MALL_BYTES = 256 << 20
 
 
@@ -562,7 +572,9 @@ def main():
     dy = bc.DeviceArray(ctx, n)
     spmv_ms = bc.time_spmv(ctx, dA, db, dy, 50)
     cb = dA.column_bytes
-    spmv_gbs = spmv_bytes(n, nnz, cb) / (spmv_ms * 1e-3) / 1e9
+    # algorithmic bytes count 32-bit columns (the operator's own format); the
+    # library may stream a 16-bit copy (cb = 2) and then moves fewer bytes
+    spmv_gbs = spmv_bytes(n, nnz) / (spmv_ms * 1e-3) / 1e9
 
     # whole FCG solve against the HBM roofline (cached hierarchy: no setup)
     sb = solve_bytes([tuple(z) for z in rec[-1]["levels"]], rec[-1]["iterations"],
@@ -574,11 +586,12 @@ def main():
                 "frac": cached_rate * sb / (HBM_PEAK_GBS * 1e9),
                 "note": "FCG with a cached hierarchy; bytes counted the reference "
                         "algorithm's way, 32-bit columns on every level (the library "
-                        "streams 16-bit ones where they fit, so it moves fewer bytes "
-                        "than this count); setup excluded, time only"}
+                        "streams level 0 in its stencil-pattern form, one mask byte per "
+                        "row, so it moves fewer bytes than this count); setup excluded, "
+                        "time only"}
 
     sweep_avg_ms = sm_ms / max(sm_launches, 1)
-    sweep_b = sweep_bytes(n, nnz, cb)
+    sweep_b = sweep_bytes(n, nnz)
     achieved = sweep_b / (sweep_avg_ms * 1e-3) / 1e9
 
     cpu = None
@@ -650,18 +663,28 @@ def main():
         "cached_hierarchy_solves_per_s": args.steps / (tc1 - tc0),
         "spmv_gbs": spmv_gbs,
         "spmv_frac_of_hbm": spmv_gbs / HBM_PEAK_GBS,
-        "spmv": {"kernel": "level-0 y = A x (k_rows_staged<OP_SPMV>)",
-                 "bytes_per_launch": spmv_bytes(n, nnz, cb), "avg_launch_ms": spmv_ms,
+        "spmv": {"kernel": "level-0 y = A x (k_rows_pattern<OP_SPMV> in the stencil-pattern "
+                           "form, else k_rows_staged<OP_SPMV>)",
+                 "bytes_per_launch": spmv_bytes(n, nnz), "avg_launch_ms": spmv_ms,
                  "achieved_gbs": spmv_gbs, "column_bytes": cb,
-                 "traffic_bytes": SPMV_TRAFFIC_BYTES_160 if (N == GRID and cb == 2) else None,
+                 "streamed_bytes_per_launch": streamed_bytes(n, nnz, cb, 2),
+                 "traffic_bytes": SPMV_TRAFFIC_BYTES_160 if (N == GRID and cb == 0) else None,
                  "working_set_bytes": spmv_bytes(n, nnz, cb), "mall_bytes": MALL_BYTES},
         "solve_roofline": solve_rl,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": SWEEP_TRAFFIC_BYTES_160 if (N == GRID and cb == 2) else None,
-                     "kernel": "level-0 Jacobi/L1 sweep (k_rows_staged<OP_JACOBI, non-temporal>)",
+                     "traffic": SWEEP_TRAFFIC_BYTES_160 if (N == GRID and cb == 0) else None,
+                     "kernel": "level-0 Jacobi/L1 sweep (k_rows_pattern<OP_JACOBI, non-temporal> "
+                               "in the stencil-pattern form, else k_rows_staged<OP_JACOBI>)",
                      "bytes_per_launch": sweep_b, "avg_launch_ms": sweep_avg_ms,
-                     "launches": sm_launches, "column_bytes": cb},
+                     "launches": sm_launches, "column_bytes": cb,
+                     "streamed_bytes_per_launch": streamed_bytes(n, nnz, cb, 4),
+                     "note": "achieved = algorithmic bytes (8-byte values, 32-bit columns, "
+                             "row pointers, u gathered once, f, d, u') / event-timed launch; "
+                             "the library streams the column form column_bytes names "
+                             "(0: the stencil pattern, one mask byte per row), "
+                             "streamed_bytes_per_launch, and traffic is the measured HBM "
+                             "bytes"},
         "cpu_baseline": cpu,
         "precision_form": precision,
         "multi_rhs": multi,

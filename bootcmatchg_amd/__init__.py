@@ -241,11 +241,16 @@ class Csr:
 
     @property
     def column_bytes(self):
-        """Bytes per column index the streamed row kernels read (2 with the
-        16-bit column-minus-row copy, else 4)."""
+        """Bytes per column index the streamed row kernels read (0 with the
+        stencil pattern, 2 with the 16-bit column-minus-row copy, else 4)."""
         b = ctypes.c_int()
         _check(lib().bcmg_csr_column_bytes(self.h, ctypes.byref(b)))
         return b.value
+
+    def set_columns(self, nbytes):
+        """Restrict the row kernels to a column form: 0 the stencil pattern,
+        2 the 16-bit offsets, 4 the int32 columns (same results)."""
+        _check(lib().bcmg_csr_set_columns(self.h, nbytes))
 
     def download(self):
         row = np.empty(self.n + 1, np.int32)

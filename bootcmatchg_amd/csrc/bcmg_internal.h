@@ -120,6 +120,9 @@ struct DBuf {
 /* which stored values a row kernel reads */
 enum MatType { MT_F64 = 0, MT_F32 = 1, MT_BF16 = 2 };
 
+/* most distinct column offsets of a stencil-pattern operator */
+constexpr int PAT_MAX = 8;
+
 /* CSR, 0-based, int32 indices, FP64 values; columns ascending with a diagonal
    entry in every row (the layout of src/matrix/CSR.h:13-28). */
 struct DevCsr {
@@ -137,6 +140,21 @@ struct DevCsr {
        type they use would be non-temporal */
     DBuf<short> col16;
     bool narrow = false;
+    /* stencil pattern of the same operators when their column offsets take
+       at most PAT_MAX distinct values: poff holds them ascending and bit k of
+       pmask[r] is set when row r has column r + poff[k]; a row's entries are
+       its set bits in ascending k, which is the CSR order, so the values stay
+       where they are and the row kernels need neither the columns nor the
+       row pointers beyond one per 256-row block (pk = 0: no pattern) */
+    DBuf<unsigned char> pmask;
+    int pk = 0;
+    int poff[PAT_MAX] = {};
+    bool cols_analysed = false; /* narrow copy and pattern considered */
+    /* the narrowest column form the row kernels may read: 2 the pattern,
+       1 the 16-bit offsets, 0 the int32 columns (bcmg_csr_set_columns) */
+    int col_mode = 2;
+    bool use_pattern() const { return pk > 0 && col_mode >= 2; }
+    bool use_narrow() const { return narrow && col_mode >= 1; }
     int group = 8;          /* lanes per row of the row-group kernels */
     int tile = 0;           /* max nnz of a 256-row block; 0 = not analysed, -1 pending */
     int maxrow = 0;         /* longest row (valid with tile) */
@@ -152,7 +170,8 @@ void analyze_rows(Ctx &c, DevCsr &A);
 void launch_block_nnz(Ctx &c, DevCsr &A);
 void resolve_tiles(Ctx &c, const std::vector<DevCsr *> &mats);
 /* build the narrow column copy of every square operator in mats above the
-   non-temporal size whose column offsets all fit 16 bits (one round trip
+   non-temporal size whose column offsets all fit 16 bits, and its stencil
+   pattern when those offsets take at most PAT_MAX values (one round trip
    for the lot) */
 void make_narrow(Ctx &c, const std::vector<DevCsr *> &mats);
 
@@ -323,6 +342,10 @@ template <typename VT>
 void launch_jacobi_zero(hipStream_t s, int n, const VT *f, const VT *d, VT *y);
 template <typename VT>
 void launch_prolong(hipStream_t s, int n, const int *agg, const VT *p, const VT *xc, VT *x);
+/* y = x + P x_c with y another buffer than x */
+template <typename VT>
+void launch_prolong_to(hipStream_t s, int n, const int *agg, const VT *p, const VT *xc,
+                       const VT *x, VT *y);
 template <typename VT>
 void launch_fill(hipStream_t s, int n, VT v, VT *x);
 template <typename VT>
@@ -378,10 +401,43 @@ bool relax_block_ok(const DevCsr &A);
 /* whether the first two sweeps from zero run as one OP_JACOBI_Z pass on A */
 template <typename VT>
 bool fuse_zero_sweeps(const DevCsr &A, MatType mt);
+/* work folded into a single-column one-workgroup relaxation (same
+   arithmetic as its separate launches): with agg set, the prolongation
+   x = x_in + P x_c (x_in = 0 when zero) before the sweeps; with fc set, the
+   residual restricted to the next level after them, fc = R (f - A x) over
+   the nc rows of R (CSR rrp, rci, rv) */
+template <typename VT>
+struct BlockFuse {
+    const int *agg = nullptr;
+    const VT *p = nullptr, *xc = nullptr;
+    int nc = 0;
+    const int *rrp = nullptr, *rci = nullptr;
+    const VT *rv = nullptr;
+    VT *fc = nullptr;
+};
 template <typename VT>
 void launch_relax_block(hipStream_t s, const DevCsr &A, MatType mt, const VT *d, const VT *f,
                         const VT *xin, VT *xout, int sweeps, bool zero, int ncols = 1,
-                        long cs = 0, int rs = 1, const int *act = nullptr);
+                        long cs = 0, int rs = 1, const int *act = nullptr,
+                        const BlockFuse<VT> &fu = BlockFuse<VT>());
+/* whether the row kernels sum A's rows in storage order from 0 (the staged
+   kernels; otherwise the row-group kernels), the order fused kernels repeat */
+template <typename VT>
+bool rows_sequential(const DevCsr &A);
+/* y = one Jacobi/L1 sweep from x + P x_c (the prolongation formed where it
+   is read; the bits of a prolongation and a sweep launch); false when A
+   takes the row-group kernels (then nothing is launched) */
+template <typename VT>
+bool launch_jacobi_prolong(hipStream_t s, const DevCsr &A, MatType mt, const int *agg,
+                           const VT *p, const VT *xc, const VT *x, const VT *f, const VT *d,
+                           VT *y);
+/* the residual of a small cache-resident level restricted to the next one
+   in one launch, y = R (f - A x), the bits of the residual and restriction
+   launches; false when A or R takes the row-group kernels (then nothing is
+   launched) */
+template <typename VT>
+bool launch_resid_restrict(hipStream_t s, const DevCsr &A, MatType mt, const DevCsr &R,
+                           const VT *x, const VT *f, VT *y);
 
 /* ---------------------------------------- multi-column kernels (kernels_multi.hip)
    Column blocks: column j of a level's block at base + j * ld. Each kernel

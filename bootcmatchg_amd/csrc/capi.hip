@@ -398,7 +398,30 @@ int bcmg_csr_info(bcmg_csr A, int *n, int *nnz)
 int bcmg_csr_column_bytes(bcmg_csr A, int *bytes)
 {
     if (!A || !bytes) return BCMG_EINVAL;
-    *bytes = A->A.narrow ? 2 : 4;
+    *bytes = A->A.use_pattern() ? 0 : A->A.use_narrow() ? 2 : 4;
+    return BCMG_OK;
+}
+
+int bcmg_csr_set_columns(bcmg_csr A, int bytes)
+{
+    if (!A) return BCMG_EINVAL;
+    if (bytes == 4) {
+        A->A.col_mode = 0;
+    } else if (bytes == 2) {
+        if (!A->A.narrow) {
+            set_last_error("bcmg_csr_set_columns: no 16-bit column copy for this matrix");
+            return BCMG_EUNSUP;
+        }
+        A->A.col_mode = 1;
+    } else if (bytes == 0) {
+        if (A->A.pk == 0) {
+            set_last_error("bcmg_csr_set_columns: no stencil pattern for this matrix");
+            return BCMG_EUNSUP;
+        }
+        A->A.col_mode = 2;
+    } else {
+        return BCMG_EINVAL;
+    }
     return BCMG_OK;
 }
 

@@ -392,6 +392,29 @@ void relax_level(Ctx &c, const Level &lv, MatType mt, int type, int sweeps, cons
     }
 }
 
+/* levels of at most this many rows restrict their residual in the launch
+   that forms it (where a launch costs more than the work) */
+constexpr int SMALL_LEVEL_ROWS = 65536;
+
+/* R's values in the cycle's type */
+template <typename VT>
+const VT *rvals(const DevCsr &R);
+template <>
+const double *rvals<double>(const DevCsr &R) { return R.val.get(); }
+template <>
+const float *rvals<float>(const DevCsr &R) { return R.val32.get(); }
+
+/* Whether `sweeps` post-smoothing launches that start from iterate a (other
+   buffer b) leave the result outside fo: relax_level redirects the last
+   sweep into fo unless that sweep reads fo, and no sweep leaves it in a. */
+template <typename VT>
+bool post_lands_off(int sweeps, const VT *a, const VT *b, const VT *fo)
+{
+    if (!fo) return false;
+    if (sweeps == 0) return a != fo;
+    return ((sweeps - 1) % 2 == 0 ? a : b) == fo;
+}
+
 /* inner-iteration tolerance of the K-cycle (src/SOLVER/GAMG_cycle.cu:5) */
 constexpr double KCYCLE_RTOL = 0.25;
 
@@ -463,16 +486,42 @@ void cycle(CycleRun<VT> &R, int l, const VT *fin)
                         top ? R.final_out : nullptr, top);
         return;
     }
-    relax_level<VT>(c, lv, mt, B.prm.relax_type, B.prm.prerelax_sweeps, f, s, nullptr, top);
     const DevCsr &A = lv.mat();
-    const VT *rsrc = f;
-    if (!s.zero) {
-        launch_rows<VT>(c.stream, A, mt, OP_RESID, s.x, f, nullptr, W.res.get());
-        rsrc = W.res.get();
-    }
     /* R holds the prolongator values: FP32 whenever the cycle runs in FP32 */
-    launch_rows<VT>(c.stream, lv.R, std::is_same<VT, double>::value ? MT_F64 : MT_F32, OP_SPMV,
-                    rsrc, nullptr, nullptr, W.f[l].get());
+    const MatType rmt = std::is_same<VT, double>::value ? MT_F64 : MT_F32;
+    VT *fc = W.f[l].get();
+    const bool blk = relax_block_ok<VT>(A) && !(top && c.time_smoother);
+    bool restricted = false;
+    if (blk && B.prm.prerelax_sweeps > 0 && rows_sequential<VT>(lv.R)) {
+        /* a one-workgroup level: pre-smoothing, residual and restriction in
+           one launch (the separate launches' bits) */
+        BlockFuse<VT> fu;
+        fu.nc = lv.R.n;
+        fu.rrp = lv.R.row.get();
+        fu.rci = lv.R.col.get();
+        fu.rv = rvals<VT>(lv.R);
+        fu.fc = fc;
+        launch_relax_block<VT>(c.stream, A, mt, divisor<VT>(lv, B.prm.relax_type), f, s.x, s.x,
+                               B.prm.prerelax_sweeps, s.zero, 1, 0, 1, nullptr, fu);
+        s.zero = false;
+        restricted = true;
+    } else {
+        relax_level<VT>(c, lv, mt, B.prm.relax_type, B.prm.prerelax_sweeps, f, s, nullptr, top);
+    }
+    if (!restricted) {
+        const VT *rsrc = f;
+        if (!s.zero) {
+            /* a small level: residual and restriction in one launch */
+            if (A.n <= SMALL_LEVEL_ROWS &&
+                launch_resid_restrict<VT>(c.stream, A, mt, lv.R, s.x, f, fc)) {
+                restricted = true;
+            } else {
+                launch_rows<VT>(c.stream, A, mt, OP_RESID, s.x, f, nullptr, W.res.get());
+                rsrc = W.res.get();
+            }
+        }
+        if (!restricted) launch_rows<VT>(c.stream, lv.R, rmt, OP_SPMV, rsrc, nullptr, nullptr, fc);
+    }
     LevelState<VT> &sc = R.st[l];
     sc.zero = true;
     if (B.prm.cycle_type == 3 && l != L - 1) {
@@ -483,8 +532,9 @@ void cycle(CycleRun<VT> &R, int l, const VT *fin)
             if (l == L - 1) break;
         }
     }
+    bool prolong_in_block = false, prolong_in_sweep = false;
     if (!sc.zero) {
-        if (s.zero) {
+        if (s.zero && !(blk && !lv.smoothed && B.prm.postrelax_sweeps > 0)) {
             launch_fill<VT>(c.stream, A.n, VT(0), s.x);
             s.zero = false;
         }
@@ -493,9 +543,58 @@ void cycle(CycleRun<VT> &R, int l, const VT *fin)
             launch_rows<VT>(c.stream, lv.Pneg, std::is_same<VT, double>::value ? MT_F64 : MT_F32,
                             OP_RESID, sc.x, s.x, nullptr, s.x2);
             std::swap(s.x, s.x2);
+        } else if (blk && B.prm.postrelax_sweeps > 0) {
+            /* a one-workgroup level: the prolongation folded into the
+               post-smoothing launch (below) */
+            prolong_in_block = true;
+        } else if (!blk && A.n <= SMALL_LEVEL_ROWS && B.prm.postrelax_sweeps > 0 &&
+                   !(top && c.time_smoother) && rows_sequential<VT>(A)) {
+            /* a small level: the prolongation folded into the first
+               post-sweep (below) */
+            prolong_in_sweep = true;
+        } else if (top && post_lands_off(B.prm.postrelax_sweeps, s.x, s.x2, R.final_out) &&
+                   !post_lands_off(B.prm.postrelax_sweeps, s.x2, s.x, R.final_out) && !blk) {
+            /* into the other iterate: the post-smoothing's last sweep then
+               writes the preconditioner's output itself (no copy after) */
+            launch_prolong_to<VT>(c.stream, A.n, lv.agg.get(), pvals<VT>(lv), sc.x, s.x, s.x2);
+            std::swap(s.x, s.x2);
         } else {
             launch_prolong<VT>(c.stream, A.n, lv.agg.get(), pvals<VT>(lv), sc.x, s.x);
         }
+    }
+    if (prolong_in_block) {
+        BlockFuse<VT> fu;
+        fu.agg = lv.agg.get();
+        fu.p = pvals<VT>(lv);
+        fu.xc = sc.x;
+        VT *fo = top ? R.final_out : nullptr;
+        VT *out = (fo && s.x != fo) ? fo : s.x;
+        launch_relax_block<VT>(c.stream, A, mt, divisor<VT>(lv, B.prm.relax_type), f, s.x, out,
+                               B.prm.postrelax_sweeps, s.zero, 1, 0, 1, nullptr, fu);
+        if (out != s.x) {
+            s.x2 = s.x;
+            s.x = out;
+        }
+        s.zero = false;
+        return;
+    }
+    if (prolong_in_sweep) {
+        /* the fused sweep writes the other iterate, or the preconditioner's
+           output when that saves the copy after the remaining sweeps */
+        const int rest = B.prm.postrelax_sweeps - 1;
+        VT *fo = top ? R.final_out : nullptr;
+        VT *out = s.x2;
+        if (fo && fo != s.x && post_lands_off(rest, s.x2, s.x, fo) &&
+            !post_lands_off(rest, fo, s.x, fo))
+            out = fo;
+        if (launch_jacobi_prolong<VT>(c.stream, A, mt, lv.agg.get(), pvals<VT>(lv), sc.x, s.x, f,
+                                      divisor<VT>(lv, B.prm.relax_type), out)) {
+            s.x2 = s.x;
+            s.x = out;
+            relax_level<VT>(c, lv, mt, B.prm.relax_type, rest, f, s, fo, top);
+            return;
+        }
+        launch_prolong<VT>(c.stream, A.n, lv.agg.get(), pvals<VT>(lv), sc.x, s.x);
     }
     relax_level<VT>(c, lv, mt, B.prm.relax_type, B.prm.postrelax_sweeps, f, s,
                     top ? R.final_out : nullptr, top);

@@ -211,6 +211,114 @@ __device__ __forceinline__ VT staged_row_sum(const IT *__restrict__ lcol,
     return t;
 }
 
+/* ------------------------------------------------ stencil-pattern rows */
+
+/* the ascending column offsets of a pattern operator (kernel argument) */
+struct PatOffsets {
+    int d[PAT_MAX];
+};
+
+/* Phase 1 of the pattern kernels: the block's values [e0, e0 + cnt) into
+   LDS at lval + (e - stage_base(e0)), the copy of stage_entries without the
+   columns (16-byte chunks, UV in flight per lane, partial chunks element by
+   element, loaded first and stored last). */
+template <typename MT, bool NT>
+__device__ __forceinline__ void stage_values(const MT *__restrict__ av, int e0, int cnt,
+                                             MT *__restrict__ lval)
+{
+    const Chunks<MT> sv(e0, cnt);
+    const int t = threadIdx.x;
+    constexpr int UV = 4;
+    MT vh = 0, vt = 0;
+    if (sv.head()) vh = ldm<NT>(av + e0 + t);
+    if (sv.tail()) vt = ldm<NT>(av + sv.b + t);
+    const int nv = sv.count();
+    const u32x4 *vs = reinterpret_cast<const u32x4 *>(av + sv.a);
+    u32x4 *vd = reinterpret_cast<u32x4 *>(lval + (sv.a - sv.a0));
+    for (int j = 0; j * (BS * UV) < nv; j++) {
+        const int qv = j * (BS * UV) + t;
+        u32x4 v[UV];
+#pragma unroll
+        for (int u = 0; u < UV; u++)
+            if (qv + u * BS < nv) v[u] = ldm<NT>(vs + qv + u * BS);
+#pragma unroll
+        for (int u = 0; u < UV; u++)
+            if (qv + u * BS < nv) vd[qv + u * BS] = v[u];
+    }
+    if (sv.head()) lval[e0 - sv.a0 + t] = vh;
+    if (sv.tail()) lval[sv.b - sv.a0 + t] = vt;
+}
+
+/* The masks of a pattern block (BS = 256 rows) without a barrier: lane l of
+   every wave holds word l of the block's masks (rows 4l .. 4l + 3, read
+   before the value stream is issued, so that its latency overlaps it) and
+   the waves scan the words' bit counts across their lanes; thread t takes
+   its row's mask and the number of entries of the block's rows before it.
+   The mask array is padded with zero masks to a whole number of blocks. */
+__device__ __forceinline__ void pattern_block_rows(unsigned w, unsigned &m, int &pre)
+{
+    const int t = threadIdx.x, lane = t & 63;
+    int s = __popc(w);
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(s, o, 64);
+        if (lane >= o) s += u;
+    }
+    const int g = t >> 2;
+    const unsigned wg = __shfl(w, g, 64);
+    const int sg = __shfl(s, g, 64);
+    const int sh = 8 * (t & 3);
+    m = (wg >> sh) & 0xFFu;
+    pre = sg - __popc(wg) + __popc(wg & ((1u << sh) - 1u));
+}
+
+/* Phase 2 of the pattern kernels: row r's entries are the set bits k of its
+   mask m in ascending order, column r + po.d[k], value lv[j] for the j-th
+   set bit; all gathers go out at once and p = a_ij x_j (rounded) is added in
+   that order - the CSR order, so the sum is staged_row_sum's bit for bit
+   (FZ as there). */
+template <typename MT, typename VT, bool FZ = false>
+__device__ __forceinline__ VT pattern_row_sum(unsigned m, const PatOffsets &po,
+                                              const MT *__restrict__ lv,
+                                              const VT *__restrict__ x, int r,
+                                              const VT *__restrict__ zf = nullptr,
+                                              const VT *__restrict__ zd = nullptr)
+{
+    VT g[PAT_MAX], v[PAT_MAX];
+#pragma unroll
+    for (int k = 0; k < PAT_MAX; k++) {
+        if ((m >> k) & 1u) {
+            v[k] = mval<VT>(lv[__popc(m & ((1u << k) - 1u))]);
+            const int c = r + po.d[k];
+            if constexpr (FZ)
+                g[k] = zf[c];
+            else
+                g[k] = x[c];
+        }
+    }
+    if constexpr (FZ) {
+        VT gd[PAT_MAX];
+#pragma unroll
+        for (int k = 0; k < PAT_MAX; k++)
+            if ((m >> k) & 1u) gd[k] = zd[r + po.d[k]];
+#pragma unroll
+        for (int k = 0; k < PAT_MAX; k++)
+            if ((m >> k) & 1u) g[k] = g[k] / gd[k];
+    }
+    VT t = 0;
+#pragma unroll
+    for (int k = 0; k < PAT_MAX; k++)
+        if ((m >> k) & 1u) t = t + v[k] * g[k];
+    return t;
+}
+
+/* LDS of the pattern kernels: the tile's values (room for the leading
+   partial chunk) */
+inline size_t pattern_lds_bytes(int tile, size_t mt_size)
+{
+    return ((size_t)tile + 16 / mt_size) * mt_size;
+}
+
 /* LDS of the staged row kernels: the tile's columns (room for the leading
    partial chunk, rounded to 16 bytes), then its values */
 __host__ __device__ inline int staged_col_bytes(int tile, int it_size)
@@ -426,7 +534,7 @@ inline bool stream_ok(const DevCsr &A)
    LDS and a barrier separates the sweeps. Row sums are sequential in column
    order (the order of k_rows_staged), so the result equals `sweeps` separate
    launches bit for bit. */
-template <typename MT, typename VT, int RPT, int MAXNZ>
+template <typename MT, typename VT, int RPT, int MAXNZ, bool PRO = false, bool RR = false>
 __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restrict__ rp,
                                                       const int *__restrict__ ci,
                                                       const MT *__restrict__ av,
@@ -435,8 +543,13 @@ __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restri
                                                       const VT *__restrict__ xin,
                                                       VT *__restrict__ xout, int sweeps,
                                                       int zero, long cs, int rs,
-                                                      const int *__restrict__ act)
+                                                      const int *__restrict__ act,
+                                                      BlockFuse<VT> fu)
 {
+    /* PRO: x starts as x_in + P x_c (k_prolong's fma; x_in = 0 when zero)
+       and every sweep runs; RR: after the sweeps the residual rows go to the
+       free LDS iterate and the restriction rows of R sum them (the residual
+       and SpMV kernels' orders) into fu.fc. Single column only. */
     /* block j works on column j of (f, xin, xout): element i of column j at
        j * cs + i * rs; columns whose act flag is 0 are skipped (act ==
        nullptr: one column) */
@@ -472,10 +585,14 @@ __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restri
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
         const int i = threadIdx.x + 1024 * q;
-        if (i < n) cur[i] = zero ? fi[q] / di[q] : xin[(size_t)i * rs];
+        if (i >= n) continue;
+        if constexpr (PRO)
+            cur[i] = fma(fu.p[i], fu.xc[fu.agg[i]], zero ? VT(0) : xin[(size_t)i * rs]);
+        else
+            cur[i] = zero ? fi[q] / di[q] : xin[(size_t)i * rs];
     }
     __syncthreads();
-    for (int s = zero ? 1 : 0; s < sweeps; s++) {
+    for (int s = (zero && !PRO) ? 1 : 0; s < sweeps; s++) {
 #pragma unroll
         for (int q = 0; q < RPT; q++) {
             const int i = threadIdx.x + 1024 * q;
@@ -497,6 +614,74 @@ __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restri
         const int i = threadIdx.x + 1024 * q;
         if (i < n) xout[(size_t)i * rs] = cur[i];
     }
+    if constexpr (RR) {
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int i = threadIdx.x + 1024 * q;
+            if (i < n) {
+                VT t = 0;
+#pragma unroll
+                for (int k = 0; k < MAXNZ; k++)
+                    if (k < len[q]) t = t + a[q][k] * cur[cj[q][k]];
+                nxt[i] = fi[q] - t;
+            }
+        }
+        __syncthreads();
+        for (int I = threadIdx.x; I < fu.nc; I += 1024) {
+            VT t = 0;
+            const int e1 = fu.rrp[I + 1];
+            for (int e = fu.rrp[I]; e < e1; e++) t = t + fu.rv[e] * nxt[fu.rci[e]];
+            fu.fc[I] = t;
+        }
+    }
+}
+
+/* k_rows_staged's residual of each fine row of a coarse row's R entries,
+   restricted as the SpMV with R restricts it: y_I = sum over R's row I of
+   R_Ii (f_i - (A x)_i), every sum from 0 in storage order with products
+   rounded. One thread per coarse row walks its fine rows (8 entries' loads
+   at once); for small cache-resident levels, where a launch costs more than
+   the walk. */
+template <typename MT, typename VT>
+__global__ __launch_bounds__(BS) void k_resid_restrict(int nc, const int *__restrict__ rrp,
+                                                       const int *__restrict__ rci,
+                                                       const VT *__restrict__ rv,
+                                                       const int *__restrict__ arp,
+                                                       const int *__restrict__ aci,
+                                                       const MT *__restrict__ av,
+                                                       const VT *__restrict__ x,
+                                                       const VT *__restrict__ f,
+                                                       VT *__restrict__ y)
+{
+    const int I = blockIdx.x * BS + threadIdx.x;
+    if (I >= nc) return;
+    VT acc = 0;
+    const int e1 = rrp[I + 1];
+    for (int e = rrp[I]; e < e1; e++) {
+        const int i = rci[e];
+        const VT w = rv[e];
+        const int j1 = arp[i + 1];
+        VT t = 0;
+        constexpr int PF = 8;
+        for (int j0 = arp[i]; j0 < j1; j0 += PF) {
+            int c[PF];
+            VT v[PF], g[PF];
+#pragma unroll
+            for (int u = 0; u < PF; u++)
+                if (j0 + u < j1) {
+                    c[u] = aci[j0 + u];
+                    v[u] = mval<VT>(av[j0 + u]);
+                }
+#pragma unroll
+            for (int u = 0; u < PF; u++)
+                if (j0 + u < j1) g[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < PF; u++)
+                if (j0 + u < j1) t = t + v[u] * g[u];
+        }
+        acc = acc + w * (f[i] - t);
+    }
+    y[I] = acc;
 }
 
 } // namespace

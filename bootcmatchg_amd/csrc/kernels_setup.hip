@@ -284,7 +284,12 @@ __device__ __forceinline__ int key_u(unsigned long long sv) { return (int)(unsig
    CAS; a displaced suitor is searched for by the same thread
    (suitor.cu:49-114). Every successful CAS strictly raises a vertex's offer,
    so the loop ends; the final state is the proposer-optimal stable outcome,
-   which does not depend on the interleaving. */
+   which does not depend on the interleaving. A step costs dependent memory
+   round trips, and the displacement chains of tied weights are followed
+   step by step, so each step saves what it can: the offer read during the
+   scan is the CAS's expected value (a stale one only fails the CAS, which
+   returns the current offer), and the row bounds of the proposer that the
+   CAS would displace are loaded while the CAS is in flight. */
 __global__ __launch_bounds__(BS) void k_suitor(int n, const int *__restrict__ rp,
                                                const int *__restrict__ ci,
                                                const double *__restrict__ W,
@@ -296,10 +301,11 @@ __global__ __launch_bounds__(BS) void k_suitor(int n, const int *__restrict__ rp
     /* blocks start in index order, so thread t takes vertex n-1-t: the high
        ids (which win every tie) propose first and fewer offers get undone */
     int cur = n - 1 - t;
+    int j0 = rp[cur], j1 = rp[cur + 1];
     while (cur >= 0) {
         int best = -1, bj = -1;
         double bw = -1.0;
-        const int j0 = rp[cur], j1 = rp[cur + 1];
+        unsigned long long bsv = ~0ull;
         for (int jb = j0; jb < j1; jb += SB) {
             /* issue all loads of the batch before any use: one round trip for
                W and the columns, one for the neighbours' offers, one for the
@@ -330,18 +336,22 @@ __global__ __launch_bounds__(BS) void k_suitor(int n, const int *__restrict__ rp
                 bw = w;
                 best = y[k];
                 bj = jb + k;
+                bsv = sv[k];
             }
         }
         if (best < 0) break;
         const unsigned long long mine =
             ((unsigned long long)(unsigned)bj << 32) | (unsigned long long)(unsigned)cur;
-        unsigned long long old =
-            __hip_atomic_load(&S[best], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int next = -2;
+        unsigned long long old = bsv;
+        int next = -2, nj0 = 0, nj1 = 0;
         while (true) {
             const double wo = key_w(old, W);
-            const int uo = key_u(old);
+            const int uo = key_u(old); /* -1: no offer yet */
             if (bw > wo || (bw == wo && cur > uo)) {
+                if (uo >= 0) {
+                    nj0 = rp[uo];
+                    nj1 = rp[uo + 1];
+                }
                 const unsigned long long prev = atomicCAS(&S[best], old, mine);
                 if (prev == old) {
                     next = uo;
@@ -354,6 +364,8 @@ __global__ __launch_bounds__(BS) void k_suitor(int n, const int *__restrict__ rp
         }
         if (next == -2) continue; /* outbid meanwhile: search again */
         cur = next;               /* displaced suitor, or -1 */
+        j0 = nj0;
+        j1 = nj1;
     }
 }
 

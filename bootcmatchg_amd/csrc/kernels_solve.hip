@@ -14,6 +14,8 @@
  */
 #include "kernels_common.h"
 
+#include <algorithm>
+
 namespace bcmg {
 
 namespace {
@@ -78,6 +80,212 @@ __global__ __launch_bounds__(BS) void k_rows_staged(int n, const int *__restrict
         y[r] = fr - t;
     else
         y[r] = (fr - t) / dr + xr;
+}
+
+/* k_rows_staged on a stencil-pattern operator: the block reads its rows'
+   one-byte masks, two row pointers (its first value and its end) and its
+   values (into LDS); each row's value positions come from a block scan of
+   the masks' bit counts and its columns from the offset table. Same
+   products in the same order as k_rows_staged, so the same bits. */
+template <typename MT, typename VT, int OP, bool NT>
+__global__ __launch_bounds__(BS) void k_rows_pattern(int n, const int *__restrict__ rp,
+                                                     const unsigned char *__restrict__ pm,
+                                                     PatOffsets po, const MT *__restrict__ av,
+                                                     const VT *__restrict__ x,
+                                                     const VT *__restrict__ f,
+                                                     const VT *__restrict__ d,
+                                                     VT *__restrict__ y)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_pat[];
+    MT *lval = reinterpret_cast<MT *>(smem_pat);
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = b * BS;
+    const int r1 = min(r0 + BS, n);
+    const int r = r0 + threadIdx.x;
+    const bool own = r < r1;
+    const unsigned w = ldm<NT>(reinterpret_cast<const unsigned *>(pm + r0) + (threadIdx.x & 63));
+    VT fr = 0, dr = 1, xr = 0;
+    if (own) {
+        if (OP != OP_SPMV) fr = f[r];
+        if (OP == OP_JACOBI) {
+            dr = d[r];
+            xr = x[r];
+        } else if (OP == OP_JACOBI_Z) {
+            dr = d[r];
+            xr = fr / dr;
+        }
+    }
+    const int e0 = rp[r0];
+    const int cnt = rp[r1] - e0;
+    stage_values<MT, NT>(av, e0, cnt, lval);
+    unsigned m;
+    int pre;
+    pattern_block_rows(w, m, pre);
+    __syncthreads();
+    if (!own) return;
+    const VT t = pattern_row_sum<MT, VT, OP == OP_JACOBI_Z>(
+        m, po, lval + (e0 - stage_base<MT>(e0)) + pre, x, r, f, d);
+    if (OP == OP_SPMV)
+        y[r] = t;
+    else if (OP == OP_RESID)
+        y[r] = fr - t;
+    else
+        y[r] = (fr - t) / dr + xr;
+}
+
+/* The first Jacobi/L1 sweep after the prolongation, from x~ = x + P x_c
+   formed where it is read (k_prolong's fma at every gather and for the row
+   itself) and summed in column order from 0, so y equals a prolongation
+   launch followed by a sweep launch bit for bit; x~ is not stored (the
+   sweep's result replaces it). One thread per row: for small levels, where
+   the launch saved costs more than the dependent loads. */
+template <typename MT, typename VT>
+__global__ __launch_bounds__(BS) void k_jacobi_prolong(int n, const int *__restrict__ rp,
+                                                       const int *__restrict__ ci,
+                                                       const MT *__restrict__ av,
+                                                       const int *__restrict__ agg,
+                                                       const VT *__restrict__ p,
+                                                       const VT *__restrict__ xc,
+                                                       const VT *__restrict__ x,
+                                                       const VT *__restrict__ f,
+                                                       const VT *__restrict__ d,
+                                                       VT *__restrict__ y)
+{
+    const int r = blockIdx.x * BS + threadIdx.x;
+    if (r >= n) return;
+    const VT xr = fma(p[r], xc[agg[r]], x[r]);
+    const VT fr = f[r], dr = d[r];
+    const int j1 = rp[r + 1];
+    VT t = 0;
+    constexpr int PF = 8;
+    for (int j0 = rp[r]; j0 < j1; j0 += PF) {
+        int c[PF];
+        VT v[PF], g[PF];
+#pragma unroll
+        for (int u = 0; u < PF; u++)
+            if (j0 + u < j1) {
+                c[u] = ci[j0 + u];
+                v[u] = mval<VT>(av[j0 + u]);
+            }
+#pragma unroll
+        for (int u = 0; u < PF; u++)
+            if (j0 + u < j1) g[u] = fma(p[c[u]], xc[agg[c[u]]], x[c[u]]);
+#pragma unroll
+        for (int u = 0; u < PF; u++)
+            if (j0 + u < j1) t = t + v[u] * g[u];
+    }
+    y[r] = (fr - t) / dr + xr;
+}
+
+/* Setup of the pattern, step 1: the distinct column offsets of each 256-row
+   block (an LDS table of 2 PAT_MAX slots, PAT_EMPTY where unused) into
+   bset[16 b ...]; a block with more distinct offsets than slots raises
+   flag[0]. */
+constexpr int PAT_SLOTS = 2 * PAT_MAX;
+constexpr int PAT_EMPTY = (int)0x80000000; /* no offset col - row equals it */
+__global__ __launch_bounds__(BS) void k_pattern_offsets(int n, const int *__restrict__ rp,
+                                                        const int *__restrict__ ci,
+                                                        int *__restrict__ bset,
+                                                        int *__restrict__ flag)
+{
+    __shared__ int tab[PAT_SLOTS];
+    __shared__ int over;
+    if (threadIdx.x < PAT_SLOTS) tab[threadIdx.x] = PAT_EMPTY;
+    if (threadIdx.x == 0) over = 0;
+    __syncthreads();
+    const int r = blockIdx.x * BS + threadIdx.x;
+    if (r < n) {
+        const int k1 = rp[r + 1];
+        for (int k = rp[r]; k < k1; k++) {
+            const int o = ci[k] - r;
+            int s = 0;
+            for (; s < PAT_SLOTS; s++) {
+                const int v = __hip_atomic_load(&tab[s], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (v == o) break;
+                if (v == PAT_EMPTY) {
+                    const int prev = atomicCAS(&tab[s], PAT_EMPTY, o);
+                    if (prev == PAT_EMPTY || prev == o) break;
+                }
+            }
+            if (s == PAT_SLOTS) over = 1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < PAT_SLOTS) bset[(size_t)blockIdx.x * PAT_SLOTS + threadIdx.x] = tab[threadIdx.x];
+    if (threadIdx.x == 0 && over) atomicOr(flag, 1);
+}
+
+/* step 2 (one block): the union of the block tables into out[1 ..], its size
+   into out[0] (PAT_MAX + 1 when larger than PAT_MAX) */
+__global__ __launch_bounds__(BS) void k_pattern_union(int nblk, const int *__restrict__ bset,
+                                                      int *__restrict__ out)
+{
+    __shared__ int lists[BS * PAT_MAX];
+    __shared__ int counts[BS];
+    int *loc = lists + threadIdx.x * PAT_MAX;
+    int c = 0;
+    bool over = false;
+    for (int b = threadIdx.x; b < nblk; b += BS) {
+        for (int s = 0; s < PAT_SLOTS; s++) {
+            const int o = bset[(size_t)b * PAT_SLOTS + s];
+            if (o == PAT_EMPTY) continue;
+            bool seen = false;
+            for (int q = 0; q < c; q++) seen |= (loc[q] == o);
+            if (seen) continue;
+            if (c == PAT_MAX)
+                over = true;
+            else
+                loc[c++] = o;
+        }
+    }
+    counts[threadIdx.x] = over ? PAT_MAX + 1 : c;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    int c0 = counts[0];
+    bool ov = c0 > PAT_MAX;
+    for (int t = 1; t < BS && !ov; t++) {
+        const int ct = counts[t];
+        if (ct > PAT_MAX) {
+            ov = true;
+            break;
+        }
+        for (int q = 0; q < ct; q++) {
+            const int o = lists[t * PAT_MAX + q];
+            bool seen = false;
+            for (int p = 0; p < c0; p++) seen |= (lists[p] == o);
+            if (seen) continue;
+            if (c0 == PAT_MAX) {
+                ov = true;
+                break;
+            }
+            lists[c0++] = o;
+        }
+    }
+    out[0] = ov ? PAT_MAX + 1 : c0;
+    for (int q = 0; q < PAT_MAX; q++) out[1 + q] = q < c0 ? lists[q] : PAT_EMPTY;
+}
+
+/* step 3: the row masks over the sorted offset table (every offset of the
+   operator is in it); the rows that pad the last 256-row block get 0 */
+__global__ __launch_bounds__(BS) void k_pattern_masks(int n, const int *__restrict__ rp,
+                                                      const int *__restrict__ ci, PatOffsets po,
+                                                      int pk, unsigned char *__restrict__ pm)
+{
+    const int r = blockIdx.x * BS + threadIdx.x;
+    if (r >= n) {
+        pm[r] = 0;
+        return;
+    }
+    unsigned m = 0;
+    const int k1 = rp[r + 1];
+    for (int k = rp[r]; k < k1; k++) {
+        const int o = ci[k] - r;
+#pragma unroll
+        for (int q = 0; q < PAT_MAX; q++)
+            if (q < pk && po.d[q] == o) m |= 1u << q;
+    }
+    pm[r] = (unsigned char)m;
 }
 
 /* per 256-row block b: its nnz (the LDS tile of k_rows_staged) and its
@@ -204,6 +412,19 @@ __global__ __launch_bounds__(BS) void k_prolong(int n, const int *__restrict__ a
 {
     const int i = blockIdx.x * BS + threadIdx.x;
     if (i < n) x[i] = fma(p[i], xc[agg[i]], x[i]);
+}
+
+/* the prolongation into another iterate: y = x + P x_c (k_prolong's
+   arithmetic) */
+template <typename VT>
+__global__ __launch_bounds__(BS) void k_prolong_to(int n, const int *__restrict__ agg,
+                                                   const VT *__restrict__ p,
+                                                   const VT *__restrict__ xc,
+                                                   const VT *__restrict__ x,
+                                                   VT *__restrict__ y)
+{
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i < n) y[i] = fma(p[i], xc[agg[i]], x[i]);
 }
 
 template <typename VT>
@@ -357,6 +578,54 @@ __global__ __launch_bounds__(BS) void k_spmv_dots(int n, const int *__restrict__
     block_reduce_store<K>(sdot, part + (size_t)b * K);
 }
 
+/* k_spmv_dots on a stencil-pattern operator (the reads of k_rows_pattern,
+   the products and partials of k_spmv_dots) */
+template <int K, bool NT>
+__global__ __launch_bounds__(BS) void k_spmv_dots_pattern(int n, const int *__restrict__ rp,
+                                                          const unsigned char *__restrict__ pm,
+                                                          PatOffsets po,
+                                                          const double *__restrict__ av,
+                                                          const double *__restrict__ z,
+                                                          double *__restrict__ t,
+                                                          const double *__restrict__ rv,
+                                                          const double *__restrict__ to,
+                                                          double *__restrict__ part)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_pat[];
+    double *lval = reinterpret_cast<double *>(smem_pat);
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = b * BS;
+    const int r1 = min(r0 + BS, n);
+    const int r = r0 + threadIdx.x;
+    const bool own = r < r1;
+    const unsigned w = ldm<NT>(reinterpret_cast<const unsigned *>(pm + r0) + (threadIdx.x & 63));
+    double zr = 0.0, rr = 0.0, tor = 0.0;
+    if (own) {
+        zr = z[r];
+        rr = rv[r];
+        if (K > 2) tor = to[r];
+    }
+    const int e0 = rp[r0];
+    const int cnt = rp[r1] - e0;
+    stage_values<double, NT>(av, e0, cnt, lval);
+    unsigned m;
+    int pre;
+    pattern_block_rows(w, m, pre);
+    __syncthreads();
+    double sdot[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) sdot[k] = 0.0;
+    if (own) {
+        const double tr = pattern_row_sum<double, double>(
+            m, po, lval + (e0 - stage_base<double>(e0)) + pre, z, r);
+        t[r] = tr;
+        sdot[0] = rr * zr;
+        sdot[1] = tr * zr;
+        if (K > 2) sdot[K - 1] = tor * zr;
+    }
+    block_reduce_store<K>(sdot, part + (size_t)b * K);
+}
+
 /* fused double_merged_axpy pair + ||r||^2 partials (src/SOLVER/FCG.cu:88-98,
    536-558): z = -theta zo + z; x += theta2 z; t = -theta to + t; r += -theta2 t */
 __global__ __launch_bounds__(BS) void k_fcg_update(int n, const double *__restrict__ scal,
@@ -481,6 +750,13 @@ __global__ __launch_bounds__(BS) void k_true_residual(int n, const double *__res
     block_reduce_store<1>(s, part + blockIdx.x);
 }
 
+PatOffsets pattern_offsets(const DevCsr &A)
+{
+    PatOffsets po;
+    for (int k = 0; k < PAT_MAX; k++) po.d[k] = A.poff[k];
+    return po;
+}
+
 template <typename MT, typename VT, int OP>
 void rows_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x, const VT *f,
                 const VT *d, VT *y)
@@ -489,8 +765,22 @@ void rows_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x, const
     const int *rp = A.row.get(), *ci = A.col.get();
     if (stream_ok<VT>(A)) {
         const int nb = (n + BS - 1) / BS;
+        if (A.use_pattern()) {
+            if (!stage_aligned(av, av))
+                throw Error(BCMG_EINVAL, "row kernels: matrix values not 16-byte aligned");
+            const size_t lds = pattern_lds_bytes(A.tile, sizeof(MT));
+            if (lds > 65536) throw Error(BCMG_EUNSUP, "row tile too large for this value type");
+            const PatOffsets po = pattern_offsets(A);
+            if ((size_t)A.nnz * sizeof(MT) + (size_t)n > NT_BYTES)
+                k_rows_pattern<MT, VT, OP, true><<<nb, BS, lds, s>>>(n, rp, A.pmask.get(), po, av,
+                                                                     x, f, d, y);
+            else
+                k_rows_pattern<MT, VT, OP, false><<<nb, BS, lds, s>>>(n, rp, A.pmask.get(), po,
+                                                                      av, x, f, d, y);
+            return;
+        }
         /* the narrow columns where the int32-column stream is non-temporal */
-        const bool use16 = A.narrow && (size_t)A.nnz * (4 + sizeof(MT)) > NT_BYTES;
+        const bool use16 = A.use_narrow() && (size_t)A.nnz * (4 + sizeof(MT)) > NT_BYTES;
         if (!stage_aligned(use16 ? (const void *)A.col16.get() : (const void *)ci, av))
             throw Error(BCMG_EINVAL, "row kernels: matrix arrays not 16-byte aligned");
         if (use16) {
@@ -542,28 +832,59 @@ void rows_op(hipStream_t s, const DevCsr &A, const MT *av, RowOp op, const VT *x
         rows_typed<MT, VT, OP_JACOBI_Z>(s, A, av, x, f, d, y);
 }
 
-template <typename MT, typename VT>
-void relax_block_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *d, const VT *f,
-                       const VT *xin, VT *xout, int sweeps, int z, int ncols, long cs,
-                       int rs, const int *act)
+template <typename MT, typename VT, bool PRO, bool RR>
+void relax_block_shape(hipStream_t s, const DevCsr &A, const MT *av, const VT *d, const VT *f,
+                       const VT *xin, VT *xout, int sweeps, int z, int ncols, long cs, int rs,
+                       const int *act, const BlockFuse<VT> &fu)
 {
     const size_t lds = (size_t)2 * A.n * sizeof(VT);
     const int *rp = A.row.get(), *ci = A.col.get();
     const int n = A.n;
     if (n <= 1024 && A.maxrow <= 8)
-        k_relax_block<MT, VT, 1, 8><<<ncols, 1024, lds, s>>>(n, rp, ci, av, d, f, xin, xout,
-                                                              sweeps, z, cs, rs, act);
+        k_relax_block<MT, VT, 1, 8, PRO, RR><<<ncols, 1024, lds, s>>>(
+            n, rp, ci, av, d, f, xin, xout, sweeps, z, cs, rs, act, fu);
     else if (n <= 4096 && A.maxrow <= 8)
-        k_relax_block<MT, VT, 4, 8><<<ncols, 1024, lds, s>>>(n, rp, ci, av, d, f, xin, xout,
-                                                              sweeps, z, cs, rs, act);
+        k_relax_block<MT, VT, 4, 8, PRO, RR><<<ncols, 1024, lds, s>>>(
+            n, rp, ci, av, d, f, xin, xout, sweeps, z, cs, rs, act, fu);
     else if (n <= 2048 && A.maxrow <= 16)
-        k_relax_block<MT, VT, 2, 16><<<ncols, 1024, lds, s>>>(n, rp, ci, av, d, f, xin, xout,
-                                                               sweeps, z, cs, rs, act);
+        k_relax_block<MT, VT, 2, 16, PRO, RR><<<ncols, 1024, lds, s>>>(
+            n, rp, ci, av, d, f, xin, xout, sweeps, z, cs, rs, act, fu);
     else if (n <= 1024 && A.maxrow <= 32)
-        k_relax_block<MT, VT, 1, 32><<<ncols, 1024, lds, s>>>(n, rp, ci, av, d, f, xin, xout,
-                                                               sweeps, z, cs, rs, act);
+        k_relax_block<MT, VT, 1, 32, PRO, RR><<<ncols, 1024, lds, s>>>(
+            n, rp, ci, av, d, f, xin, xout, sweeps, z, cs, rs, act, fu);
     else
         throw Error(BCMG_EINVAL, "relax_block: level too large");
+}
+
+template <typename MT, typename VT>
+void relax_block_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *d, const VT *f,
+                       const VT *xin, VT *xout, int sweeps, int z, int ncols, long cs,
+                       int rs, const int *act, const BlockFuse<VT> &fu)
+{
+    const bool pro = fu.agg != nullptr, rr = fu.fc != nullptr;
+    if ((pro || rr) && ncols != 1)
+        throw Error(BCMG_EINVAL, "relax_block: fused work is single-column");
+    if (pro && rr)
+        throw Error(BCMG_EINVAL, "relax_block: prolongation and restriction in one launch");
+    if (pro)
+        relax_block_shape<MT, VT, true, false>(s, A, av, d, f, xin, xout, sweeps, z, ncols, cs,
+                                               rs, act, fu);
+    else if (rr)
+        relax_block_shape<MT, VT, false, true>(s, A, av, d, f, xin, xout, sweeps, z, ncols, cs,
+                                               rs, act, fu);
+    else
+        relax_block_shape<MT, VT, false, false>(s, A, av, d, f, xin, xout, sweeps, z, ncols,
+                                                cs, rs, act, fu);
+}
+
+template <typename MT, typename VT>
+void resid_restrict_typed(hipStream_t s, const DevCsr &A, const MT *av, const DevCsr &R,
+                          const VT *rv, const VT *x, const VT *f, VT *y)
+{
+    const int nc = R.n;
+    k_resid_restrict<MT, VT><<<blocks_for(nc), BS, 0, s>>>(nc, R.row.get(), R.col.get(), rv,
+                                                           A.row.get(), A.col.get(), av, x, f,
+                                                           y);
 }
 
 } // namespace
@@ -621,25 +942,54 @@ void make_narrow(Ctx &c, const std::vector<DevCsr *> &mats)
 {
     std::vector<DevCsr *> cand;
     for (DevCsr *A : mats)
-        if (A->n == A->m && (size_t)A->nnz * 12 > NT_BYTES && !A->narrow) cand.push_back(A);
+        if (A->n == A->m && (size_t)A->nnz * 12 > NT_BYTES && !A->cols_analysed)
+            cand.push_back(A);
     if (cand.empty()) return;
     const int nc = (int)cand.size();
-    DBuf<int> wide(nc);
-    k_fill<int><<<(nc + BS - 1) / BS, BS, 0, c.stream>>>(nc, 0, wide.get());
+    /* per candidate: the 16-bit overflow flag, the block-table overflow flag
+       and the offset union (count + PAT_MAX offsets) */
+    constexpr int W = 3 + PAT_MAX;
+    DBuf<int> info((size_t)nc * W);
+    k_fill<int><<<(nc * W + BS - 1) / BS, BS, 0, c.stream>>>(nc * W, 0, info.get());
+    std::vector<DBuf<int>> bset(nc);
     for (int k = 0; k < nc; k++) {
         DevCsr &A = *cand[k];
+        int *in = info.get() + (size_t)k * W;
+        const int nblk = (A.n + BS - 1) / BS;
         A.col16.alloc(A.nnz);
-        k_narrow_cols<<<(A.n + BS - 1) / BS, BS, 0, c.stream>>>(A.n, A.row.get(), A.col.get(),
-                                                                A.col16.get(), wide.get() + k);
+        k_narrow_cols<<<nblk, BS, 0, c.stream>>>(A.n, A.row.get(), A.col.get(), A.col16.get(),
+                                                 in);
+        bset[k].alloc((size_t)nblk * PAT_SLOTS);
+        k_pattern_offsets<<<nblk, BS, 0, c.stream>>>(A.n, A.row.get(), A.col.get(),
+                                                     bset[k].get(), in + 1);
+        k_pattern_union<<<1, BS, 0, c.stream>>>(nblk, bset[k].get(), in + 2);
     }
     BCMG_CHECK_LAUNCH();
-    std::vector<int> h(nc);
-    BCMG_HIP(hipMemcpyAsync(h.data(), wide.get(), nc * sizeof(int), hipMemcpyDeviceToHost,
+    std::vector<int> h((size_t)nc * W);
+    BCMG_HIP(hipMemcpyAsync(h.data(), info.get(), h.size() * sizeof(int), hipMemcpyDeviceToHost,
                             c.stream));
     BCMG_HIP(hipStreamSynchronize(c.stream));
     for (int k = 0; k < nc; k++) {
-        cand[k]->narrow = (h[k] == 0);
-        if (!cand[k]->narrow) cand[k]->col16.release();
+        DevCsr &A = *cand[k];
+        const int *hk = h.data() + (size_t)k * W;
+        A.cols_analysed = true;
+        A.narrow = (hk[0] == 0);
+        if (!A.narrow) A.col16.release();
+        const int cnt = hk[2];
+        if (hk[1] != 0 || cnt < 1 || cnt > PAT_MAX) continue;
+        std::vector<int> off(hk + 3, hk + 3 + cnt);
+        std::sort(off.begin(), off.end());
+        PatOffsets po;
+        for (int q = 0; q < PAT_MAX; q++) po.d[q] = q < cnt ? off[q] : PAT_EMPTY;
+        /* whole 256-row blocks: the kernels read every block's masks as 64
+           words */
+        const int nblk = (A.n + BS - 1) / BS;
+        A.pmask.alloc((size_t)nblk * BS);
+        k_pattern_masks<<<nblk, BS, 0, c.stream>>>(A.n, A.row.get(), A.col.get(), po, cnt,
+                                                    A.pmask.get());
+        BCMG_CHECK_LAUNCH();
+        A.pk = cnt;
+        for (int q = 0; q < PAT_MAX; q++) A.poff[q] = po.d[q];
     }
 }
 
@@ -704,25 +1054,88 @@ bool relax_block_ok(const DevCsr &A)
 template <typename VT>
 void launch_relax_block(hipStream_t s, const DevCsr &A, MatType mt, const VT *d, const VT *f,
                         const VT *xin, VT *xout, int sweeps, bool zero, int ncols, long cs,
-                        int rs, const int *act)
+                        int rs, const int *act, const BlockFuse<VT> &fu)
 {
     const int z = zero ? 1 : 0;
     if (ncols <= 0) return;
     if constexpr (std::is_same<VT, double>::value) {
+        if (mt != MT_F64) throw Error(BCMG_EINVAL, "FP64 vectors need FP64 matrix values");
         relax_block_typed<double, double>(s, A, A.val.get(), d, f, xin, xout, sweeps, z, ncols,
-                                          cs, rs, act);
+                                          cs, rs, act, fu);
     } else {
         if (mt == MT_F32)
             relax_block_typed<float, float>(s, A, A.val32.get(), d, f, xin, xout, sweeps, z,
-                                            ncols, cs, rs, act);
+                                            ncols, cs, rs, act, fu);
         else if (mt == MT_BF16)
             relax_block_typed<unsigned short, float>(s, A, A.val16.get(), d, f, xin, xout,
-                                                     sweeps, z, ncols, cs, rs, act);
+                                                     sweeps, z, ncols, cs, rs, act, fu);
         else
             relax_block_typed<double, float>(s, A, A.val.get(), d, f, xin, xout, sweeps, z,
-                                             ncols, cs, rs, act);
+                                             ncols, cs, rs, act, fu);
     }
     BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+bool rows_sequential(const DevCsr &A)
+{
+    return stream_ok<VT>(A);
+}
+
+template <typename VT>
+bool launch_jacobi_prolong(hipStream_t s, const DevCsr &A, MatType mt, const int *agg,
+                           const VT *p, const VT *xc, const VT *x, const VT *f, const VT *d,
+                           VT *y)
+{
+    if (!stream_ok<VT>(A)) return false;
+    if (x == y) throw Error(BCMG_EINVAL, "prolongation sweep into its own input");
+    const int n = A.n;
+    const int *rp = A.row.get(), *ci = A.col.get();
+    if (n <= 0) return true;
+    if constexpr (std::is_same<VT, double>::value) {
+        if (mt != MT_F64) throw Error(BCMG_EINVAL, "FP64 vectors need FP64 matrix values");
+        k_jacobi_prolong<double, double><<<blocks_for(n), BS, 0, s>>>(n, rp, ci, A.val.get(), agg,
+                                                                      p, xc, x, f, d, y);
+    } else {
+        if (mt == MT_F32)
+            k_jacobi_prolong<float, float><<<blocks_for(n), BS, 0, s>>>(
+                n, rp, ci, A.val32.get(), agg, p, xc, x, f, d, y);
+        else if (mt == MT_BF16)
+            k_jacobi_prolong<unsigned short, float><<<blocks_for(n), BS, 0, s>>>(
+                n, rp, ci, A.val16.get(), agg, p, xc, x, f, d, y);
+        else
+            k_jacobi_prolong<double, float><<<blocks_for(n), BS, 0, s>>>(
+                n, rp, ci, A.val.get(), agg, p, xc, x, f, d, y);
+    }
+    BCMG_CHECK_LAUNCH();
+    return true;
+}
+
+template <typename VT>
+bool launch_resid_restrict(hipStream_t s, const DevCsr &A, MatType mt, const DevCsr &R,
+                           const VT *x, const VT *f, VT *y)
+{
+    /* the staged kernels' sequential sums are what this kernel repeats */
+    if (!stream_ok<VT>(A) || !stream_ok<VT>(R) || R.n <= 0) return false;
+    const VT *rv;
+    if constexpr (std::is_same<VT, double>::value)
+        rv = R.val.get();
+    else
+        rv = R.val32.get();
+    if (!rv) return false;
+    if constexpr (std::is_same<VT, double>::value) {
+        if (mt != MT_F64) throw Error(BCMG_EINVAL, "FP64 vectors need FP64 matrix values");
+        resid_restrict_typed<double, double>(s, A, A.val.get(), R, rv, x, f, y);
+    } else {
+        if (mt == MT_F32)
+            resid_restrict_typed<float, float>(s, A, A.val32.get(), R, rv, x, f, y);
+        else if (mt == MT_BF16)
+            resid_restrict_typed<unsigned short, float>(s, A, A.val16.get(), R, rv, x, f, y);
+        else
+            resid_restrict_typed<double, float>(s, A, A.val.get(), R, rv, x, f, y);
+    }
+    BCMG_CHECK_LAUNCH();
+    return true;
 }
 
 template <typename VT>
@@ -738,6 +1151,16 @@ void launch_prolong(hipStream_t s, int n, const int *agg, const VT *p, const VT 
 {
     if (n <= 0) return;
     k_prolong<VT><<<blocks_for(n), BS, 0, s>>>(n, agg, p, xc, x);
+    BCMG_CHECK_LAUNCH();
+}
+
+template <typename VT>
+void launch_prolong_to(hipStream_t s, int n, const int *agg, const VT *p, const VT *xc,
+                       const VT *x, VT *y)
+{
+    if (n <= 0) return;
+    if (x == y) throw Error(BCMG_EINVAL, "prolongation into its own input");
+    k_prolong_to<VT><<<blocks_for(n), BS, 0, s>>>(n, agg, p, xc, x, y);
     BCMG_CHECK_LAUNCH();
 }
 
@@ -793,10 +1216,18 @@ void launch_f2d(hipStream_t s, size_t n, const float *a, double *b)
     template bool relax_block_ok<VT>(const DevCsr &);                                        \
     template void launch_relax_block<VT>(hipStream_t, const DevCsr &, MatType, const VT *,   \
                                          const VT *, const VT *, VT *, int, bool, int, long, \
-                                         int, const int *);                                  \
+                                         int, const int *, const BlockFuse<VT> &);           \
+    template bool rows_sequential<VT>(const DevCsr &);                                       \
+    template bool launch_jacobi_prolong<VT>(hipStream_t, const DevCsr &, MatType, const int *,\
+                                            const VT *, const VT *, const VT *, const VT *,  \
+                                            const VT *, VT *);                               \
+    template bool launch_resid_restrict<VT>(hipStream_t, const DevCsr &, MatType,            \
+                                            const DevCsr &, const VT *, const VT *, VT *);   \
     template void launch_jacobi_zero<VT>(hipStream_t, int, const VT *, const VT *, VT *);    \
     template void launch_prolong<VT>(hipStream_t, int, const int *, const VT *, const VT *,  \
                                      VT *);                                                  \
+    template void launch_prolong_to<VT>(hipStream_t, int, const int *, const VT *,           \
+                                        const VT *, const VT *, VT *);                       \
     template void launch_fill<VT>(hipStream_t, int, VT, VT *);                               \
     template void launch_copy<VT>(hipStream_t, int, const VT *, VT *);                       \
     template void launch_axpy<VT>(hipStream_t, int, VT, const VT *, VT *);                   \
@@ -964,7 +1395,29 @@ void launch_spmv_dots(Ctx &c, const DevCsr &A, int k, const double *z, double *t
     double *part = c.red.get() + RED_SPMV;
     const int *rp = A.row.get();
     const double *av = A.val.get();
-    if (A.narrow)
+    if (A.use_pattern()) {
+        if (!stage_aligned(av, av))
+            throw Error(BCMG_EINVAL, "SpMV: matrix values not 16-byte aligned");
+        const size_t lds = pattern_lds_bytes(A.tile, sizeof(double));
+        const PatOffsets po = pattern_offsets(A);
+        const bool nt = (size_t)A.nnz * sizeof(double) + (size_t)n > NT_BYTES;
+        const unsigned char *pm = A.pmask.get();
+        if (k == 2) {
+            if (nt)
+                k_spmv_dots_pattern<2, true><<<nb, BS, lds, c.stream>>>(n, rp, pm, po, av, z, t,
+                                                                        r, to, part);
+            else
+                k_spmv_dots_pattern<2, false><<<nb, BS, lds, c.stream>>>(n, rp, pm, po, av, z, t,
+                                                                         r, to, part);
+        } else {
+            if (nt)
+                k_spmv_dots_pattern<3, true><<<nb, BS, lds, c.stream>>>(n, rp, pm, po, av, z, t,
+                                                                        r, to, part);
+            else
+                k_spmv_dots_pattern<3, false><<<nb, BS, lds, c.stream>>>(n, rp, pm, po, av, z, t,
+                                                                         r, to, part);
+        }
+    } else if (A.use_narrow())
         spmv_dots_typed<short>(c.stream, nb, n, k, rp, A.col16.get(), av, A.nnz, A.tile, z, t, r,
                                to, part);
     else

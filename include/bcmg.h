@@ -91,13 +91,25 @@ int bcmg_csr_upload(bcmg_ctx ctx, int n, int nnz, const int *row, const int *col
                     const double *val, bcmg_csr *out);
 int bcmg_csr_download(bcmg_csr A, int *row, int *col, double *val);
 int bcmg_csr_info(bcmg_csr A, int *n, int *nnz);
-/* Bytes per column index that the FP64 row kernels read for A: 2 when the
-   library keeps a 16-bit copy (column minus row) of a square operator of
-   more than 16.7 M entries (an FP64 stream above 192 MB) whose offsets all
-   fit, else 4; FP32/BF16 value copies use it when their 32-bit-column stream
-   also exceeds 192 MB. No counterpart in the reference (which always streams
-   32-bit columns); it lets a caller price a kernel's memory traffic. */
+/* Bytes per column index that the FP64 row kernels read for A. A square
+   operator of more than 16.7 M entries (an FP64 stream above 192 MB) gets
+   narrower copies of its columns at upload: 16-bit offsets (column minus
+   row) when they all fit, and a stencil pattern when those offsets take at
+   most 8 distinct values (one byte per row: which offsets of a sorted table
+   the row has). The row kernels read the narrowest one that is kept: 0 the
+   pattern (no per-entry column bytes, one byte per row, and only one row
+   pointer per 256 rows), 2 the 16-bit offsets (FP32/BF16 value copies use
+   them when their 32-bit-column stream also exceeds 192 MB), else 4. Results
+   are the same bits in every form. No counterpart in the reference (which
+   always streams 32-bit columns); it lets a caller price a kernel's memory
+   traffic. */
 int bcmg_csr_column_bytes(bcmg_csr A, int *bytes);
+/* Restricts the row kernels on A to a column form: bytes 0 (the pattern),
+   2 (the 16-bit offsets) or 4 (the int32 columns); BCMG_EUNSUP when A has no
+   such copy. Composites set up on A before the call keep the launches they
+   captured. No counterpart in the reference: for A/B measurements and tests
+   of the column forms. */
+int bcmg_csr_set_columns(bcmg_csr A, int bytes);
 /* Replaces CSRm::free (src/matrix/CSR.cu:160-181). */
 int bcmg_csr_free(bcmg_csr A);
 

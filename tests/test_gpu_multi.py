@@ -93,14 +93,14 @@ def test_multi_column_counts(ctx, k, prec):
 
 @pytest.mark.parametrize("prec", [bc.PREC_FP64, bc.PREC_FP32])
 def test_multi_equals_single_narrow_columns(ctx, prec):
-    """An operator whose FP64 stream exceeds 192 MB: the single-column FP64
-    solve streams its 16-bit column copy on level 0 (sweeps, residual, the
-    fused FCG SpMV), the multi-column kernels its 32-bit columns; the columns
-    still agree bit for bit (in FP32 both read 32-bit columns)."""
+    """An operator whose FP64 stream exceeds 192 MB: the single-column
+    solve streams its stencil pattern on level 0 (sweeps, residual, the fused
+    FCG SpMV), the multi-column kernels its 32-bit columns; the columns still
+    agree bit for bit."""
     A = ops.poisson3d(140)
     n = len(A[0]) - 1
     dA = bc.Csr(ctx, A)
-    assert dA.column_bytes == 2
+    assert dA.column_bytes == 0
     B = bc.Boot(ctx, dA, bc.make_params(**SETTINGS["V_L1_2sweeps"]))
     B.set_precision(prec)
     check_multi_vs_single(ctx, B, rhs_set(n, 3))
