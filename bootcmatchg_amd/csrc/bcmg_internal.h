@@ -276,6 +276,7 @@ struct Boot {
     DevCsr *A0 = nullptr;       /* fine operator (owned by the caller's handle) */
     int prec = BCMG_PREC_FP64;  /* storage/compute precision of the cycles */
     int restarts = 0;           /* true-residual restarts of the last FCG */
+    int fcg_calls = 0;          /* FCG solves started on this composite */
     CycleBufs<double> b64;
     CycleBufs<float> b32;
     DBuf<float> r32, z32;       /* level-0 input/output of an FP32 cycle */

@@ -965,9 +965,12 @@ int fcg_solve(Ctx &c, Boot &B, const double *b, double *x, int *iters, double *r
     /* The iteration is the same launch sequence every time (two variants for
        the alternating direction slots), so it is captured once per composite
        and output vector as a HIP graph and replayed: the kernels then go out
-       without per-launch host work. Event timing of the smoother needs the
-       plain launches. */
-    const bool graphs = c.use_graphs && !c.time_smoother;
+       without per-launch host work. A composite's first solve launches
+       directly: capturing and instantiating the graphs costs more than a
+       solve's replays save on small operators (a setup-and-solve call pays
+       it once), while repeated solves on a kept composite replay them. Event
+       timing of the smoother needs the plain launches. */
+    const bool graphs = c.use_graphs && !c.time_smoother && B.fcg_calls++ > 0;
     if (graphs && (B.graph_x != x || B.graph_red != c.red.get())) B.drop_graphs();
     B.restarts = 0;
     int iter = 0;  /* iterations after the first step, restarts included */

@@ -178,14 +178,15 @@ __global__ __launch_bounds__(BS) void k_jacobi_prolong(int n, const int *__restr
 }
 
 /* Setup of the pattern, step 1: the distinct column offsets of each 256-row
-   block (an LDS table of 2 PAT_MAX slots, PAT_EMPTY where unused) into
-   bset[16 b ...]; a block with more distinct offsets than slots raises
-   flag[0]. */
+   block (an LDS table of 2 PAT_MAX slots) are inserted into the operator's
+   table gtab of as many slots (PAT_EMPTY where unused: a slot, once set,
+   keeps its value, so no value is inserted twice); more distinct offsets
+   than slots, in a block or in all, raise flag[0]. */
 constexpr int PAT_SLOTS = 2 * PAT_MAX;
 constexpr int PAT_EMPTY = (int)0x80000000; /* no offset col - row equals it */
 __global__ __launch_bounds__(BS) void k_pattern_offsets(int n, const int *__restrict__ rp,
                                                         const int *__restrict__ ci,
-                                                        int *__restrict__ bset,
+                                                        int *__restrict__ gtab,
                                                         int *__restrict__ flag)
 {
     __shared__ int tab[PAT_SLOTS];
@@ -212,58 +213,20 @@ __global__ __launch_bounds__(BS) void k_pattern_offsets(int n, const int *__rest
         }
     }
     __syncthreads();
-    if (threadIdx.x < PAT_SLOTS) bset[(size_t)blockIdx.x * PAT_SLOTS + threadIdx.x] = tab[threadIdx.x];
     if (threadIdx.x == 0 && over) atomicOr(flag, 1);
-}
-
-/* step 2 (one block): the union of the block tables into out[1 ..], its size
-   into out[0] (PAT_MAX + 1 when larger than PAT_MAX) */
-__global__ __launch_bounds__(BS) void k_pattern_union(int nblk, const int *__restrict__ bset,
-                                                      int *__restrict__ out)
-{
-    __shared__ int lists[BS * PAT_MAX];
-    __shared__ int counts[BS];
-    int *loc = lists + threadIdx.x * PAT_MAX;
-    int c = 0;
-    bool over = false;
-    for (int b = threadIdx.x; b < nblk; b += BS) {
-        for (int s = 0; s < PAT_SLOTS; s++) {
-            const int o = bset[(size_t)b * PAT_SLOTS + s];
-            if (o == PAT_EMPTY) continue;
-            bool seen = false;
-            for (int q = 0; q < c; q++) seen |= (loc[q] == o);
-            if (seen) continue;
-            if (c == PAT_MAX)
-                over = true;
-            else
-                loc[c++] = o;
+    if (threadIdx.x >= PAT_SLOTS) return;
+    const int o = tab[threadIdx.x];
+    if (o == PAT_EMPTY) return;
+    int s = 0;
+    for (; s < PAT_SLOTS; s++) {
+        const int v = __hip_atomic_load(&gtab[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v == o) break;
+        if (v == PAT_EMPTY) {
+            const int prev = atomicCAS(&gtab[s], PAT_EMPTY, o);
+            if (prev == PAT_EMPTY || prev == o) break;
         }
     }
-    counts[threadIdx.x] = over ? PAT_MAX + 1 : c;
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    int c0 = counts[0];
-    bool ov = c0 > PAT_MAX;
-    for (int t = 1; t < BS && !ov; t++) {
-        const int ct = counts[t];
-        if (ct > PAT_MAX) {
-            ov = true;
-            break;
-        }
-        for (int q = 0; q < ct; q++) {
-            const int o = lists[t * PAT_MAX + q];
-            bool seen = false;
-            for (int p = 0; p < c0; p++) seen |= (lists[p] == o);
-            if (seen) continue;
-            if (c0 == PAT_MAX) {
-                ov = true;
-                break;
-            }
-            lists[c0++] = o;
-        }
-    }
-    out[0] = ov ? PAT_MAX + 1 : c0;
-    for (int q = 0; q < PAT_MAX; q++) out[1 + q] = q < c0 ? lists[q] : PAT_EMPTY;
+    if (s == PAT_SLOTS) atomicOr(flag, 1);
 }
 
 /* step 3: the row masks over the sorted offset table (every offset of the
@@ -946,12 +909,14 @@ void make_narrow(Ctx &c, const std::vector<DevCsr *> &mats)
             cand.push_back(A);
     if (cand.empty()) return;
     const int nc = (int)cand.size();
-    /* per candidate: the 16-bit overflow flag, the block-table overflow flag
-       and the offset union (count + PAT_MAX offsets) */
-    constexpr int W = 3 + PAT_MAX;
+    /* per candidate: the 16-bit overflow flag, the pattern overflow flag and
+       the offset table */
+    constexpr int W = 2 + PAT_SLOTS;
     DBuf<int> info((size_t)nc * W);
-    k_fill<int><<<(nc * W + BS - 1) / BS, BS, 0, c.stream>>>(nc * W, 0, info.get());
-    std::vector<DBuf<int>> bset(nc);
+    std::vector<int> init((size_t)nc * W, PAT_EMPTY);
+    for (int k = 0; k < nc; k++) init[(size_t)k * W] = init[(size_t)k * W + 1] = 0;
+    BCMG_HIP(hipMemcpyAsync(info.get(), init.data(), init.size() * sizeof(int),
+                            hipMemcpyHostToDevice, c.stream));
     for (int k = 0; k < nc; k++) {
         DevCsr &A = *cand[k];
         int *in = info.get() + (size_t)k * W;
@@ -959,10 +924,8 @@ void make_narrow(Ctx &c, const std::vector<DevCsr *> &mats)
         A.col16.alloc(A.nnz);
         k_narrow_cols<<<nblk, BS, 0, c.stream>>>(A.n, A.row.get(), A.col.get(), A.col16.get(),
                                                  in);
-        bset[k].alloc((size_t)nblk * PAT_SLOTS);
-        k_pattern_offsets<<<nblk, BS, 0, c.stream>>>(A.n, A.row.get(), A.col.get(),
-                                                     bset[k].get(), in + 1);
-        k_pattern_union<<<1, BS, 0, c.stream>>>(nblk, bset[k].get(), in + 2);
+        k_pattern_offsets<<<nblk, BS, 0, c.stream>>>(A.n, A.row.get(), A.col.get(), in + 2,
+                                                     in + 1);
     }
     BCMG_CHECK_LAUNCH();
     std::vector<int> h((size_t)nc * W);
@@ -975,9 +938,11 @@ void make_narrow(Ctx &c, const std::vector<DevCsr *> &mats)
         A.cols_analysed = true;
         A.narrow = (hk[0] == 0);
         if (!A.narrow) A.col16.release();
-        const int cnt = hk[2];
+        std::vector<int> off;
+        for (int q = 0; q < PAT_SLOTS; q++)
+            if (hk[2 + q] != PAT_EMPTY) off.push_back(hk[2 + q]);
+        const int cnt = (int)off.size();
         if (hk[1] != 0 || cnt < 1 || cnt > PAT_MAX) continue;
-        std::vector<int> off(hk + 3, hk + 3 + cnt);
         std::sort(off.begin(), off.end());
         PatOffsets po;
         for (int q = 0; q < PAT_MAX; q++) po.d[q] = q < cnt ? off[q] : PAT_EMPTY;

@@ -97,12 +97,12 @@ int bcmg_csr_info(bcmg_csr A, int *n, int *nnz);
    row) when they all fit, and a stencil pattern when those offsets take at
    most 8 distinct values (one byte per row: which offsets of a sorted table
    the row has). The row kernels read the narrowest one that is kept: 0 the
-   pattern (no per-entry column bytes, one byte per row, and only one row
-   pointer per 256 rows), 2 the 16-bit offsets (FP32/BF16 value copies use
-   them when their 32-bit-column stream also exceeds 192 MB), else 4. Results
-   are the same bits in every form. No counterpart in the reference (which
-   always streams 32-bit columns); it lets a caller price a kernel's memory
-   traffic. */
+   pattern (no per-entry column bytes, one byte per row, and only two row
+   pointers per 256 rows), 2 the 16-bit offsets (FP32/BF16 value copies use
+   them when their 32-bit-column stream also exceeds 192 MB), else 4.
+   Results are the same bits in every form. No counterpart in the reference
+   (which always streams 32-bit columns); it lets a caller price a kernel's
+   memory traffic. */
 int bcmg_csr_column_bytes(bcmg_csr A, int *bytes);
 /* Restricts the row kernels on A to a column form: bytes 0 (the pattern),
    2 (the 16-bit offsets) or 4 (the int32 columns); BCMG_EUNSUP when A has no
