@@ -811,42 +811,18 @@ void inner_iterations(Ctx &c, Boot &B, DBuf<double> &w)
     DBuf<double> x(n), rhs(n), tmp(n);
     launch_fill(c.stream, n, 1.0, x.get());
     launch_fill(c.stream, n, 0.0, rhs.get());
-    const bool graphs = c.use_graphs && !c.time_smoother;
-    hipGraphExec_t ge = nullptr;
+    /* direct launches: on the 128^3 bootstrap workload an application took
+       0.93 ms either way, and capturing and instantiating its graph cost
+       about 13 ms per bootstrap round, more than 15 replays could save */
     double normold = 0.0, normnew = 0.0, ratio = 1.0;
-    try {
-        for (int i = 1; i <= IT; i++) {
-            if (graphs && i > 1) {
-                if (!ge) {
-                    hipGraph_t g = nullptr;
-                    BCMG_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-                    try {
-                        precond_apply(c, B, rhs.get(), x.get(), false);
-                    } catch (...) {
-                        (void)hipStreamEndCapture(c.stream, &g);
-                        if (g) (void)hipGraphDestroy(g);
-                        throw;
-                    }
-                    BCMG_HIP(hipStreamEndCapture(c.stream, &g));
-                    const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(g);
-                    BCMG_HIP(e);
-                }
-                BCMG_HIP(hipGraphLaunch(ge, c.stream));
-            } else {
-                precond_apply(c, B, rhs.get(), x.get(), false);
-            }
-            if (i >= IT - 1) {
-                normnew = a_norm(c, A, x.get(), tmp.get());
-                if (i == IT) ratio = normnew / normold;
-                normold = normnew;
-            }
+    for (int i = 1; i <= IT; i++) {
+        precond_apply(c, B, rhs.get(), x.get(), false);
+        if (i >= IT - 1) {
+            normnew = a_norm(c, A, x.get(), tmp.get());
+            if (i == IT) ratio = normnew / normold;
+            normold = normnew;
         }
-    } catch (...) {
-        if (ge) (void)hipGraphExecDestroy(ge);
-        throw;
     }
-    if (ge) BCMG_HIP(hipGraphExecDestroy(ge));
     launch_scale(c.stream, n, 1. / normnew, x.get());
     launch_copy(c.stream, n, x.get(), w.get());
     BCMG_HIP(hipStreamSynchronize(c.stream));

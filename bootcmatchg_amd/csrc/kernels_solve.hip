@@ -901,11 +901,16 @@ void analyze_rows(Ctx &c, DevCsr &A)
 }
 
 
+/* square operators whose FP64 stream (12 B per entry) exceeds this are
+   checked for a stencil pattern; the 16-bit copy is kept for streamed ones
+   (above NT_BYTES) only */
+constexpr size_t PAT_MIN_BYTES = (size_t)128 << 20;
+
 void make_narrow(Ctx &c, const std::vector<DevCsr *> &mats)
 {
     std::vector<DevCsr *> cand;
     for (DevCsr *A : mats)
-        if (A->n == A->m && (size_t)A->nnz * 12 > NT_BYTES && !A->cols_analysed)
+        if (A->n == A->m && (size_t)A->nnz * 12 > PAT_MIN_BYTES && !A->cols_analysed)
             cand.push_back(A);
     if (cand.empty()) return;
     const int nc = (int)cand.size();
@@ -921,9 +926,11 @@ void make_narrow(Ctx &c, const std::vector<DevCsr *> &mats)
         DevCsr &A = *cand[k];
         int *in = info.get() + (size_t)k * W;
         const int nblk = (A.n + BS - 1) / BS;
-        A.col16.alloc(A.nnz);
-        k_narrow_cols<<<nblk, BS, 0, c.stream>>>(A.n, A.row.get(), A.col.get(), A.col16.get(),
-                                                 in);
+        if ((size_t)A.nnz * 12 > NT_BYTES) {
+            A.col16.alloc(A.nnz);
+            k_narrow_cols<<<nblk, BS, 0, c.stream>>>(A.n, A.row.get(), A.col.get(),
+                                                     A.col16.get(), in);
+        }
         k_pattern_offsets<<<nblk, BS, 0, c.stream>>>(A.n, A.row.get(), A.col.get(), in + 2,
                                                      in + 1);
     }
@@ -936,7 +943,7 @@ void make_narrow(Ctx &c, const std::vector<DevCsr *> &mats)
         DevCsr &A = *cand[k];
         const int *hk = h.data() + (size_t)k * W;
         A.cols_analysed = true;
-        A.narrow = (hk[0] == 0);
+        A.narrow = A.col16.get() != nullptr && hk[0] == 0;
         if (!A.narrow) A.col16.release();
         std::vector<int> off;
         for (int q = 0; q < PAT_SLOTS; q++)

@@ -390,10 +390,10 @@ def test_graph_replay_matches_eager_launches(ctx):
 
 
 @pytest.mark.parametrize("setting", ["boot3_mult", "boot3_sym", "boot2_add_mixed"])
-def test_bootstrap_graph_replay_matches_eager_launches(ctx, setting):
-    """The bootstrap's composite applications replayed from a captured graph
-    give the same hierarchies bit for bit as host launches, for the three
-    composite kinds."""
+def test_bootstrap_same_hierarchies_with_graphs_on_and_off(ctx, setting):
+    """The bootstrap builds the same hierarchies bit for bit whether the
+    context captures graphs or not (its composite applications launch
+    directly either way), for the three composite kinds."""
     A = OPERATORS["lognormal3d_14"]()
     dA = bc.Csr(ctx, A)
     prm = bc.make_params(**SETTINGS[setting])
