@@ -639,10 +639,12 @@ __global__ __launch_bounds__(1024) void k_relax_block(int n, const int *__restri
 /* k_rows_staged's residual of each fine row of a coarse row's R entries,
    restricted as the SpMV with R restricts it: y_I = sum over R's row I of
    R_Ii (f_i - (A x)_i), every sum from 0 in storage order with products
-   rounded. One thread per coarse row walks its fine rows (8 entries' loads
-   at once); for small cache-resident levels, where a launch costs more than
+   rounded. G lanes per coarse row: lane g forms the residual of the fine
+   rows of R entries g, g + G, ... (8 entries' loads at once), and the
+   group's first lane adds the products in R's entry order, fetched through
+   shuffles. For small cache-resident levels, where a launch costs more than
    the walk. */
-template <typename MT, typename VT>
+template <typename MT, typename VT, int G>
 __global__ __launch_bounds__(BS) void k_resid_restrict(int nc, const int *__restrict__ rrp,
                                                        const int *__restrict__ rci,
                                                        const VT *__restrict__ rv,
@@ -653,35 +655,51 @@ __global__ __launch_bounds__(BS) void k_resid_restrict(int nc, const int *__rest
                                                        const VT *__restrict__ f,
                                                        VT *__restrict__ y)
 {
-    const int I = blockIdx.x * BS + threadIdx.x;
-    if (I >= nc) return;
-    VT acc = 0;
-    const int e1 = rrp[I + 1];
-    for (int e = rrp[I]; e < e1; e++) {
-        const int i = rci[e];
-        const VT w = rv[e];
-        const int j1 = arp[i + 1];
-        VT t = 0;
-        constexpr int PF = 8;
-        for (int j0 = arp[i]; j0 < j1; j0 += PF) {
-            int c[PF];
-            VT v[PF], g[PF];
-#pragma unroll
-            for (int u = 0; u < PF; u++)
-                if (j0 + u < j1) {
-                    c[u] = aci[j0 + u];
-                    v[u] = mval<VT>(av[j0 + u]);
-                }
-#pragma unroll
-            for (int u = 0; u < PF; u++)
-                if (j0 + u < j1) g[u] = x[c[u]];
-#pragma unroll
-            for (int u = 0; u < PF; u++)
-                if (j0 + u < j1) t = t + v[u] * g[u];
-        }
-        acc = acc + w * (f[i] - t);
+    const int t = blockIdx.x * BS + threadIdx.x;
+    const int I = t / G, g = t % G;
+    /* whole groups stay in the block (G divides BS), so the shuffles below
+       see every lane of the group */
+    int e0 = 0, e1 = 0;
+    if (I < nc) {
+        e0 = rrp[I];
+        e1 = rrp[I + 1];
     }
-    y[I] = acc;
+    VT acc = 0;
+    for (int b = e0; b < e1; b += G) {
+        const int e = b + g;
+        VT prod = 0;
+        if (e < e1) {
+            const int i = rci[e];
+            const VT w = rv[e];
+            const int j1 = arp[i + 1];
+            VT s = 0;
+            constexpr int PF = 8;
+            for (int j0 = arp[i]; j0 < j1; j0 += PF) {
+                int c[PF];
+                VT v[PF], gx[PF];
+#pragma unroll
+                for (int u = 0; u < PF; u++)
+                    if (j0 + u < j1) {
+                        c[u] = aci[j0 + u];
+                        v[u] = mval<VT>(av[j0 + u]);
+                    }
+#pragma unroll
+                for (int u = 0; u < PF; u++)
+                    if (j0 + u < j1) gx[u] = x[c[u]];
+#pragma unroll
+                for (int u = 0; u < PF; u++)
+                    if (j0 + u < j1) s = s + v[u] * gx[u];
+            }
+            prod = w * (f[i] - s);
+        }
+        /* R's order: products of entries b, b + 1, ... added one by one */
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+            const VT pq = __shfl(prod, (int)(threadIdx.x & 63 & ~(G - 1)) + q, 64);
+            if (b + q < e1) acc = acc + pq;
+        }
+    }
+    if (I < nc && g == 0) y[I] = acc;
 }
 
 } // namespace

@@ -845,9 +845,9 @@ void resid_restrict_typed(hipStream_t s, const DevCsr &A, const MT *av, const De
                           const VT *rv, const VT *x, const VT *f, VT *y)
 {
     const int nc = R.n;
-    k_resid_restrict<MT, VT><<<blocks_for(nc), BS, 0, s>>>(nc, R.row.get(), R.col.get(), rv,
-                                                           A.row.get(), A.col.get(), av, x, f,
-                                                           y);
+    constexpr int G = 4;
+    k_resid_restrict<MT, VT, G><<<blocks_for((long long)nc * G), BS, 0, s>>>(
+        nc, R.row.get(), R.col.get(), rv, A.row.get(), A.col.get(), av, x, f, y);
 }
 
 } // namespace
