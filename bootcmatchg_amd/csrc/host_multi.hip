@@ -98,7 +98,9 @@ MultiWork &ensure_multi_into(Ctx &c, Boot &B, MultiWork &M, int kmax)
         M.scal.alloc((size_t)kmax * 8);
         M.tol.alloc(kmax);
         M.part.alloc(multi_part_size(n, kmax));
-        M.flags.alloc(4 * (size_t)kmax);
+        /* act, par, it, mask; the pad lets kernels read MULTI_MAX_COLS flags
+           from act or mask unconditionally */
+        M.flags.alloc(4 * (size_t)kmax + MULTI_MAX_COLS);
         if (M.h_scal) (void)hipHostFree(M.h_scal);
         if (M.h_tol) (void)hipHostFree(M.h_tol);
         if (M.h_flags) (void)hipHostFree(M.h_flags);
@@ -126,7 +128,8 @@ MultiWork &ensure_multi_into(Ctx &c, Boot &B, MultiWork &M, int kmax)
         F.it = F.act + 2 * kmax;
         M.mask = F.act + 3 * kmax;
         /* no slot active, no stale flags */
-        BCMG_HIP(hipMemsetAsync(M.flags.get(), 0, 4 * (size_t)kmax * sizeof(int), c.stream));
+        BCMG_HIP(hipMemsetAsync(M.flags.get(), 0, (4 * (size_t)kmax + MULTI_MAX_COLS) * sizeof(int),
+                                c.stream));
         std::memset(M.h_flags, 0, 4 * (size_t)kmax * sizeof(int));
         M.prec = -1;
         M.prec64 = false;

@@ -24,6 +24,29 @@ __device__ __forceinline__ bool col_on(int j, int nc, const int *act)
     return j < nc && (act == nullptr || act[j] != 0);
 }
 
+/* The flags of columns [0, nc) as a bit word (bit j: column j active). A
+   flag array holds at least MULTI_MAX_COLS readable entries (its allocation
+   is padded), so the loads are unconditional and wave-uniform: the scalar
+   unit issues them. Kernels whose lanes own different columns test this
+   word instead of loading their flags per lane, which added vector memory
+   instructions to kernels bound by them (a fifth of the interleaved
+   sweep's rate). */
+static_assert(MULTI_MAX_COLS == 32, "column flags fit one 32-bit word");
+__device__ __forceinline__ unsigned col_word(int nc, const int *__restrict__ act)
+{
+    const unsigned live = nc >= 32 ? ~0u : (1u << nc) - 1u;
+    if (act == nullptr) return live;
+    unsigned w = 0;
+#pragma unroll
+    for (int j = 0; j < 32; j++) w |= (act[j] != 0 ? 1u : 0u) << j;
+    return w & live;
+}
+
+__device__ __forceinline__ bool word_on(unsigned w, int j)
+{
+    return j < 32 && ((w >> j) & 1u) != 0;
+}
+
 /* Stream-order row kernel (the arithmetic of k_rows_staged): one row per
    thread and CH columns per block row (chunk blockIdx.y); p = a_ij * x_j
    rounded, then added left to right from 0.
@@ -202,15 +225,13 @@ __global__ __launch_bounds__(BS) void k_scale_multi(int n, VT alpha, VT *__restr
    neighbours, an aggregate's scattered fine rows, a coarse level's long
    rows). Threads are (row, column) pairs, column fastest. */
 
-__device__ __forceinline__ bool il_index(long t, int lkp, int n, int nc, const int *act, int &i,
-                                         int &j)
+__device__ __forceinline__ bool il_index(long t, int n, int nc, unsigned cw, int &i, int &j)
 {
     /* threads cover the nc columns in use of each row (the row stride stays
        2^lkp) */
-    (void)lkp;
     i = (int)(t / nc);
     j = (int)(t % nc);
-    return i < n && col_on(j, nc, act);
+    return i < n && word_on(cw, j);
 }
 
 /* W adjacent columns of one row as a native vector (one 16-byte access) */
@@ -235,13 +256,12 @@ __device__ __forceinline__ VT vget(const vec_t<VT, W> &v, int u)
 }
 
 /* the W columns [jg W, jg W + W) of a lane: any of them active? all? */
-__device__ __forceinline__ void group_on(int jg, int W, int nc, const int *act, bool &any,
-                                         bool &all)
+__device__ __forceinline__ void group_on(int jg, int W, unsigned cw, bool &any, bool &all)
 {
     any = false;
     all = true;
     for (int u = 0; u < W; u++) {
-        const bool on = col_on(jg * W + u, nc, act);
+        const bool on = word_on(cw, jg * W + u);
         any |= on;
         all &= on;
     }
@@ -311,6 +331,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_
     VT *__restrict__ y, int lkp, int nc, const int *__restrict__ act)
 {
     using V = vec_t<VT, W>;
+    const unsigned cw = col_word(nc, act);
     const int gpr = (nc + W - 1) / W; /* lanes per row: the columns in use */
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const long t = (long)b * BS + threadIdx.x;
@@ -318,7 +339,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_
     const int jg = (int)(t % gpr);
     if (r >= n) return;
     bool any, all;
-    group_on(jg, W, nc, act, any, all);
+    group_on(jg, W, cw, any, all);
     if (!any) return;
     const V res = il_row<MT, VT, OP, W, PF>(r, jg, rp, ci, av, x, f, d, lkp);
     const size_t o = ((size_t)r << lkp) + jg * W;
@@ -326,7 +347,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_
         *reinterpret_cast<V *>(y + o) = res;
     } else {
         for (int u = 0; u < W; u++)
-            if (col_on(jg * W + u, nc, act)) y[o + u] = vget<VT, W>(res, u);
+            if (word_on(cw, jg * W + u)) y[o + u] = vget<VT, W>(res, u);
     }
 }
 
@@ -344,13 +365,14 @@ __global__ __launch_bounds__(BS) void k_jacobi_il_c64(
     const VT *__restrict__ x, const VT *__restrict__ f, const VT *__restrict__ d,
     double *__restrict__ z, long ldz, int lkp, int nc, const int *__restrict__ act)
 {
+    const unsigned cw = col_word(nc, act);
     const int gpr = (nc + W - 1) / W;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const long t = (long)b * BS + threadIdx.x;
     const int r = (int)(t / gpr);
     const int jg = (int)(t % gpr);
     bool any = false, all = false;
-    if (r < n) group_on(jg, W, nc, act, any, all);
+    if (r < n) group_on(jg, W, cw, any, all);
     vec_t<VT, W> res{};
     if (any) res = il_row<MT, VT, OP_JACOBI, W, PF>(r, jg, rp, ci, av, x, f, d, lkp);
     if constexpr (!TILE) {
@@ -358,7 +380,7 @@ __global__ __launch_bounds__(BS) void k_jacobi_il_c64(
 #pragma unroll
         for (int u = 0; u < W; u++) {
             const int j = jg * W + u;
-            if (col_on(j, nc, act)) z[(size_t)j * ldz + r] = (double)vget<VT, W>(res, u);
+            if (word_on(cw, j)) z[(size_t)j * ldz + r] = (double)vget<VT, W>(res, u);
         }
     } else {
         /* rows [r0, r0 + rb) of the block; column j of local row q at
@@ -373,7 +395,7 @@ __global__ __launch_bounds__(BS) void k_jacobi_il_c64(
         __syncthreads();
         for (int e = threadIdx.x; e < gpr * W * rb; e += BS) {
             const int j = e / rb, qq = e - j * rb;
-            if (r0 + qq < n && col_on(j, nc, act))
+            if (r0 + qq < n && word_on(cw, j))
                 z[(size_t)j * ldz + r0 + qq] = tile[j * pitch + qq];
         }
     }
@@ -416,18 +438,20 @@ __global__ __launch_bounds__(BS) void k_rows_il_g(int n, int G, const int *__res
 
 #define IL_BEGIN                                                                    \
     int i, j;                                                                       \
-    if (!il_index((long)blockIdx.x * BS + threadIdx.x, lkp, n, nc, act, i, j)) return; \
+    const unsigned cw = col_word(nc, act);                                          \
+    if (!il_index((long)blockIdx.x * BS + threadIdx.x, n, nc, cw, i, j)) return;    \
     const size_t o = ((size_t)i << lkp) + j;
 
 /* lane = (row, W adjacent columns) for the vectorised element-wise kernels */
 #define ILW_BEGIN                                                                   \
     using V = vec_t<VT, W>;                                                         \
+    const unsigned cw = col_word(nc, act);                                          \
     const int gpr = (nc + W - 1) / W;                                               \
     const long t_ = (long)blockIdx.x * BS + threadIdx.x;                            \
     const int i = (int)(t_ / gpr), jg = (int)(t_ % gpr);                            \
     if (i >= n) return;                                                             \
     bool any, all;                                                                  \
-    group_on(jg, W, nc, act, any, all);                                             \
+    group_on(jg, W, cw, any, all);                                                  \
     if (!any) return;                                                               \
     const size_t o = ((size_t)i << lkp) + jg * W;
 
@@ -436,7 +460,7 @@ __global__ __launch_bounds__(BS) void k_rows_il_g(int n, int G, const int *__res
         *reinterpret_cast<V *>((y) + o) = (res);                                    \
     } else {                                                                        \
         for (int u = 0; u < W; u++)                                                 \
-            if (col_on(jg * W + u, nc, act)) (y)[o + u] = vget<VT, W>((res), u);    \
+            if (word_on(cw, jg * W + u)) (y)[o + u] = vget<VT, W>((res), u);        \
     }
 
 template <typename VT, int W>
@@ -514,6 +538,7 @@ __global__ __launch_bounds__(BS) void k_transpose_il(int n, const S *__restrict_
                                                      const int *__restrict__ act)
 {
     __shared__ D tile[TR_ROWS * (MULTI_MAX_COLS + 1)];
+    const unsigned cw = col_word(nc, act);
     const int i0 = blockIdx.x * TR_ROWS;
     const int w = nc + 1;
     for (int q = threadIdx.x; q < TR_ROWS * nc; q += BS) {
@@ -527,7 +552,7 @@ __global__ __launch_bounds__(BS) void k_transpose_il(int n, const S *__restrict_
             j = q % nc;
         }
         const int i = i0 + ii;
-        if (i >= n || !col_on(j, nc, act)) continue;
+        if (i >= n || !word_on(cw, j)) continue;
         if (TO_IL)
             tile[ii * w + j] = (D)a[(size_t)j * ldc + i];
         else
@@ -544,7 +569,7 @@ __global__ __launch_bounds__(BS) void k_transpose_il(int n, const S *__restrict_
             ii = q % TR_ROWS;
         }
         const int i = i0 + ii;
-        if (i >= n || !col_on(j, nc, act)) continue;
+        if (i >= n || !word_on(cw, j)) continue;
         if (TO_IL)
             b[((size_t)i << lkp) + j] = tile[ii * w + j];
         else
@@ -569,10 +594,11 @@ __global__ __launch_bounds__(BS) void k_to_il_v(int n, const double *__restrict_
     using V = vec_t<D, W>;
     __shared__ D tile[TRV * (MULTI_MAX_COLS + 1)];
     constexpr int pitch = MULTI_MAX_COLS + 1;
+    const unsigned cw = col_word(nc, act);
     const int i0 = blockIdx.x * TRV;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int j = wv; j < nc; j += BS / 64) {
-        if (!col_on(j, nc, act)) continue;
+        if (!word_on(cw, j)) continue;
         const int i = i0 + 2 * lane;
         const double *col = a + (size_t)j * lda;
         double v0 = 0.0, v1 = 0.0;
@@ -598,7 +624,7 @@ __global__ __launch_bounds__(BS) void k_to_il_v(int n, const double *__restrict_
 #pragma unroll
         for (int u = 0; u < W; u++) {
             v[u] = tile[ii * pitch + j0 + u];
-            all = all && col_on(j0 + u, nc, act);
+            all = all && word_on(cw, j0 + u);
         }
         D *dst = b + ((size_t)i << lkp) + j0;
         if (all) {
@@ -606,7 +632,7 @@ __global__ __launch_bounds__(BS) void k_to_il_v(int n, const double *__restrict_
         } else {
 #pragma unroll
             for (int u = 0; u < W; u++)
-                if (col_on(j0 + u, nc, act)) dst[u] = v[u];
+                if (word_on(cw, j0 + u)) dst[u] = v[u];
         }
     }
 }
@@ -924,7 +950,10 @@ __global__ __launch_bounds__(RED_FINAL_BS) void k_fcg_final_multi(int nb,
 }
 
 /* FCG update (the arithmetic of k_fcg_update): z = -theta zo + Z into the
-   new pair, x += theta2 z; t = -theta to + t; r += -theta2 t; ||r||^2 */
+   new pair, x += theta2 z; t = -theta to + t; r += -theta2 t; ||r||^2.
+   NT: the columns' ten streams exceed the last-level cache, so they are read
+   and written non-temporally. */
+template <bool NT>
 __global__ __launch_bounds__(BS) void k_fcg_update_multi(int n, FcgCols F,
                                                          double *__restrict__ part)
 {
@@ -943,13 +972,21 @@ __global__ __launch_bounds__(BS) void k_fcg_update_multi(int n, FcgCols F,
     double *r = F.R + (size_t)j * ld;
     double s[1] = {0.0};
     for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
-        const double zi = fma(-theta, zo[i], zc[i]);
-        zn[i] = zi;
-        x[i] = fma(theta2, zi, x[i]);
-        const double ti = fma(-theta, to[i], t[i]);
-        t[i] = ti;
-        const double ri = fma(-theta2, ti, r[i]);
-        r[i] = ri;
+        const double zi = fma(-theta, ldm<NT>(zo + i), ldm<NT>(zc + i));
+        const double xi = fma(theta2, zi, ldm<NT>(x + i));
+        const double ti = fma(-theta, ldm<NT>(to + i), ldm<NT>(t + i));
+        const double ri = fma(-theta2, ti, ldm<NT>(r + i));
+        if (NT) {
+            __builtin_nontemporal_store(zi, zn + i);
+            __builtin_nontemporal_store(xi, x + i);
+            __builtin_nontemporal_store(ti, t + i);
+            __builtin_nontemporal_store(ri, r + i);
+        } else {
+            zn[i] = zi;
+            x[i] = xi;
+            t[i] = ti;
+            r[i] = ri;
+        }
         s[0] = fma(ri, ri, s[0]);
     }
     block_reduce_store<1>(s, part + (size_t)j * gridDim.x + blockIdx.x);
@@ -1530,7 +1567,10 @@ void launch_fcg_update_multi(Ctx &c, int n, MultiFcg &M, double *part)
 {
     const int nb = red_blocks(n);
     const FcgCols F = fcg_cols(M);
-    k_fcg_update_multi<<<dim3(nb, M.nc), BS, 0, c.stream>>>(n, F, part);
+    if ((size_t)n * M.nc * 10 * sizeof(double) > NT_BYTES)
+        k_fcg_update_multi<true><<<dim3(nb, M.nc), BS, 0, c.stream>>>(n, F, part);
+    else
+        k_fcg_update_multi<false><<<dim3(nb, M.nc), BS, 0, c.stream>>>(n, F, part);
     k_fcg_final_multi<1, true><<<M.nc, RED_FINAL_BS, 0, c.stream>>>(nb, part, F, M.act);
     BCMG_CHECK_LAUNCH();
 }

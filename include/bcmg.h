@@ -92,11 +92,12 @@ int bcmg_csr_upload(bcmg_ctx ctx, int n, int nnz, const int *row, const int *col
 int bcmg_csr_download(bcmg_csr A, int *row, int *col, double *val);
 int bcmg_csr_info(bcmg_csr A, int *n, int *nnz);
 /* Bytes per column index that the FP64 row kernels read for A. A square
-   operator of more than 16.7 M entries (an FP64 stream above 192 MB) gets
-   narrower copies of its columns at upload: 16-bit offsets (column minus
-   row) when they all fit, and a stencil pattern when those offsets take at
-   most 8 distinct values (one byte per row: which offsets of a sorted table
-   the row has). The row kernels read the narrowest one that is kept: 0 the
+   operator can get narrower copies of its columns at upload: 16-bit offsets
+   (column minus row) when they all fit and it has more than 16.7 M entries
+   (an FP64 stream above 192 MB), and a stencil pattern when it has more
+   than 11.2 M entries (above 128 MB) and its offsets take at most 8
+   distinct values (one byte per row: which offsets of a sorted table the
+   row has). The row kernels read the narrowest one that is kept: 0 the
    pattern (no per-entry column bytes, one byte per row, and only two row
    pointers per 256 rows), 2 the 16-bit offsets (FP32/BF16 value copies use
    them when their 32-bit-column stream also exceeds 192 MB), else 4.

@@ -2,9 +2,9 @@
 over a sorted table of at most 8 column offsets), the 16-bit offsets and the
 int32 columns give the same bits in every row kernel and in whole solves.
 
-The pattern is kept for square operators above the non-temporal size (an
-FP64 stream of more than 192 MB) whose column offsets take at most 8
-distinct values; the 16-bit copy when those offsets fit 16 bits. A/B runs
+The pattern is kept for square operators whose FP64 stream exceeds 128 MB
+and whose column offsets take at most 8 distinct values; the 16-bit copy
+for those above the non-temporal size (192 MB) whose offsets fit 16 bits. A/B runs
 restrict one matrix handle to a form (bcmg_csr_set_columns); composites are
 set up after the restriction, so their captured launches use it.
 """
