@@ -808,9 +808,12 @@ __global__ void k_fcg_scalars_multi(double *scal, int nc, int first, const int *
    into pair 0; K = 3: t into pair par ^ 1, to from pair par. One row per
    thread, CH columns per block. The grid is one-dimensional: the nch column
    chunks of a row block are consecutive blocks of one XCD, so the matrix
-   rows that the first chunk brings into that XCD's L2 serve the others. */
+   rows that the first chunk brings into that XCD's L2 serve the others.
+   NT: r, the previous direction and t, touched once per launch, are read
+   and written non-temporally (the columns' streams exceed the last-level
+   cache). */
 constexpr int SD_PF = 8; /* entries whose indices are loaded at once */
-template <int K>
+template <int K, bool NT>
 __global__ __launch_bounds__(BS) void k_spmv_dots_multi(int n, const int *__restrict__ rp,
                                                         const int *__restrict__ ci,
                                                         const double *__restrict__ av, FcgCols F,
@@ -886,12 +889,15 @@ __global__ __launch_bounds__(BS) void k_spmv_dots_multi(int n, const int *__rest
             const size_t o = (size_t)j * ld + r;
             const int p = F.par[j];
             const double zr = F.Z[o];
-            const double rr = F.R[o];
+            const double rr = ldm<NT>(F.R + o);
             double *tv = slot(F.W0, F.W1, K == 2 ? 0 : (p ^ 1), j, ld);
-            tv[r] = t[u];
+            if (NT)
+                __builtin_nontemporal_store(t[u], tv + r);
+            else
+                tv[r] = t[u];
             sd[0] = rr * zr;
             sd[1] = t[u] * zr;
-            if constexpr (K > 2) sd[K - 1] = slot(F.W0, F.W1, p, j, ld)[r] * zr;
+            if constexpr (K > 2) sd[K - 1] = ldm<NT>(slot(F.W0, F.W1, p, j, ld) + r) * zr;
         }
         /* the in-wave part of block_reduce_store<K> */
 #pragma unroll
@@ -1551,13 +1557,27 @@ void launch_spmv_dots_multi(Ctx &c, const DevCsr &A, int k, MultiFcg &M, const i
     const int nb = (n + BS - 1) / BS;
     const int nch = (M.nc + CH - 1) / CH;
     const unsigned grid = (unsigned)nb * (unsigned)nch;
+    /* r, the previous direction and t of every column */
+    const bool nt = (size_t)n * M.nc * 3 * sizeof(double) > NT_BYTES;
     if (k == 2) {
-        k_spmv_dots_multi<2><<<grid, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(),
-                                                        A.val.get(), F, M.nc, mask, part);
+        if (nt)
+            k_spmv_dots_multi<2, true><<<grid, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(),
+                                                                  A.val.get(), F, M.nc, mask,
+                                                                  part);
+        else
+            k_spmv_dots_multi<2, false><<<grid, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(),
+                                                                   A.val.get(), F, M.nc, mask,
+                                                                   part);
         k_fcg_final_multi<2, false><<<M.nc, RED_FINAL_BS, 0, c.stream>>>(nb, part, F, mask);
     } else {
-        k_spmv_dots_multi<3><<<grid, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(),
-                                                        A.val.get(), F, M.nc, mask, part);
+        if (nt)
+            k_spmv_dots_multi<3, true><<<grid, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(),
+                                                                  A.val.get(), F, M.nc, mask,
+                                                                  part);
+        else
+            k_spmv_dots_multi<3, false><<<grid, BS, 0, c.stream>>>(n, A.row.get(), A.col.get(),
+                                                                   A.val.get(), F, M.nc, mask,
+                                                                   part);
         k_fcg_final_multi<3, false><<<M.nc, RED_FINAL_BS, 0, c.stream>>>(nb, part, F, mask);
     }
     BCMG_CHECK_LAUNCH();
