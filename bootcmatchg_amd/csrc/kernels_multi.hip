@@ -47,6 +47,15 @@ __device__ __forceinline__ bool word_on(unsigned w, int j)
     return j < 32 && ((w >> j) & 1u) != 0;
 }
 
+template <bool NT, typename T>
+__device__ __forceinline__ void st_nt(T *p, T v)
+{
+    if (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
 /* Stream-order row kernel (the arithmetic of k_rows_staged): one row per
    thread and CH columns per block row (chunk blockIdx.y); p = a_ij * x_j
    rounded, then added left to right from 0.
@@ -277,8 +286,9 @@ __device__ __forceinline__ void group_on(int jg, int W, unsigned cw, bool &any, 
    il_row: the sums of lane (row r, column group jg) and the operator's
    epilogue. The row's own operands come first, so that their latency
    overlaps the gathers; then PF entries at a time: their indices and values,
-   all their gathers in flight, the sums in entry order. */
-template <typename MT, typename VT, int OP, int W, int PF>
+   all their gathers in flight, the sums in entry order. NT: f is read
+   non-temporally (a block beyond the last-level cache). */
+template <typename MT, typename VT, int OP, int W, int PF, bool NT = false>
 __device__ __forceinline__ vec_t<VT, W> il_row(int r, int jg, const int *__restrict__ rp,
                                                const int *__restrict__ ci,
                                                const MT *__restrict__ av,
@@ -293,7 +303,7 @@ __device__ __forceinline__ vec_t<VT, W> il_row(int r, int jg, const int *__restr
     const size_t o = ((size_t)r << lkp) + jg * W;
     V fo = V(0), xo = V(0);
     VT dr = VT(1);
-    if (OP != OP_SPMV) fo = *reinterpret_cast<const V *>(f + o);
+    if (OP != OP_SPMV) fo = ldm<NT>(reinterpret_cast<const V *>(f + o));
     if (OP == OP_JACOBI) {
         xo = *reinterpret_cast<const V *>(x + o);
         dr = d[r];
@@ -323,8 +333,10 @@ __device__ __forceinline__ vec_t<VT, W> il_row(int r, int jg, const int *__restr
 
 /* PF entries of a row are gathered at once. 7 waves per SIMD (at most 72
    VGPRs) hold without spills for every instance launched: PF 8 only with
-   FP32 values and vectors, PF 6 otherwise. */
-template <typename MT, typename VT, int OP, int W, int PF>
+   FP32 values and vectors, PF 6 otherwise. NT: f and y, each touched once,
+   are read and written non-temporally (blocks beyond the last-level cache,
+   where the next launch finds none of y there anyway). */
+template <typename MT, typename VT, int OP, int W, int PF, bool NT>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_rows_il(
     int n, const int *__restrict__ rp, const int *__restrict__ ci, const MT *__restrict__ av,
     const VT *__restrict__ x, const VT *__restrict__ f, const VT *__restrict__ d,
@@ -341,10 +353,13 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_
     bool any, all;
     group_on(jg, W, cw, any, all);
     if (!any) return;
-    const V res = il_row<MT, VT, OP, W, PF>(r, jg, rp, ci, av, x, f, d, lkp);
+    const V res = il_row<MT, VT, OP, W, PF, NT>(r, jg, rp, ci, av, x, f, d, lkp);
     const size_t o = ((size_t)r << lkp) + jg * W;
     if (all) {
-        *reinterpret_cast<V *>(y + o) = res;
+        if (NT)
+            __builtin_nontemporal_store(res, reinterpret_cast<V *>(y + o));
+        else
+            *reinterpret_cast<V *>(y + o) = res;
     } else {
         for (int u = 0; u < W; u++)
             if (word_on(cw, jg * W + u)) y[o + u] = vget<VT, W>(res, u);
@@ -358,8 +373,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(7))) void k_
    divide the block (TILE), the block's rows are whole: the results pass
    through LDS so that each column's rows leave as one contiguous run;
    otherwise every lane stores its own columns. No thread leaves before the
-   barrier. */
-template <typename MT, typename VT, int W, int PF, bool TILE>
+   barrier. NT: f is read and z written non-temporally. */
+template <typename MT, typename VT, int W, int PF, bool TILE, bool NT>
 __global__ __launch_bounds__(BS) void k_jacobi_il_c64(
     int n, const int *__restrict__ rp, const int *__restrict__ ci, const MT *__restrict__ av,
     const VT *__restrict__ x, const VT *__restrict__ f, const VT *__restrict__ d,
@@ -374,13 +389,13 @@ __global__ __launch_bounds__(BS) void k_jacobi_il_c64(
     bool any = false, all = false;
     if (r < n) group_on(jg, W, cw, any, all);
     vec_t<VT, W> res{};
-    if (any) res = il_row<MT, VT, OP_JACOBI, W, PF>(r, jg, rp, ci, av, x, f, d, lkp);
+    if (any) res = il_row<MT, VT, OP_JACOBI, W, PF, NT>(r, jg, rp, ci, av, x, f, d, lkp);
     if constexpr (!TILE) {
         if (!any) return;
 #pragma unroll
         for (int u = 0; u < W; u++) {
             const int j = jg * W + u;
-            if (word_on(cw, j)) z[(size_t)j * ldz + r] = (double)vget<VT, W>(res, u);
+            if (word_on(cw, j)) st_nt<NT>(z + (size_t)j * ldz + r, (double)vget<VT, W>(res, u));
         }
     } else {
         /* rows [r0, r0 + rb) of the block; column j of local row q at
@@ -396,7 +411,7 @@ __global__ __launch_bounds__(BS) void k_jacobi_il_c64(
         for (int e = threadIdx.x; e < gpr * W * rb; e += BS) {
             const int j = e / rb, qq = e - j * rb;
             if (r0 + qq < n && word_on(cw, j))
-                z[(size_t)j * ldz + r0 + qq] = tile[j * pitch + qq];
+                st_nt<NT>(z + (size_t)j * ldz + r0 + qq, tile[j * pitch + qq]);
         }
     }
 }
@@ -1120,19 +1135,26 @@ void rows_il_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT *x, co
         constexpr int WM = 16 / sizeof(VT);
         constexpr bool all32 = std::is_same<MT, float>::value && std::is_same<VT, float>::value;
         const bool pf8 = all32 && A.maxrow > 6 && A.maxrow <= 8;
-        const unsigned nb = (unsigned)blocks_for((long long)n * ((nc + WM - 1) / WM));
+        const bool nt = ((size_t)n << lkp) * sizeof(VT) > NT_BYTES;
+#define ROWS_IL(W, PF)                                                                      \
+    do {                                                                                    \
+        const unsigned g = (unsigned)blocks_for((long long)n * ((nc + (W) - 1) / (W)));     \
+        if (nt)                                                                             \
+            k_rows_il<MT, VT, OP, W, PF, true><<<g, BS, 0, s>>>(n, rp, ci, av, x, f, d, y,  \
+                                                                lkp, nc, act);              \
+        else                                                                                \
+            k_rows_il<MT, VT, OP, W, PF, false><<<g, BS, 0, s>>>(n, rp, ci, av, x, f, d, y, \
+                                                                 lkp, nc, act);             \
+    } while (0)
         if (nc >= WM && pf8)
-            k_rows_il<MT, VT, OP, WM, (all32 ? 8 : 6)><<<nb, BS, 0, s>>>(
-                n, rp, ci, av, x, f, d, y, lkp, nc, act);
+            ROWS_IL(WM, (all32 ? 8 : 6));
         else if (nc >= WM)
-            k_rows_il<MT, VT, OP, WM, 6><<<nb, BS, 0, s>>>(n, rp, ci, av, x, f, d, y, lkp, nc,
-                                                           act);
+            ROWS_IL(WM, 6);
         else if (nc >= 2)
-            k_rows_il<MT, VT, OP, 2, 6><<<blocks_for((long long)n * ((nc + 1) / 2)), BS, 0, s>>>(
-                n, rp, ci, av, x, f, d, y, lkp, nc, act);
+            ROWS_IL(2, 6);
         else
-            k_rows_il<MT, VT, OP, 1, 6><<<blocks_for((long long)n * nc), BS, 0, s>>>(
-                n, rp, ci, av, x, f, d, y, lkp, nc, act);
+            ROWS_IL(1, 6);
+#undef ROWS_IL
     } else {
         const int G = A.group;
         k_rows_il_g<MT, VT, OP><<<blocks_for((long long)n * G), BS, 0, s>>>(
@@ -1154,18 +1176,25 @@ bool jacobi_il_c64_typed(hipStream_t s, const DevCsr &A, const MT *av, const VT 
     const unsigned nb = (unsigned)blocks_for((long long)n * ((nc + WM - 1) / WM));
     const int gpr = (nc + WM - 1) / WM;
     const bool tile = (BS % gpr) == 0;
+    const bool nt = ((size_t)n << lkp) * sizeof(VT) > NT_BYTES;
+#define C64(PF, TILE)                                                                        \
+    do {                                                                                     \
+        if (nt)                                                                              \
+            k_jacobi_il_c64<MT, VT, WM, PF, TILE, true><<<nb, BS, 0, s>>>(                   \
+                n, A.row.get(), A.col.get(), av, x, f, d, z, ldz, lkp, nc, act);             \
+        else                                                                                 \
+            k_jacobi_il_c64<MT, VT, WM, PF, TILE, false><<<nb, BS, 0, s>>>(                  \
+                n, A.row.get(), A.col.get(), av, x, f, d, z, ldz, lkp, nc, act);             \
+    } while (0)
     if (pf8 && tile)
-        k_jacobi_il_c64<MT, VT, WM, (all32 ? 8 : 6), true><<<nb, BS, 0, s>>>(
-            n, A.row.get(), A.col.get(), av, x, f, d, z, ldz, lkp, nc, act);
+        C64((all32 ? 8 : 6), true);
     else if (pf8)
-        k_jacobi_il_c64<MT, VT, WM, (all32 ? 8 : 6), false><<<nb, BS, 0, s>>>(
-            n, A.row.get(), A.col.get(), av, x, f, d, z, ldz, lkp, nc, act);
+        C64((all32 ? 8 : 6), false);
     else if (tile)
-        k_jacobi_il_c64<MT, VT, WM, 6, true><<<nb, BS, 0, s>>>(
-            n, A.row.get(), A.col.get(), av, x, f, d, z, ldz, lkp, nc, act);
+        C64(6, true);
     else
-        k_jacobi_il_c64<MT, VT, WM, 6, false><<<nb, BS, 0, s>>>(
-            n, A.row.get(), A.col.get(), av, x, f, d, z, ldz, lkp, nc, act);
+        C64(6, false);
+#undef C64
     return true;
 }
 
