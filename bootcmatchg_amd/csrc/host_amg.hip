@@ -580,13 +580,22 @@ void cycle(CycleRun<VT> &R, int l, const VT *fin)
     }
     if (prolong_in_sweep) {
         /* the fused sweep writes the other iterate, or the preconditioner's
-           output when that saves the copy after the remaining sweeps */
+           output when it is the last sweep. When the other iterate is that
+           output and sweeps remain, it writes a third level-0 buffer
+           instead (the multiplicative composites leave xb[0] free): the
+           remaining sweeps then alternate between two buffers that are not
+           the output, so the last one writes the output and no copy
+           follows. */
         const int rest = B.prm.postrelax_sweeps - 1;
         VT *fo = top ? R.final_out : nullptr;
         VT *out = s.x2;
-        if (fo && fo != s.x && post_lands_off(rest, s.x2, s.x, fo) &&
-            !post_lands_off(rest, fo, s.x, fo))
-            out = fo;
+        if (fo && fo != s.x) {
+            VT *third = bufs<VT>(B).xb[0].get();
+            if (rest == 0)
+                out = fo;
+            else if (s.x2 == fo && third != s.x && third != fo)
+                out = third;
+        }
         if (launch_jacobi_prolong<VT>(c.stream, A, mt, lv.agg.get(), pvals<VT>(lv), sc.x, s.x, f,
                                       divisor<VT>(lv, B.prm.relax_type), out)) {
             s.x2 = s.x;
