@@ -468,7 +468,7 @@ def main():
                          "captured HIP graphs (used for kernel-trace profiling)")
     ap.add_argument("--no-smoothed", action="store_true",
                     help="skip the smoothed-aggregation measurement")
-    ap.add_argument("--only", choices=("multi", "serving", "smoothed"), default=None,
+    ap.add_argument("--only", choices=("multi", "serving", "smoothed", "small"), default=None,
                     help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -487,6 +487,8 @@ def main():
             res = multi_rhs(bc, ctx, max(1, args.steps // 2))
         elif args.only == "smoothed":
             res = smoothed(bc, ctx, max(1, args.steps // 2))
+        elif args.only == "small":
+            res = small_case(bc, ctx, args.steps)
         else:
             res = serving(bc, ctx, quick=args.steps <= 1)
         print(json.dumps({args.only: res}))
