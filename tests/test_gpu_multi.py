@@ -147,3 +147,17 @@ def test_multi_restarts_match_single(ctx):
     B.set_precision(bc.PREC_FP32)
     _, restarts = check_multi_vs_single(ctx, B, rhs_set(n, 3))
     assert restarts >= 1
+
+
+@pytest.mark.parametrize("prec", [bc.PREC_FP64, bc.PREC_FP32])
+def test_multi_streamed_blocks(ctx, prec):
+    """32 columns on 3D Poisson 100^3: the FCG column blocks exceed the
+    non-temporal size (192 MB), so the FCG SpMV and update stream them
+    non-temporally, and in FP64 so do the level-0 interleaved blocks (256 MB;
+    128 MB in FP32), so the row kernels and the last sweep stream too; every
+    column keeps its single solve's iterations and bits."""
+    A = ops.poisson3d(100)
+    n = len(A[0]) - 1
+    B = bc.Boot(ctx, bc.Csr(ctx, A), bc.make_params(**SETTINGS["V_L1_2sweeps"]))
+    B.set_precision(prec)
+    check_multi_vs_single(ctx, B, rhs_set(n, 32))
